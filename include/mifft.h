@@ -1,0 +1,202 @@
+/*
+ * mifft.h -- C ABI of the MI355X (gfx950) FFT engine.
+ *
+ * This is the drop-in boundary for wgpuFFT's public plan surface
+ * (FftConfig / FftPlan / FftRecorder, src/config.rs, src/plan.rs).  wgpu
+ * objects become HIP objects: a wgpu::Buffer becomes a device pointer plus its
+ * size in bytes, a CommandEncoder / compute pass becomes a hipStream_t, and
+ * "recording" becomes enqueueing on that stream (optionally captured into a
+ * hipGraph).  Every call returns an mifft_status; nothing aborts or panics.
+ *
+ * Layouts are the reference's (README.md:111-128): axis 0 is contiguous,
+ * complex data is interleaved [re, im], df64 is [re_hi, re_lo, im_hi, im_lo],
+ * and R2C output is the packed spectrum [floor(N0/2)+1, N1, ...].
+ */
+#ifndef MIFFT_H
+#define MIFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIFFT_MAX_RANK 8
+#define MIFFT_MAX_FORCED_AXES 8
+
+/* Status codes: one per wgpuFFT FftError variant on this path (src/error.rs:12-180). */
+typedef enum mifft_status {
+    MIFFT_SUCCESS = 0,
+    MIFFT_ERR_ZERO_LENGTH = 1,                 /* FftError::ZeroLength */
+    MIFFT_ERR_ZERO_BATCH = 2,                  /* FftError::ZeroBatch */
+    MIFFT_ERR_LENGTH_TOO_LARGE = 3,            /* FftError::LengthTooLarge */
+    MIFFT_ERR_UNSUPPORTED_LENGTH = 4,          /* FftError::UnsupportedLength */
+    MIFFT_ERR_PRECISION_UNSUPPORTED = 5,       /* FftError::PrecisionUnsupported */
+    MIFFT_ERR_INVALID_TUNING = 6,              /* FftError::InvalidTuning */
+    MIFFT_ERR_EMPTY_AXES = 7,                  /* FftError::EmptyAxes */
+    MIFFT_ERR_INVALID_AXIS = 8,                /* FftError::InvalidAxis */
+    MIFFT_ERR_DUPLICATE_AXIS = 9,              /* FftError::DuplicateAxis */
+    MIFFT_ERR_INVALID_REAL_DIRECTION = 10,     /* FftError::InvalidRealTransformDirection */
+    MIFFT_ERR_UNSUPPORTED_REAL_AXES = 11,      /* FftError::UnsupportedRealAxes */
+    MIFFT_ERR_BUFFER_TOO_SMALL = 12,           /* FftError::BufferViewTooSmall */
+    MIFFT_ERR_WORKSPACE_TOO_SMALL = 13,        /* FftError::WorkspaceTooSmall */
+    MIFFT_ERR_IN_PLACE_UNSUPPORTED = 14,       /* FftError::InPlaceUnsupported */
+    MIFFT_ERR_ALLOCATION_FAILED = 15,          /* FftError::GpuPlanResourceAllocationFailed */
+    MIFFT_ERR_HOST_BUFFER_LENGTH = 16,         /* FftError::HostBufferLength */
+    MIFFT_ERR_NULL_ARGUMENT = 17,              /* no counterpart: a NULL handle or pointer */
+    MIFFT_ERR_INVALID_ENUM = 18,               /* no counterpart: enum value out of range */
+    MIFFT_ERR_RANK_TOO_LARGE = 19,             /* no counterpart: rank > MIFFT_MAX_RANK */
+    MIFFT_ERR_HIP = 20,                        /* a HIP runtime call failed */
+    MIFFT_ERR_RECORDER_STATE = 21,             /* recorder used after end */
+    MIFFT_ERR_OVER_BUDGET = 22,                /* server admission: request exceeds the memory budget */
+    MIFFT_ERR_BUSY = 23                        /* server: no free slot, try again */
+} mifft_status;
+
+/* FftTransformKind (plan.rs:24-29). */
+typedef enum mifft_kind { MIFFT_C2C = 0, MIFFT_R2C = 1, MIFFT_C2R = 2 } mifft_kind;
+/* FftDirection (config.rs:42-47). */
+typedef enum mifft_direction { MIFFT_FORWARD = 0, MIFFT_INVERSE = 1 } mifft_direction;
+/* Normalization (config.rs:59-71); default INVERSE. */
+typedef enum mifft_normalization {
+    MIFFT_NORM_NONE = 0,
+    MIFFT_NORM_FORWARD = 1,
+    MIFFT_NORM_INVERSE = 2,
+    MIFFT_NORM_ORTHOGONAL = 3
+} mifft_normalization;
+/* FftPrecision (config.rs:5-17). */
+typedef enum mifft_precision { MIFFT_F32 = 0, MIFFT_F64 = 1, MIFFT_DF64 = 2 } mifft_precision;
+/* AxisKind (runtime/axis_policy.rs). */
+typedef enum mifft_axis_kind {
+    MIFFT_AXIS_MIXED = 0,
+    MIFFT_AXIS_RADER = 1,
+    MIFFT_AXIS_BLUESTEIN = 2
+} mifft_axis_kind;
+/* C2cRoute (runtime/c2c.rs:7631-7650). */
+typedef enum mifft_route {
+    MIFFT_ROUTE_DIRECT_DFT = 0,
+    MIFFT_ROUTE_MIXED_RADIX = 1,
+    MIFFT_ROUTE_RADER = 2,
+    MIFFT_ROUTE_BLUESTEIN = 3,
+    MIFFT_ROUTE_AXIS_SEQUENCE = 4
+} mifft_route;
+
+/* FftTuning (tuning.rs:35-77); mifft_tuning_init() gives the reference defaults. */
+typedef struct mifft_tuning {
+    uint32_t workgroup_size;               /* 64; power of two */
+    uint32_t fused_workgroup_size;         /* 256; power of two */
+    uint64_t rader_max_prime;              /* 8192; >= 2 */
+    uint64_t direct_max_prime;             /* 127 */
+    int32_t force_rader_axes[MIFFT_MAX_FORCED_AXES];
+    int32_t n_force_rader_axes;
+    int32_t force_bluestein_axes[MIFFT_MAX_FORCED_AXES];
+    int32_t n_force_bluestein_axes;
+    int32_t fuse_long_axes;                /* 1 */
+    int32_t fuse_small_volumes;            /* 1 */
+    uint64_t fused_min_convolution_length; /* 128 */
+} mifft_tuning;
+
+/* FftConfig (config.rs:75-84); mifft_config_init() mirrors FftConfig::new_nd. */
+typedef struct mifft_config {
+    uint64_t shape[MIFFT_MAX_RANK];
+    int32_t rank;
+    int32_t axes[MIFFT_MAX_RANK];
+    int32_t naxes;
+    uint64_t batch;
+    int32_t direction;     /* mifft_direction */
+    int32_t normalization; /* mifft_normalization */
+    int32_t precision;     /* mifft_precision */
+    mifft_tuning tuning;
+} mifft_config;
+
+typedef struct mifft_plan mifft_plan;
+typedef struct mifft_recorder mifft_recorder;
+
+/* ------------------------------------------------------------- config */
+/* FftTuning::default (tuning.rs:55-77). */
+void mifft_tuning_init(mifft_tuning* t);
+/* FftConfig::new_nd (config.rs:90-103): all axes, batch 1, forward, Inverse, f32. */
+mifft_status mifft_config_init(mifft_config* c, int32_t rank, const uint64_t* shape);
+/* FftConfig::validate (config.rs:178-211) with FftTuning::validate_for_config (tuning.rs:280-336). */
+mifft_status mifft_config_validate(const mifft_config* c);
+/* FftConfig::scale_f64 (config.rs:274-286); scale uses prod(shape). */
+mifft_status mifft_config_scale(const mifft_config* c, double* scale);
+/* FftConfig::total_complex_len (config.rs:218-229). */
+mifft_status mifft_config_total_len(const mifft_config* c, uint64_t* len);
+
+/* --------------------------------------------------------------- plans */
+/* FftPlan::c2c / r2c / c2r (plan.rs:221,327,355) on the current HIP device.
+ * Plan creation uploads device tables (twiddles, chirps, filter spectra) and
+ * synchronises; the plan is immutable afterwards. */
+mifft_status mifft_plan_create(const mifft_config* c, mifft_kind kind, mifft_plan** out);
+void mifft_plan_destroy(mifft_plan* p);
+
+/* Queries (plan.rs:390-492, 815-820). */
+mifft_kind mifft_plan_kind(const mifft_plan* p);
+mifft_status mifft_plan_config(const mifft_plan* p, mifft_config* out);
+/* FftPlan::factors: radices of the first transformed axis; returns the count. */
+int32_t mifft_plan_factors(const mifft_plan* p, uint64_t* out, int32_t cap);
+/* FftPlan::axis_factors: radices of transformed axis `axis_index` (in config order). */
+int32_t mifft_plan_axis_factors(const mifft_plan* p, int32_t axis_index, uint64_t* out, int32_t cap);
+/* FftPlan::axis_kinds: one mifft_axis_kind per transformed axis; returns the count. */
+int32_t mifft_plan_axis_kinds(const mifft_plan* p, int32_t* out, int32_t cap);
+mifft_route mifft_plan_route(const mifft_plan* p);
+uint64_t mifft_plan_workspace_bytes(const mifft_plan* p);
+uint64_t mifft_plan_required_input_bytes(const mifft_plan* p);
+uint64_t mifft_plan_required_output_bytes(const mifft_plan* p);
+uint64_t mifft_plan_required_bytes(const mifft_plan* p);
+/* FftPlan::packed_shape: rank entries for R2C/C2R, 0 for C2C. */
+int32_t mifft_plan_packed_shape(const mifft_plan* p, uint64_t* out, int32_t cap);
+int32_t mifft_plan_supports_in_place(const mifft_plan* p);
+/* Number of kernel launches one execution enqueues, and a text description
+ * of them ("reg N=4096 16x16x16 lines=2048; ..."), for diagnostics. */
+int32_t mifft_plan_launch_count(const mifft_plan* p);
+mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap);
+
+/* ------------------------------------------------------------ execution */
+/* FftPlan::execute_checked (plan.rs:510-518): input -> output on `stream`.
+ * The input is left unchanged.  Buffer sizes are validated like
+ * BufferView::validate_min_size (BufferViewTooSmall). */
+mifft_status mifft_execute(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
+                           void* out, uint64_t out_bytes);
+/* FftPlan::execute_with_workspace (plan.rs:535-550): caller-owned workspace,
+ * which lets one plan run concurrently on several streams. */
+mifft_status mifft_execute_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
+                                          uint64_t in_bytes, void* out, uint64_t out_bytes,
+                                          void* workspace, uint64_t workspace_bytes);
+/* FftPlan::execute_in_place (plan.rs:827-835). */
+mifft_status mifft_execute_in_place(const mifft_plan* p, hipStream_t stream, void* buf, uint64_t bytes);
+
+/* ------------------------------------------------------------ recorder */
+/* FftRecorder (plan.rs:38-95): executions recorded into one recorder run in
+ * order on its stream; with capture_graph != 0 they are captured into one
+ * hipGraph, instantiated by mifft_recorder_end and replayed by
+ * mifft_recorder_launch.  Caller work may be enqueued on
+ * mifft_recorder_stream() between records (FftRecorder::compute_pass). */
+mifft_status mifft_recorder_begin(hipStream_t stream, int32_t capture_graph, mifft_recorder** out);
+hipStream_t mifft_recorder_stream(const mifft_recorder* r);
+mifft_status mifft_record(mifft_recorder* r, const mifft_plan* p, const void* in, uint64_t in_bytes,
+                          void* out, uint64_t out_bytes);
+mifft_status mifft_record_in_place(mifft_recorder* r, const mifft_plan* p, void* buf, uint64_t bytes);
+mifft_status mifft_recorder_end(mifft_recorder* r);
+mifft_status mifft_recorder_launch(mifft_recorder* r, hipStream_t stream);
+void mifft_recorder_destroy(mifft_recorder* r);
+
+/* --------------------------------------------------------------- errors */
+const char* mifft_status_string(mifft_status s);
+/* Thread-local detail text of the last failing call on this thread. */
+const char* mifft_last_error_detail(void);
+
+/* --------------------------------------------------------------- misc */
+const char* mifft_version(void);
+/* Synthetic inputs on the device: SplitMix64 uniform[-1,1) and N(0,1) f32
+ * (element k uses SplitMix64 output k of `seed`). */
+mifft_status mifft_fill_uniform_f32(float* dst, uint64_t count, uint64_t seed, hipStream_t stream);
+mifft_status mifft_fill_normal_f32(float* dst, uint64_t count, uint64_t seed, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIFFT_H */

@@ -1,0 +1,94 @@
+"""GPU parity of R2C/C2R plans (packed [N0/2+1, N1, ...] spectrum, axis 0 halved)
+against the FP64 oracle restating reference_r2c_packed_interleaved and
+reference_c2r_from_packed_interleaved (src/math/mod.rs:81-153)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import empty_dev, from_dev, to_dev
+from wgpufft_amd import FftConfig, FftError, FftPlan, FftPrecision
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+REAL_SHAPES = [([4], 1), ([5], 3), ([16], 2), ([120], 4), ([1024], 8), ([2048, 2048], 1),
+               ([64, 48], 2), ([30, 7], 1), ([256, 17, 3], 1), ([4096], 2), ([3, 2], 1)]
+
+
+@pytest.mark.parametrize("shape,batch", REAL_SHAPES)
+def test_r2c(torch_cuda, shape, batch):
+    cnt = int(np.prod(shape)) * batch
+    x = oracle.normal_f32(3, cnt)
+    plan = FftPlan.r2c(FftConfig.new_nd(shape).with_batch(batch))
+    assert plan.packed_shape() == [shape[0] // 2 + 1] + list(shape[1:])
+    d = to_dev(x)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    _sync()
+    y = from_dev(out, np.complex64)
+    ref = oracle.r2c_nd(x.astype(np.float64), shape, batch, nthreads=16)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"r2c {shape} b={batch} rel_l2={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 <= 1e-6 and em <= 5e-6
+
+
+@pytest.mark.parametrize("shape,batch", REAL_SHAPES)
+def test_c2r_non_hermitian_input(torch_cuda, shape, batch):
+    """Random packed input (not Hermitian-consistent): the C2R semantics of the
+    reference (mirror, drop self-conjugate imaginary parts, real part) apply."""
+    pk = int(np.prod(oracle.packed_shape(shape))) * batch
+    p = oracle.uniform_f32(4, 2 * pk).view(np.complex64)
+    plan = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
+    d = to_dev(p)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    _sync()
+    y = from_dev(out, np.float32)
+    ref = oracle.c2r_nd(p.astype(np.complex128), shape, batch, nthreads=16)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"c2r {shape} b={batch} rel_l2={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 <= 1e-6 and em <= 5e-6
+    # the input buffer is left unchanged
+    assert np.array_equal(from_dev(d, np.complex64).view(np.uint32), p.view(np.uint32))
+
+
+def test_r2c_c2r_round_trip_2048(torch_cuda):
+    shape = [2048, 2048]
+    x = oracle.normal_f32(3, 2048 * 2048)
+    r2c = FftPlan.r2c(FftConfig.new_nd(shape))
+    c2r = FftPlan.c2r(FftConfig.inverse_nd(shape))
+    d = to_dev(x)
+    pk = empty_dev(r2c.required_output_buffer_size_bytes())
+    back = empty_dev(c2r.required_output_buffer_size_bytes())
+    r2c.execute(d, pk)
+    c2r.execute(pk, back)
+    _sync()
+    z = from_dev(back, np.float32)
+    e = oracle.rel_l2(z, x)
+    print(f"r2c/c2r 2048^2 round trip rel_l2={e:.3e}", flush=True)
+    assert e <= 1e-6
+
+
+def test_real_in_place(torch_cuda):
+    shape = [64, 10]
+    x = oracle.normal_f32(8, 640)
+    plan = FftPlan.r2c(FftConfig.new_nd(shape))
+    assert not plan.supports_in_place()
+    buf = empty_dev(plan.required_buffer_size_bytes())
+    buf[: x.nbytes].copy_(to_dev(x))
+    plan.execute_in_place(buf)
+    _sync()
+    y = from_dev(buf, np.complex64, 33 * 10)
+    ref = oracle.r2c_nd(x.astype(np.float64), shape)
+    assert oracle.rel_l2(y, ref) <= 1e-6
+
+
+def test_real_precision_unsupported(torch_cuda):
+    with pytest.raises(FftError) as e:
+        FftPlan.r2c(FftConfig.new(16).with_precision(FftPrecision.F64))
+    assert e.value.kind == "PrecisionUnsupported"

@@ -1,0 +1,109 @@
+"""ctypes binding of libmifft.so (declarations mirror include/mifft.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmifft.so")
+
+MAX_RANK = 8
+MAX_FORCED = 8
+
+u32, u64, i32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
+vp, cp = ctypes.c_void_p, ctypes.c_char_p
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("workgroup_size", u32),
+        ("fused_workgroup_size", u32),
+        ("rader_max_prime", u64),
+        ("direct_max_prime", u64),
+        ("force_rader_axes", i32 * MAX_FORCED),
+        ("n_force_rader_axes", i32),
+        ("force_bluestein_axes", i32 * MAX_FORCED),
+        ("n_force_bluestein_axes", i32),
+        ("fuse_long_axes", i32),
+        ("fuse_small_volumes", i32),
+        ("fused_min_convolution_length", u64),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("shape", u64 * MAX_RANK),
+        ("rank", i32),
+        ("axes", i32 * MAX_RANK),
+        ("naxes", i32),
+        ("batch", u64),
+        ("direction", i32),
+        ("normalization", i32),
+        ("precision", i32),
+        ("tuning", Tuning),
+    ]
+
+
+# (name, restype, argtypes) for every exported function of include/mifft.h.
+SIGNATURES = [
+    ("mifft_tuning_init", None, [ctypes.POINTER(Tuning)]),
+    ("mifft_config_init", i32, [ctypes.POINTER(Config), i32, ctypes.POINTER(u64)]),
+    ("mifft_config_validate", i32, [ctypes.POINTER(Config)]),
+    ("mifft_config_scale", i32, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]),
+    ("mifft_config_total_len", i32, [ctypes.POINTER(Config), ctypes.POINTER(u64)]),
+    ("mifft_plan_create", i32, [ctypes.POINTER(Config), i32, ctypes.POINTER(vp)]),
+    ("mifft_plan_destroy", None, [vp]),
+    ("mifft_plan_kind", i32, [vp]),
+    ("mifft_plan_config", i32, [vp, ctypes.POINTER(Config)]),
+    ("mifft_plan_factors", i32, [vp, ctypes.POINTER(u64), i32]),
+    ("mifft_plan_axis_factors", i32, [vp, i32, ctypes.POINTER(u64), i32]),
+    ("mifft_plan_axis_kinds", i32, [vp, ctypes.POINTER(i32), i32]),
+    ("mifft_plan_route", i32, [vp]),
+    ("mifft_plan_workspace_bytes", u64, [vp]),
+    ("mifft_plan_required_input_bytes", u64, [vp]),
+    ("mifft_plan_required_output_bytes", u64, [vp]),
+    ("mifft_plan_required_bytes", u64, [vp]),
+    ("mifft_plan_packed_shape", i32, [vp, ctypes.POINTER(u64), i32]),
+    ("mifft_plan_supports_in_place", i32, [vp]),
+    ("mifft_plan_launch_count", i32, [vp]),
+    ("mifft_plan_describe", i32, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+    ("mifft_execute", i32, [vp, vp, vp, u64, vp, u64]),
+    ("mifft_execute_with_workspace", i32, [vp, vp, vp, u64, vp, u64, vp, u64]),
+    ("mifft_execute_in_place", i32, [vp, vp, vp, u64]),
+    ("mifft_recorder_begin", i32, [vp, i32, ctypes.POINTER(vp)]),
+    ("mifft_recorder_stream", vp, [vp]),
+    ("mifft_record", i32, [vp, vp, vp, u64, vp, u64]),
+    ("mifft_record_in_place", i32, [vp, vp, vp, u64]),
+    ("mifft_recorder_end", i32, [vp]),
+    ("mifft_recorder_launch", i32, [vp, vp]),
+    ("mifft_recorder_destroy", None, [vp]),
+    ("mifft_status_string", cp, [i32]),
+    ("mifft_last_error_detail", cp, []),
+    ("mifft_version", cp, []),
+    ("mifft_fill_uniform_f32", i32, [vp, u64, u64, vp]),
+    ("mifft_fill_normal_f32", i32, [vp, u64, u64, vp]),
+]
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Loads the in-tree libmifft.so; raises LibraryMissing if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            f"{LIB_PATH} is missing: build it with `python -m wgpufft_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

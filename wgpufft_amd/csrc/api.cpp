@@ -1,0 +1,329 @@
+// C ABI (include/mifft.h) over the host runtime.  Every entry point converts
+// internal errors into an mifft_status and a thread-local detail string; no
+// exception crosses the boundary.
+#include <cstring>
+#include <string>
+
+#include "plan.hpp"
+
+using namespace mifft;
+
+struct mifft_plan {
+    std::unique_ptr<Plan> p;
+};
+
+struct mifft_recorder {
+    hipStream_t stream = nullptr;
+    bool capture = false;
+    bool ended = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static thread_local std::string g_detail;
+
+static mifft_status set_err(mifft_status s, const std::string& d) {
+    g_detail = d;
+    return s;
+}
+
+template <class F>
+static mifft_status guard(F&& f) {
+    try {
+        f();
+        return MIFFT_SUCCESS;
+    } catch (const Error& e) {
+        return set_err(e.code, e.detail);
+    } catch (const std::bad_alloc&) {
+        return set_err(MIFFT_ERR_ALLOCATION_FAILED, "host allocation failed");
+    } catch (...) {
+        return set_err(MIFFT_ERR_HIP, "unexpected internal error");
+    }
+}
+
+extern "C" {
+
+void mifft_tuning_init(mifft_tuning* t) {
+    if (!t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->workgroup_size = 64;
+    t->fused_workgroup_size = 256;
+    t->rader_max_prime = 8192;
+    t->direct_max_prime = 127;
+    t->fuse_long_axes = 1;
+    t->fuse_small_volumes = 1;
+    t->fused_min_convolution_length = 128;
+}
+
+mifft_status mifft_config_init(mifft_config* c, int32_t rank, const uint64_t* shape) {
+    if (!c || (rank > 0 && !shape)) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null config or shape");
+    if (rank < 0 || rank > MIFFT_MAX_RANK) return set_err(MIFFT_ERR_RANK_TOO_LARGE, "rank out of range");
+    std::memset(c, 0, sizeof(*c));
+    c->rank = rank;
+    for (int i = 0; i < rank; i++) {
+        c->shape[i] = shape[i];
+        c->axes[i] = i;
+    }
+    c->naxes = rank;
+    c->batch = 1;
+    c->direction = MIFFT_FORWARD;
+    c->normalization = MIFFT_NORM_INVERSE;
+    c->precision = MIFFT_F32;
+    mifft_tuning_init(&c->tuning);
+    return MIFFT_SUCCESS;
+}
+
+mifft_status mifft_config_validate(const mifft_config* c) {
+    if (!c) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null config");
+    return guard([&] { validate_config(*c); });
+}
+
+mifft_status mifft_config_scale(const mifft_config* c, double* scale) {
+    if (!c || !scale) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    return guard([&] {
+        validate_config(*c);
+        *scale = config_scale(*c);
+    });
+}
+
+mifft_status mifft_config_total_len(const mifft_config* c, uint64_t* len) {
+    if (!c || !len) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    return guard([&] {
+        validate_config(*c);
+        *len = config_total(*c);
+    });
+}
+
+mifft_status mifft_plan_create(const mifft_config* c, mifft_kind kind, mifft_plan** out) {
+    if (!c || !out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null config or output handle");
+    *out = nullptr;
+    return guard([&] {
+        auto h = std::make_unique<mifft_plan>();
+        h->p = build_plan(*c, kind);
+        if (h->p->ws_bytes) {
+            hipError_t e = hipMalloc(&h->p->own_ws.p, h->p->ws_bytes);
+            if (e != hipSuccess) fail(MIFFT_ERR_ALLOCATION_FAILED, "plan workspace");
+            h->p->own_ws.bytes = h->p->ws_bytes;
+        }
+        *out = h.release();
+    });
+}
+
+void mifft_plan_destroy(mifft_plan* p) { delete p; }
+
+mifft_kind mifft_plan_kind(const mifft_plan* p) { return p ? p->p->kind : MIFFT_C2C; }
+
+mifft_status mifft_plan_config(const mifft_plan* p, mifft_config* out) {
+    if (!p || !out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    *out = p->p->cfg;
+    return MIFFT_SUCCESS;
+}
+
+int32_t mifft_plan_axis_factors(const mifft_plan* p, int32_t axis_index, uint64_t* out, int32_t cap) {
+    if (!p || axis_index < 0 || axis_index >= (int32_t)p->p->axes.size()) return -1;
+    const auto& f = p->p->axes[axis_index].factors;
+    for (int32_t i = 0; i < (int32_t)f.size() && i < cap; i++) out[i] = f[i];
+    return (int32_t)f.size();
+}
+
+int32_t mifft_plan_factors(const mifft_plan* p, uint64_t* out, int32_t cap) {
+    return mifft_plan_axis_factors(p, 0, out, cap);
+}
+
+int32_t mifft_plan_axis_kinds(const mifft_plan* p, int32_t* out, int32_t cap) {
+    if (!p) return -1;
+    const auto& a = p->p->axes;
+    for (int32_t i = 0; i < (int32_t)a.size() && i < cap; i++) out[i] = a[i].kind;
+    return (int32_t)a.size();
+}
+
+mifft_route mifft_plan_route(const mifft_plan* p) {
+    return p ? (mifft_route)p->p->route : MIFFT_ROUTE_DIRECT_DFT;
+}
+uint64_t mifft_plan_workspace_bytes(const mifft_plan* p) { return p ? p->p->ws_bytes : 0; }
+uint64_t mifft_plan_required_input_bytes(const mifft_plan* p) { return p ? p->p->in_bytes : 0; }
+uint64_t mifft_plan_required_output_bytes(const mifft_plan* p) { return p ? p->p->out_bytes : 0; }
+uint64_t mifft_plan_required_bytes(const mifft_plan* p) {
+    return p ? std::max(p->p->in_bytes, p->p->out_bytes) : 0;
+}
+
+int32_t mifft_plan_packed_shape(const mifft_plan* p, uint64_t* out, int32_t cap) {
+    if (!p || p->p->kind == MIFFT_C2C) return 0;
+    const mifft_config& c = p->p->cfg;
+    for (int32_t i = 0; i < c.rank && i < cap; i++) out[i] = i == 0 ? c.shape[0] / 2 + 1 : c.shape[i];
+    return c.rank;
+}
+
+int32_t mifft_plan_supports_in_place(const mifft_plan* p) { return p && p->p->in_place_kernels ? 1 : 0; }
+
+static int32_t launches(const Plan& p) {
+    int32_t c = 0;
+    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? launches(*op.sub) : 1;
+    return c;
+}
+int32_t mifft_plan_launch_count(const mifft_plan* p) { return p ? launches(*p->p) : 0; }
+
+mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap) {
+    if (!p || !buf || cap == 0) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    std::string d = describe_plan(*p->p);
+    std::strncpy(buf, d.c_str(), cap - 1);
+    buf[cap - 1] = '\0';
+    return MIFFT_SUCCESS;
+}
+
+static void check_sizes(const Plan& P, const void* in, uint64_t in_bytes, const void* out,
+                        uint64_t out_bytes) {
+    if (!in || !out) fail(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    if (in_bytes < P.in_bytes)
+        fail(MIFFT_ERR_BUFFER_TOO_SMALL, "input needs " + std::to_string(P.in_bytes) + " bytes, got " +
+                                             std::to_string(in_bytes));
+    if (out_bytes < P.out_bytes)
+        fail(MIFFT_ERR_BUFFER_TOO_SMALL, "output needs " + std::to_string(P.out_bytes) + " bytes, got " +
+                                             std::to_string(out_bytes));
+}
+
+static void exec_impl(const mifft_plan* p, hipStream_t s, const void* in, uint64_t in_bytes, void* out,
+                      uint64_t out_bytes, void* ws, uint64_t ws_bytes) {
+    if (!p) fail(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    Plan& P = *p->p;
+    check_sizes(P, in, in_bytes, out, out_bytes);
+    if (P.ws_bytes && ws && ws_bytes < P.ws_bytes)
+        fail(MIFFT_ERR_WORKSPACE_TOO_SMALL,
+             "workspace needs " + std::to_string(P.ws_bytes) + " bytes, got " + std::to_string(ws_bytes));
+    if (in == out && !P.in_place_kernels) {
+        // Real plans change the element size: transform from a plan-owned
+        // copy of the input (FftPlan::record_in_place, plan.rs:839-880).
+        std::lock_guard<std::mutex> g(P.ws_mu);
+        if (!P.in_copy.p) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            (void)hipStreamIsCapturing(s, &cs);
+            if (cs != hipStreamCaptureStatusNone)
+                fail(MIFFT_ERR_RECORDER_STATE, "first in-place real execution cannot be graph-captured");
+            hip_check(hipMalloc(&P.in_copy.p, P.in_bytes), "in-place input copy");
+            P.in_copy.bytes = P.in_bytes;
+        }
+        hip_check(hipMemcpyAsync(P.in_copy.p, in, P.in_bytes, hipMemcpyDeviceToDevice, s), "input copy");
+        run_plan(P, s, P.in_copy.p, out, ws ? ws : P.own_ws.p);
+        return;
+    }
+    run_plan(P, s, in, out, ws ? ws : P.own_ws.p);
+}
+
+mifft_status mifft_execute(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
+                           void* out, uint64_t out_bytes) {
+    return guard([&] { exec_impl(p, stream, in, in_bytes, out, out_bytes, nullptr, 0); });
+}
+
+mifft_status mifft_execute_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
+                                          uint64_t in_bytes, void* out, uint64_t out_bytes,
+                                          void* workspace, uint64_t workspace_bytes) {
+    return guard([&] {
+        if (p && p->p->ws_bytes && !workspace) fail(MIFFT_ERR_NULL_ARGUMENT, "null workspace");
+        exec_impl(p, stream, in, in_bytes, out, out_bytes, workspace, workspace_bytes);
+    });
+}
+
+mifft_status mifft_execute_in_place(const mifft_plan* p, hipStream_t stream, void* buf, uint64_t bytes) {
+    return guard([&] { exec_impl(p, stream, buf, bytes, buf, bytes, nullptr, 0); });
+}
+
+mifft_status mifft_recorder_begin(hipStream_t stream, int32_t capture_graph, mifft_recorder** out) {
+    if (!out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null output handle");
+    *out = nullptr;
+    return guard([&] {
+        auto r = std::make_unique<mifft_recorder>();
+        r->stream = stream;
+        r->capture = capture_graph != 0;
+        if (r->capture)
+            hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        *out = r.release();
+    });
+}
+
+hipStream_t mifft_recorder_stream(const mifft_recorder* r) { return r ? r->stream : nullptr; }
+
+mifft_status mifft_record(mifft_recorder* r, const mifft_plan* p, const void* in, uint64_t in_bytes,
+                          void* out, uint64_t out_bytes) {
+    if (!r) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null recorder");
+    if (r->ended) return set_err(MIFFT_ERR_RECORDER_STATE, "recorder already ended");
+    return guard([&] { exec_impl(p, r->stream, in, in_bytes, out, out_bytes, nullptr, 0); });
+}
+
+mifft_status mifft_record_in_place(mifft_recorder* r, const mifft_plan* p, void* buf, uint64_t bytes) {
+    return mifft_record(r, p, buf, bytes, buf, bytes);
+}
+
+mifft_status mifft_recorder_end(mifft_recorder* r) {
+    if (!r) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null recorder");
+    if (r->ended) return set_err(MIFFT_ERR_RECORDER_STATE, "recorder already ended");
+    r->ended = true;
+    if (!r->capture) return MIFFT_SUCCESS;
+    return guard([&] {
+        hip_check(hipStreamEndCapture(r->stream, &r->graph), "end capture");
+        hip_check(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0), "graph instantiate");
+    });
+}
+
+mifft_status mifft_recorder_launch(mifft_recorder* r, hipStream_t stream) {
+    if (!r) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null recorder");
+    if (!r->exec) return set_err(MIFFT_ERR_RECORDER_STATE, "no captured graph");
+    return guard([&] { hip_check(hipGraphLaunch(r->exec, stream), "graph launch"); });
+}
+
+void mifft_recorder_destroy(mifft_recorder* r) {
+    if (!r) return;
+    if (r->capture && !r->ended) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(r->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    if (r->exec) (void)hipGraphExecDestroy(r->exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    delete r;
+}
+
+const char* mifft_status_string(mifft_status s) {
+    switch (s) {
+        case MIFFT_SUCCESS: return "success";
+        case MIFFT_ERR_ZERO_LENGTH: return "zero length";
+        case MIFFT_ERR_ZERO_BATCH: return "zero batch";
+        case MIFFT_ERR_LENGTH_TOO_LARGE: return "length too large";
+        case MIFFT_ERR_UNSUPPORTED_LENGTH: return "unsupported length";
+        case MIFFT_ERR_PRECISION_UNSUPPORTED: return "precision unsupported";
+        case MIFFT_ERR_INVALID_TUNING: return "invalid tuning";
+        case MIFFT_ERR_EMPTY_AXES: return "empty axes";
+        case MIFFT_ERR_INVALID_AXIS: return "invalid axis";
+        case MIFFT_ERR_DUPLICATE_AXIS: return "duplicate axis";
+        case MIFFT_ERR_INVALID_REAL_DIRECTION: return "invalid real transform direction";
+        case MIFFT_ERR_UNSUPPORTED_REAL_AXES: return "unsupported real axes";
+        case MIFFT_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+        case MIFFT_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+        case MIFFT_ERR_IN_PLACE_UNSUPPORTED: return "in-place unsupported";
+        case MIFFT_ERR_ALLOCATION_FAILED: return "allocation failed";
+        case MIFFT_ERR_HOST_BUFFER_LENGTH: return "host buffer length";
+        case MIFFT_ERR_NULL_ARGUMENT: return "null argument";
+        case MIFFT_ERR_INVALID_ENUM: return "invalid enum";
+        case MIFFT_ERR_RANK_TOO_LARGE: return "rank too large";
+        case MIFFT_ERR_HIP: return "HIP runtime error";
+        case MIFFT_ERR_RECORDER_STATE: return "recorder state";
+        case MIFFT_ERR_OVER_BUDGET: return "over memory budget";
+        case MIFFT_ERR_BUSY: return "busy";
+    }
+    return "unknown status";
+}
+
+const char* mifft_last_error_detail(void) { return g_detail.c_str(); }
+
+const char* mifft_version(void) { return "mifft 0.1.0 (gfx950)"; }
+
+mifft_status mifft_fill_uniform_f32(float* dst, uint64_t count, uint64_t seed, hipStream_t stream) {
+    if (!dst) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    return guard([&] { hip_check(fill_uniform_f32(dst, count, seed, stream), "fill"); });
+}
+
+mifft_status mifft_fill_normal_f32(float* dst, uint64_t count, uint64_t seed, hipStream_t stream) {
+    if (!dst) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    return guard([&] { hip_check(fill_normal_f32(dst, count, seed, stream), "fill"); });
+}
+
+}  // extern "C"

@@ -1,0 +1,397 @@
+// Line-FFT kernels for gfx950.
+//
+// Every axis pass of a plan transforms `lines` independent lines of length N.
+// Line g is split as i = g % S1, r = g / S1, j = r % S2, o = r / S2 and its
+// element n lives at   in  + i*in_si  + j*in_sj  + o*in_so  + n*in_sn
+// and its output k at  out + i*out_si + j*out_sj + o*out_so + k*out_sk.
+// A contiguous axis is S1 = 1 (lines N apart); a strided axis of an ND array is
+// S1 = stride (adjacent lines adjacent in memory); the two passes of a
+// four-step split use S2 for the split index.  This replaces the reference's
+// line_base_for_axis (src/runtime/nd_wgsl.rs:1-55, src/math/mod.rs:506-516).
+//
+// Algorithm: Stockham autosort, radix R_s at stage s with L_s = R_0...R_{s-1}:
+// butterfly j (< N/R) reads points j + m*N/R, multiplies point m by
+// W_{L R}^{(j mod L) m}, runs a radix-R DFT and writes point m to
+// (j - j mod L) R + (j mod L) + m L.  Stage 0 reads HBM, the last stage writes
+// HBM, and the stages in between exchange through LDS.
+#pragma once
+#include "fft_math.hpp"
+
+namespace mifft {
+
+enum LoadMode { LD_COMPLEX = 0, LD_REAL = 1, LD_HERMITIAN = 2 };
+enum StoreMode { ST_COMPLEX = 0, ST_PACKED = 1, ST_REAL = 2 };
+
+struct LineArgs {
+    const void* in;
+    void* out;
+    uint32_t lines;
+    uint32_t S1, S2;
+    DivMagic d1, d2;
+    int64_t in_si, in_sj, in_so, in_sn;
+    int64_t out_si, out_sj, out_so, out_sk;
+    const void* tw;           // W_N^t, t < N, arithmetic type
+    const double2* post_hi;   // four-step twiddle W_P^t = hi[t >> post_shift] * lo[t & post_mask]
+    const double2* post_lo;
+    uint32_t post_shift, post_mask;
+    double scale;
+    uint32_t swap_in, swap_out;
+};
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+    // run of logical tiles so neighbouring tiles share its L2.
+    if ((n & 7u) != 0u) return b;
+    return (b & 7u) * (n >> 3) + (b >> 3);
+}
+
+struct LineBase {
+    int64_t in, out;
+    uint32_t j;
+};
+__device__ __forceinline__ LineBase line_base(const LineArgs& a, uint32_t g) {
+    uint32_t r = fdiv(g, a.d1);
+    uint32_t i = g - r * a.S1;
+    uint32_t o = fdiv(r, a.d2);
+    uint32_t j = r - o * a.S2;
+    LineBase b;
+    b.in = (int64_t)i * a.in_si + (int64_t)j * a.in_sj + (int64_t)o * a.in_so;
+    b.out = (int64_t)i * a.out_si + (int64_t)j * a.out_sj + (int64_t)o * a.out_so;
+    b.j = j;
+    return b;
+}
+
+// W_P^t rounded once from an FP64 product of two FP64 table entries.
+template <class T>
+__device__ __forceinline__ cpx<T> post_twiddle(const LineArgs& a, uint64_t t) {
+    double2 h = a.post_hi[t >> a.post_shift];
+    double2 l = a.post_lo[t & a.post_mask];
+    double re = h.x * l.x - h.y * l.y;
+    double im = h.x * l.y + h.y * l.x;
+    return mk<T>((T)re, (T)im);
+}
+
+template <class Fmt, int LDM>
+__device__ __forceinline__ cpx<typename Fmt::T> load_point(const LineArgs& a, int64_t base,
+                                                           uint32_t p, uint32_t n) {
+    using T = typename Fmt::T;
+    cpx<T> v;
+    if constexpr (LDM == LD_COMPLEX) {
+        v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)p * a.in_sn));
+    } else if constexpr (LDM == LD_REAL) {
+        v = mk<T>(Fmt::ld_real(a.in, (uint64_t)(base + (int64_t)p * a.in_sn)), T(0));
+    } else {
+        // Hermitian extension of a packed half line (n/2 + 1 points): the
+        // imaginary part of the self-conjugate bins 0 and n/2 is dropped and the
+        // upper half mirrors the conjugate (src/math/mod.rs:470-502).
+        uint32_t h = n >> 1;
+        if (p <= h) {
+            v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)p * a.in_sn));
+            if (p == 0 || 2 * p == n) v.y = T(0);
+        } else {
+            v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)(n - p) * a.in_sn));
+            v.y = -v.y;
+        }
+    }
+    if (a.swap_in) v = swap_ri(v);
+    return v;
+}
+
+template <class Fmt, int STM, bool POST>
+__device__ __forceinline__ void store_point(const LineArgs& a, int64_t base, uint32_t jline,
+                                            uint32_t k, uint32_t n, cpx<typename Fmt::T> v) {
+    using T = typename Fmt::T;
+    if constexpr (POST) v = cmul(v, post_twiddle<T>(a, (uint64_t)jline * k));
+    if (a.swap_out) v = swap_ri(v);
+    const T s = (T)a.scale;
+    if constexpr (STM == ST_COMPLEX) {
+        Fmt::st(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), scale(v, s));
+    } else if constexpr (STM == ST_PACKED) {
+        if (k <= (n >> 1)) Fmt::st(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), scale(v, s));
+    } else {
+        Fmt::st_real(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), v.x * s);
+    }
+}
+
+// ------------------------------------------------- register-resident kernel
+template <int... Rs>
+struct RadixList {
+    static constexpr int n = sizeof...(Rs);
+    static constexpr int r[n] = {Rs...};
+    static constexpr int at(int s) { return r[s]; }
+    static constexpr int lprod(int s) {
+        int p = 1;
+        for (int i = 0; i < s; i++) p *= r[i];
+        return p;
+    }
+    static constexpr int prod() { return lprod(n); }
+};
+
+constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
+constexpr int lds_pad(int p) { return p + (p >> 4); }
+
+// Template parameters:
+//   N, TPL (threads per line), LINES (lines per workgroup);
+//   LD_LF / ST_LF: stage-0 loads / last-stage stores map adjacent lanes to
+//   adjacent lines (strided axes) instead of adjacent points (contiguous axes);
+//   LDM / STM: load and store modes; POST: four-step twiddle on store.
+template <class Fmt, int N, int TPL, int LINES, bool LD_LF, bool ST_LF, int LDM, int STM, bool POST,
+          class RL>
+struct RegFft {
+    using T = typename Fmt::T;
+    static constexpr int NS = RL::n;
+    static constexpr int NT = TPL * LINES;
+    static constexpr int LSTR = lds_pad(N) + 1;
+    static constexpr size_t lds_bytes() {
+        return NS > 1 ? (size_t)LINES * LSTR * sizeof(cpx<T>) : 0;
+    }
+    static constexpr int max_regs(int s) {
+        return s >= NS ? 0
+                       : (ceil_div(N / RL::at(s), TPL) * RL::at(s) > max_regs(s + 1)
+                              ? ceil_div(N / RL::at(s), TPL) * RL::at(s)
+                              : max_regs(s + 1));
+    }
+    static constexpr int EMAX = max_regs(0);
+
+    template <int s>
+    static __device__ __forceinline__ void stage(const LineArgs& a, cpx<T>* lds, uint32_t g0,
+                                                 cpx<T> (&w)[EMAX]) {
+        if constexpr (s < NS) {
+            constexpr int R = RL::at(s);
+            constexpr int L = RL::lprod(s);
+            constexpr int NR = N / R;
+            constexpr int NB = ceil_div(NR, TPL);
+            constexpr bool FIRST = (s == 0), LAST = (s == NS - 1);
+            constexpr bool LF = FIRST ? LD_LF : (LAST ? ST_LF : false);
+            const uint32_t tid = threadIdx.x;
+            const uint32_t l = LF ? (tid % LINES) : (tid / TPL);
+            const uint32_t t = LF ? (tid / LINES) : (tid % TPL);
+            const uint32_t g = g0 + l;
+            const bool valid = g < a.lines;
+            // gather
+            if constexpr (FIRST) {
+                LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    const uint32_t j = t + q * TPL;
+#pragma unroll
+                    for (int m = 0; m < R; m++) {
+                        if (valid && (NR % TPL == 0 || j < (uint32_t)NR))
+                            w[q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
+                        else
+                            w[q * R + m] = mk<T>(T(0), T(0));
+                    }
+                }
+            } else {
+                const cpx<T>* ln = lds + l * LSTR;
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    const uint32_t j = t + q * TPL;
+                    if (NR % TPL == 0 || j < (uint32_t)NR) {
+#pragma unroll
+                        for (int m = 0; m < R; m++) w[q * R + m] = ln[lds_pad(j + m * NR)];
+                    }
+                }
+                if constexpr (!LAST) __syncthreads();
+            }
+            // twiddle + butterfly
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const uint32_t j = t + q * TPL;
+                if (NR % TPL == 0 || j < (uint32_t)NR) {
+                    if constexpr (L > 1) {
+                        const uint32_t k = j % L;
+                        constexpr uint32_t step = N / (L * R);
+#pragma unroll
+                        for (int m = 1; m < R; m++)
+                            w[q * R + m] = cmul(w[q * R + m], ld_tw<T>(a.tw, k * m * step));
+                    }
+                    Dft<R, T>::run(&w[q * R]);
+                }
+            }
+            // scatter
+            if constexpr (LAST) {
+                LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    const uint32_t j = t + q * TPL;
+                    if (valid && (NR % TPL == 0 || j < (uint32_t)NR)) {
+                        const uint32_t k = j % L;
+#pragma unroll
+                        for (int m = 0; m < R; m++)
+                            store_point<Fmt, STM, POST>(a, b.out, b.j, (j - k) * R + k + m * L, N,
+                                                        w[q * R + m]);
+                    }
+                }
+            } else {
+                cpx<T>* ln = lds + l * LSTR;
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    const uint32_t j = t + q * TPL;
+                    if (NR % TPL == 0 || j < (uint32_t)NR) {
+                        const uint32_t k = j % L;
+#pragma unroll
+                        for (int m = 0; m < R; m++)
+                            ln[lds_pad((j - k) * R + k + m * L)] = w[q * R + m];
+                    }
+                }
+                __syncthreads();
+                stage<s + 1>(a, lds, g0, w);
+            }
+        }
+    }
+
+    static __device__ __forceinline__ void run(const LineArgs& a, cpx<T>* lds) {
+        const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+        cpx<T> w[EMAX];
+        stage<0>(a, lds, bid * LINES, w);
+    }
+};
+
+template <class K>
+__global__ void __launch_bounds__(K::NT) reg_fft_kernel(LineArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    K::run(a, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
+}
+
+// ------------------------------------------------ runtime-schedule LDS kernel
+constexpr int kMaxStages = 24;
+
+struct Sched {
+    uint32_t n;    // line length
+    uint32_t ns;   // stages
+    uint32_t lpw;  // lines per workgroup
+    uint32_t ld_lf, st_lf;
+    DivMagic dn;    // divide by n
+    DivMagic dlpw;  // divide by lpw
+    uint8_t rad[kMaxStages];
+    uint32_t L[kMaxStages];
+    DivMagic dnr[kMaxStages];  // divide by n / rad
+    DivMagic dl[kMaxStages];   // divide by L
+};
+
+// LDS line stride (elements) of the runtime-schedule kernels.
+__host__ __device__ constexpr uint32_t lds_line_stride(uint32_t n) { return n + (n >> 4) + 1; }
+__device__ __forceinline__ uint32_t lpad(uint32_t p) { return p + (p >> 4); }
+
+// One Stockham stage over `nl` lines resident in LDS: src -> dst.
+template <int R, class T>
+__device__ __forceinline__ void lds_stage(const cpx<T>* src, cpx<T>* dst, uint32_t nl,
+                                          uint32_t n, uint32_t lstr, uint32_t L, DivMagic dnr,
+                                          DivMagic dl, const void* tw) {
+    const uint32_t nr = n / R;
+    const uint32_t total = nl * nr;
+    const uint32_t step = n / (L * R);
+    for (uint32_t b = threadIdx.x; b < total; b += blockDim.x) {
+        const uint32_t l = fdiv(b, dnr);
+        const uint32_t j = b - l * nr;
+        const uint32_t kq = fdiv(j, dl);
+        const uint32_t k = j - kq * L;
+        const cpx<T>* s = src + l * lstr;
+        cpx<T> w[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) w[m] = s[lpad(j + m * nr)];
+        if (L > 1) {
+#pragma unroll
+            for (int m = 1; m < R; m++) w[m] = cmul(w[m], ld_tw<T>(tw, k * m * step));
+        }
+        Dft<R, T>::run(w);
+        cpx<T>* d = dst + l * lstr;
+        const uint32_t ob = kq * L * R + k;
+#pragma unroll
+        for (int m = 0; m < R; m++) d[lpad(ob + m * L)] = w[m];
+    }
+}
+
+// Transforms nl lines held in LDS buffer `a` (line stride lstr) using `b` as
+// ping-pong space; returns the buffer holding the result.  Every thread of the
+// block must call it (it synchronises after each stage).
+template <class T>
+__device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched& sc,
+                                 const void* tw) {
+    const uint32_t lstr = lds_line_stride(sc.n);
+    cpx<T>* src = a;
+    cpx<T>* dst = b;
+    for (uint32_t s = 0; s < sc.ns; s++) {
+        const uint32_t R = sc.rad[s];
+        switch (R) {
+#define MIFFT_CASE(RR)                                                                     \
+    case RR:                                                                               \
+        lds_stage<RR, T>(src, dst, nl, sc.n, lstr, sc.L[s], sc.dnr[s], sc.dl[s], tw); \
+        break;
+            MIFFT_CASE(2)
+            MIFFT_CASE(3)
+            MIFFT_CASE(4)
+            MIFFT_CASE(5)
+            MIFFT_CASE(6)
+            MIFFT_CASE(7)
+            MIFFT_CASE(8)
+            MIFFT_CASE(9)
+            MIFFT_CASE(10)
+            MIFFT_CASE(11)
+            MIFFT_CASE(12)
+            MIFFT_CASE(13)
+            MIFFT_CASE(14)
+            MIFFT_CASE(15)
+            MIFFT_CASE(16)
+#undef MIFFT_CASE
+            default:
+                break;
+        }
+        __syncthreads();
+        cpx<T>* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+template <class Fmt, int LDM, int STM, bool POST>
+__global__ void __launch_bounds__(256) lds_fft_kernel(LineArgs a, Sched sc) {
+    using T = typename Fmt::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const uint32_t lstr = lds_line_stride(sc.n);
+    cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* bufB = bufA + sc.lpw * lstr;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t g0 = bid * sc.lpw;
+    const uint32_t total = sc.lpw * sc.n;
+    // load
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+        uint32_t l, p;
+        if (sc.ld_lf) {
+            p = fdiv(e, sc.dlpw);
+            l = e - p * sc.lpw;
+        } else {
+            l = fdiv(e, sc.dn);
+            p = e - l * sc.n;
+        }
+        const uint32_t g = g0 + l;
+        cpx<T> v = mk<T>(T(0), T(0));
+        if (g < a.lines) {
+            LineBase b = line_base(a, g);
+            v = load_point<Fmt, LDM>(a, b.in, p, sc.n);
+        }
+        bufA[l * lstr + lpad(p)] = v;
+    }
+    __syncthreads();
+    cpx<T>* res = lds_fft_lines<T>(bufA, bufB, sc.lpw, sc, a.tw);
+    // store
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+        uint32_t l, p;
+        if (sc.st_lf) {
+            p = fdiv(e, sc.dlpw);
+            l = e - p * sc.lpw;
+        } else {
+            l = fdiv(e, sc.dn);
+            p = e - l * sc.n;
+        }
+        const uint32_t g = g0 + l;
+        if (g < a.lines) {
+            LineBase b = line_base(a, g);
+            store_point<Fmt, STM, POST>(a, b.out, b.j, p, sc.n, res[l * lstr + lpad(p)]);
+        }
+    }
+}
+
+}  // namespace mifft

@@ -1,0 +1,134 @@
+// Compiled line-kernel variants and their launchers.
+//
+// The registry is the gfx950 counterpart of the reference's pipeline cache
+// (src/runtime/pipeline_cache.rs:356-616): instead of generating WGSL per
+// typed key at plan time, every register-resident variant is instantiated here
+// at build time and looked up by (precision, N, layout, load/store mode).
+// Lengths without a variant run the runtime-schedule LDS kernel.
+#include <mutex>
+#include <unordered_set>
+
+#include "launch.h"
+
+namespace mifft {
+
+void ensure_lds_attr(const void* fn) {
+    static std::mutex mu;
+    static std::unordered_set<const void*> done;
+    std::lock_guard<std::mutex> g(mu);
+    if (done.count(fn)) return;
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    done.insert(fn);
+}
+
+#define MIFFT_RL(...) RadixList<__VA_ARGS__>
+#define REGV(FMT, PREC, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, RADSTR, ...)                  \
+    {PREC,                                                                                      \
+     N,                                                                                         \
+     TPL,                                                                                       \
+     LINES,                                                                                     \
+     LDLF,                                                                                      \
+     STLF,                                                                                      \
+     LDM,                                                                                       \
+     STM,                                                                                       \
+     POST,                                                                                      \
+     TPL * LINES,                                                                               \
+     RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__)>::lds_bytes(), \
+     (const void*)&reg_fft_kernel<                                                              \
+         RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__)>>,        \
+     RADSTR}
+
+// f32 power-of-two lines: 16 points per thread, radix-16 stages.
+#define F32_POW2(LDLF, STLF, LDM, STM, POST, L4096, L2048, L1024, L512, L256, L128, L64)          \
+    REGV(FmtF32, P_F32, 4096, 256, L4096, LDLF, STLF, LDM, STM, POST, "16x16x16", 16, 16, 16),    \
+        REGV(FmtF32, P_F32, 2048, 128, L2048, LDLF, STLF, LDM, STM, POST, "16x16x8", 16, 16, 8),  \
+        REGV(FmtF32, P_F32, 1024, 64, L1024, LDLF, STLF, LDM, STM, POST, "16x16x4", 16, 16, 4),   \
+        REGV(FmtF32, P_F32, 512, 32, L512, LDLF, STLF, LDM, STM, POST, "8x8x8", 8, 8, 8),         \
+        REGV(FmtF32, P_F32, 256, 16, L256, LDLF, STLF, LDM, STM, POST, "16x16", 16, 16),          \
+        REGV(FmtF32, P_F32, 128, 8, L128, LDLF, STLF, LDM, STM, POST, "16x8", 16, 8),             \
+        REGV(FmtF32, P_F32, 64, 4, L64, LDLF, STLF, LDM, STM, POST, "16x4", 16, 4)
+
+static const RegVariant kRegistry[] = {
+    // contiguous axes (axis 0)
+    F32_POW2(0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, 4, 8, 16, 32, 64),
+    // strided axes: adjacent lanes on adjacent lines, >= 128 B per row segment
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 0, 4, 8, 16, 16, 16, 32, 64),
+    // R2C axis 0: real in, packed half spectrum out
+    F32_POW2(0, 0, LD_REAL, ST_PACKED, 0, 1, 2, 4, 8, 16, 32, 64),
+    // C2R axis 0: Hermitian-extended packed in, real out
+    F32_POW2(0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, 4, 8, 16, 32, 64),
+    // four-step split passes: columns with twiddle on store; rows with transposed store
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 4, 8, 16, 16, 16, 32, 64),
+    F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 4, 8, 16, 16, 16, 32, 64),
+    // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
+    REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "15x8", 15, 8),
+    REGV(FmtF32, P_F32, 343, 49, 5, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "7x7x7", 7, 7, 7),
+    REGV(FmtF32, P_F32, 1001, 77, 3, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "13x11x7", 13, 11, 7),
+    // f64 / df64 256-point lines (256^3 volumes)
+    REGV(FmtF64, P_F64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
+};
+
+int reg_variant_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }
+const RegVariant* reg_variant_at(int i) {
+    return (i >= 0 && i < reg_variant_count()) ? &kRegistry[i] : nullptr;
+}
+
+const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
+                                   int post) {
+    for (const RegVariant& v : kRegistry) {
+        if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm &&
+            v.stm == stm && v.post == post)
+            return &v;
+    }
+    return nullptr;
+}
+
+hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s) {
+    if (v->lds_bytes > 64 * 1024) ensure_lds_attr(v->fn);
+    const uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
+    LineArgs copy = a;
+    void* args[] = {&copy};
+    return hipLaunchKernel(v->fn, dim3(blocks), dim3(v->nthreads), args, v->lds_bytes, s);
+}
+
+size_t lds_kernel_bytes(int prec, uint32_t n, uint32_t lpw) {
+    const size_t es = prec == P_F32 ? 8 : 16;
+    return 2 * (size_t)lpw * lds_line_stride(n) * es;
+}
+
+template <class Fmt>
+static hipError_t launch_lds_fmt(int ldm, int stm, int post, const LineArgs& a, const Sched& sc,
+                                 hipStream_t s) {
+    const void* fn = nullptr;
+    if (ldm == LD_COMPLEX && stm == ST_COMPLEX && !post)
+        fn = (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, false>;
+    else if (ldm == LD_COMPLEX && stm == ST_COMPLEX && post)
+        fn = (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, true>;
+    else if (ldm == LD_REAL && stm == ST_PACKED && !post)
+        fn = (const void*)&lds_fft_kernel<Fmt, LD_REAL, ST_PACKED, false>;
+    else if (ldm == LD_HERMITIAN && stm == ST_REAL && !post)
+        fn = (const void*)&lds_fft_kernel<Fmt, LD_HERMITIAN, ST_REAL, false>;
+    else
+        return hipErrorInvalidValue;
+    const size_t bytes = lds_kernel_bytes(Fmt::kId == 0 ? P_F32 : P_F64, sc.n, sc.lpw);
+    if (bytes > 64 * 1024) ensure_lds_attr(fn);
+    const uint32_t blocks = (a.lines + sc.lpw - 1) / sc.lpw;
+    LineArgs ca = a;
+    Sched cs = sc;
+    void* args[] = {&ca, &cs};
+    return hipLaunchKernel(fn, dim3(blocks), dim3(256), args, bytes, s);
+}
+
+hipError_t launch_lds(int prec, int ldm, int stm, int post, const LineArgs& a, const Sched& sc,
+                      hipStream_t s) {
+    switch (prec) {
+        case P_F32: return launch_lds_fmt<FmtF32>(ldm, stm, post, a, sc, s);
+        case P_F64: return launch_lds_fmt<FmtF64>(ldm, stm, post, a, sc, s);
+        default: return launch_lds_fmt<FmtDF64>(ldm, stm, post, a, sc, s);
+    }
+}
+
+}  // namespace mifft

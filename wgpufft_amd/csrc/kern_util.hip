@@ -1,0 +1,301 @@
+// Table generation and elementwise kernels.
+//
+// Twiddles, Bluestein chirps and the Rader/Bluestein filter sequences are
+// generated in FP64 on the device with the phase reduced exactly in integers
+// (t*4 mod n for the quadrant, n^2 mod 2N for the chirp, g^-t mod p for Rader),
+// evaluated with the ROCm device-library sincospi, and rounded once to the
+// table type.  The reference builds the same values on the host in f64
+// (src/runtime/twiddle.rs:198-217, src/runtime/bluestein_axis.rs:818-867,
+// src/runtime/rader_axis.rs:1318-1336).
+#include "launch.h"
+
+namespace mifft {
+
+// W_n^t = exp(-2*pi*i*t/n) in FP64: exact quadrant reduction, then
+// sincospi on r/(2n) with r < n.
+__device__ double2 root_f64(uint64_t t, uint64_t n) {
+    t %= n;
+    const unsigned __int128 t4 = (unsigned __int128)t * 4u;
+    const uint64_t q = (uint64_t)(t4 / n);
+    const uint64_t r = (uint64_t)(t4 - (unsigned __int128)q * n);
+    const double x = (double)r / (2.0 * (double)n);
+    double s, c;
+    sincospi(x, &s, &c);
+    double ct, st;
+    switch (q & 3u) {
+        case 0: ct = c; st = s; break;
+        case 1: ct = -s; st = c; break;
+        case 2: ct = -c; st = -s; break;
+        default: ct = s; st = -c; break;
+    }
+    return make_double2(ct, -st);
+}
+
+__device__ __forceinline__ uint64_t mod_mul(uint64_t a, uint64_t b, uint64_t m) {
+    return (uint64_t)(((unsigned __int128)a * b) % m);
+}
+__device__ uint64_t mod_pow_dev(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    b %= m;
+    while (e) {
+        if (e & 1) r = mod_mul(r, b, m);
+        b = mod_mul(b, b, m);
+        e >>= 1;
+    }
+    return r;
+}
+
+#define GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+template <class T>
+__global__ void k_gen_tw(cpx<T>* out, uint64_t n, uint64_t count) {
+    GRID_STRIDE(i, count) {
+        double2 w = root_f64(i, n);
+        out[i] = mk<T>((T)w.x, (T)w.y);
+    }
+}
+
+__global__ void k_gen_tw_f64_strided(double2* out, uint64_t n, uint64_t count, uint64_t mult) {
+    GRID_STRIDE(i, count) { out[i] = root_f64(mod_mul(i, mult, n), n); }
+}
+
+// Forward chirp c[i] = exp(-pi*i*i^2/n) = W_{2n}^(i^2 mod 2n).
+template <class T>
+__global__ void k_gen_chirp(cpx<T>* out, uint64_t n) {
+    GRID_STRIDE(i, n) {
+        const uint64_t ph = mod_mul(i, i, 2 * n);
+        double2 w = root_f64(ph, 2 * n);
+        out[i] = mk<T>((T)w.x, (T)w.y);
+    }
+}
+
+// Rader filter sequence over the convolution length m: b[t] = W_p^(ginv^t mod p)
+// for t < p-1; when m > p-1 the cyclic sequence is embedded with b[m-t] = b[p-1-t].
+__global__ void k_rader_seq(double2* out, uint64_t p, uint64_t m, uint64_t ginv) {
+    const uint64_t l = p - 1;
+    GRID_STRIDE(idx, m) {
+        int64_t t = -1;
+        if (idx < l) t = (int64_t)idx;
+        else if (m != l && idx > m - l) t = (int64_t)(l - (m - idx));
+        if (t < 0) {
+            out[idx] = make_double2(0.0, 0.0);
+        } else {
+            out[idx] = root_f64(mod_pow_dev(ginv, (uint64_t)t, p), p);
+        }
+    }
+}
+
+// Bluestein filter sequence: b[i] = conj(c[i]) for |i| < n, wrapped mod m.
+__global__ void k_bluestein_seq(double2* out, uint64_t n, uint64_t m) {
+    GRID_STRIDE(idx, m) {
+        int64_t t = -1;
+        if (idx < n) t = (int64_t)idx;
+        else if (idx > m - n) t = (int64_t)(m - idx);
+        if (t < 0) {
+            out[idx] = make_double2(0.0, 0.0);
+        } else {
+            const uint64_t ph = mod_mul((uint64_t)t, (uint64_t)t, 2 * n);
+            double2 w = root_f64(ph, 2 * n);
+            out[idx] = make_double2(w.x, -w.y);
+        }
+    }
+}
+
+template <class T>
+__global__ void k_round_table(cpx<T>* out, const double2* in, uint64_t count, double mul) {
+    GRID_STRIDE(i, count) {
+        double2 v = in[i];
+        out[i] = mk<T>((T)(v.x * mul), (T)(v.y * mul));
+    }
+}
+
+// ----------------------------------------------- multi-pass Bluestein steps
+// Workspace layout: line l of the convolution at ws[l*m + k], plan format.
+template <class Fmt>
+__global__ void k_chirp_pre(LineArgs a, const void* chirp, uint32_t n, uint32_t m) {
+    using T = typename Fmt::T;
+    const uint64_t total = (uint64_t)a.lines * m;
+    GRID_STRIDE(e, total) {
+        const uint32_t l = (uint32_t)(e / m);
+        const uint32_t p = (uint32_t)(e - (uint64_t)l * m);
+        cpx<T> v = mk<T>(T(0), T(0));
+        if (p < n) {
+            LineBase b = line_base(a, l);
+            v = load_point<Fmt, LD_COMPLEX>(a, b.in, p, n);
+            v = cmul(v, ld_tw<T>(chirp, p));
+        }
+        Fmt::st(a.out, e, v);
+    }
+}
+
+template <class Fmt>
+__global__ void k_pointwise(void* buf, const void* filt, uint64_t m, uint64_t total) {
+    using T = typename Fmt::T;
+    GRID_STRIDE(e, total) {
+        const uint64_t k = e % m;
+        cpx<T> v = Fmt::ld(buf, e);
+        Fmt::st(buf, e, swap_ri(cmul(v, ld_tw<T>(filt, (uint32_t)k))));
+    }
+}
+
+template <class Fmt>
+__global__ void k_chirp_post(LineArgs a, const void* chirp, uint32_t n, uint32_t m) {
+    using T = typename Fmt::T;
+    const uint64_t total = (uint64_t)a.lines * n;
+    GRID_STRIDE(e, total) {
+        const uint32_t l = (uint32_t)(e / n);
+        const uint32_t k = (uint32_t)(e - (uint64_t)l * n);
+        cpx<T> v = swap_ri(Fmt::ld(a.in, (uint64_t)l * m + k));
+        v = cmul(v, ld_tw<T>(chirp, k));
+        LineBase b = line_base(a, l);
+        store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, n, v);
+    }
+}
+
+template <class Fmt>
+__global__ void k_scale_copy(const void* in, void* out, uint64_t count, double s) {
+    using T = typename Fmt::T;
+    GRID_STRIDE(e, count) { Fmt::st(out, e, scale(Fmt::ld(in, e), (T)s)); }
+}
+
+// ------------------------------------------------------------ synthetic data
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
+    // State after k+1 steps of SplitMix64 (Vigna), mixed.
+    uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_fill_uniform(float* out, uint64_t n, uint64_t seed) {
+    GRID_STRIDE(i, n) {
+        const uint64_t x = splitmix_at(seed, i);
+        const float u = (float)(x >> 40) * (1.0f / 16777216.0f);
+        out[i] = 2.0f * u - 1.0f;
+    }
+}
+
+__global__ void k_fill_normal(float* out, uint64_t n, uint64_t seed) {
+    const uint64_t pairs = (n + 1) / 2;
+    GRID_STRIDE(k, pairs) {
+        const double u1 = (double)((splitmix_at(seed, 2 * k) >> 11) + 1) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(splitmix_at(seed, 2 * k + 1) >> 11) * (1.0 / 9007199254740992.0);
+        const double r = sqrt(-2.0 * log(u1));
+        double s, c;
+        sincospi(2.0 * u2, &s, &c);
+        out[2 * k] = (float)(r * c);
+        if (2 * k + 1 < n) out[2 * k + 1] = (float)(r * s);
+    }
+}
+
+// ------------------------------------------------------------- launchers
+static inline dim3 grid_for(uint64_t n, int block = 256) {
+    uint64_t g = (n + block - 1) / block;
+    if (g > 65536u * 16u) g = 65536u * 16u;
+    if (g == 0) g = 1;
+    return dim3((unsigned)g);
+}
+
+hipError_t gen_twiddles(int prec, void* out, uint64_t n, uint64_t count, hipStream_t s) {
+    if (prec == P_F32)
+        k_gen_tw<float><<<grid_for(count), 256, 0, s>>>((cpx<float>*)out, n, count);
+    else
+        k_gen_tw<double><<<grid_for(count), 256, 0, s>>>((cpx<double>*)out, n, count);
+    return hipGetLastError();
+}
+
+hipError_t gen_twiddles_f64_strided(double2* out, uint64_t n, uint64_t count, uint64_t mult,
+                                    hipStream_t s) {
+    k_gen_tw_f64_strided<<<grid_for(count), 256, 0, s>>>(out, n, count, mult);
+    return hipGetLastError();
+}
+
+hipError_t gen_chirp(int prec, void* out, uint64_t n, hipStream_t s) {
+    if (prec == P_F32)
+        k_gen_chirp<float><<<grid_for(n), 256, 0, s>>>((cpx<float>*)out, n);
+    else
+        k_gen_chirp<double><<<grid_for(n), 256, 0, s>>>((cpx<double>*)out, n);
+    return hipGetLastError();
+}
+
+hipError_t gen_rader_seq(double2* out, uint64_t p, uint64_t m, uint64_t ginv, hipStream_t s) {
+    k_rader_seq<<<grid_for(m), 256, 0, s>>>(out, p, m, ginv);
+    return hipGetLastError();
+}
+
+hipError_t gen_bluestein_seq(double2* out, uint64_t n, uint64_t m, hipStream_t s) {
+    k_bluestein_seq<<<grid_for(m), 256, 0, s>>>(out, n, m);
+    return hipGetLastError();
+}
+
+hipError_t round_table(int prec, void* out, const double2* in, uint64_t count, double mul,
+                       hipStream_t s) {
+    if (prec == P_F32)
+        k_round_table<float><<<grid_for(count), 256, 0, s>>>((cpx<float>*)out, in, count, mul);
+    else
+        k_round_table<double><<<grid_for(count), 256, 0, s>>>((cpx<double>*)out, in, count, mul);
+    return hipGetLastError();
+}
+
+// The multi-pass Bluestein workspace is F32 for f32 plans and F64 otherwise.
+hipError_t launch_chirp_pre(int prec, const LineArgs& a, const void* chirp, uint32_t n,
+                            uint32_t m, hipStream_t s) {
+    const uint64_t total = (uint64_t)a.lines * m;
+    if (prec == P_F32) k_chirp_pre<FmtF32><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
+    else if (prec == P_F64) k_chirp_pre<FmtF64><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise(int prec, void* buf, const void* filt, uint64_t m, uint64_t lines,
+                            hipStream_t s) {
+    const uint64_t total = m * lines;
+    if (prec == P_F32) k_pointwise<FmtF32><<<grid_for(total), 256, 0, s>>>(buf, filt, m, total);
+    else k_pointwise<FmtF64><<<grid_for(total), 256, 0, s>>>(buf, filt, m, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uint32_t n,
+                             uint32_t m, hipStream_t s) {
+    const uint64_t total = (uint64_t)a.lines * n;
+    if (prec == P_F32) k_chirp_post<FmtF32><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
+    else if (prec == P_F64) k_chirp_post<FmtF64><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_copy(int prec, const void* in, void* out, uint64_t count, double sc,
+                             hipStream_t s) {
+    if (prec == P_F32) k_scale_copy<FmtF32><<<grid_for(count), 256, 0, s>>>(in, out, count, sc);
+    else if (prec == P_F64) k_scale_copy<FmtF64><<<grid_for(count), 256, 0, s>>>(in, out, count, sc);
+    else k_scale_copy<FmtDF64><<<grid_for(count), 256, 0, s>>>(in, out, count, sc);
+    return hipGetLastError();
+}
+
+hipError_t fill_uniform_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s) {
+    k_fill_uniform<<<grid_for(n), 256, 0, s>>>(out, n, seed);
+    return hipGetLastError();
+}
+
+hipError_t fill_normal_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s) {
+    k_fill_normal<<<grid_for((n + 1) / 2), 256, 0, s>>>(out, n, seed);
+    return hipGetLastError();
+}
+
+DivMagic make_magic(uint32_t d) {
+    DivMagic r;
+    if (d <= 1) {
+        r.m = 1u << 31;
+        r.s = 31;
+        return r;
+    }
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    const unsigned __int128 num = (unsigned __int128)1 << (31 + l);
+    r.m = (uint32_t)((num + d - 1) / d);
+    r.s = 31 + l;
+    return r;
+}
+
+}  // namespace mifft

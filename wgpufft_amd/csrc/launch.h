@@ -1,0 +1,88 @@
+// Host-side view of the kernel library: the compiled kernel-variant registry
+// (the counterpart of the reference's typed pipeline-cache keys,
+// src/runtime/pipeline_cache.rs:2048-2260) and the launch wrappers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fft_kernels.hpp"
+
+namespace mifft {
+
+enum Prec { P_F32 = 0, P_F64 = 1, P_DF64 = 2 };
+
+// One compiled register-resident line kernel.
+struct RegVariant {
+    int prec;
+    int n;
+    int tpl;
+    int lines;
+    int ld_lf, st_lf;
+    int ldm, stm;
+    int post;
+    int nthreads;
+    size_t lds_bytes;
+    const void* fn;
+    const char* radices;  // e.g. "16x16x16"
+};
+
+// Registry lookup; nullptr when no compiled variant matches.
+const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
+                                   int post);
+int reg_variant_count();
+const RegVariant* reg_variant_at(int i);
+
+hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s);
+
+// Runtime-schedule LDS kernel (any {2..16}-smooth schedule whose lines fit LDS).
+hipError_t launch_lds(int prec, int ldm, int stm, int post, const LineArgs& a, const Sched& sc,
+                      hipStream_t s);
+size_t lds_kernel_bytes(int prec, uint32_t n, uint32_t lpw);
+constexpr size_t kLdsBudget = 160 * 1024;
+
+// Fused Rader / Bluestein convolution kernels (one line per workgroup pass).
+struct ConvArgs {
+    LineArgs io;              // the prime / Bluestein line layout (n points)
+    Sched sc;                 // schedule of the convolution length M
+    const void* filt;         // FFT_M of the filter, scaled by 1/M (arith type)
+    const void* chirp;        // Bluestein chirp c[n] (arith type), or nullptr
+    const uint32_t* perm_in;  // Rader gather g^q mod p, q < p-1
+    const uint32_t* perm_out; // Rader scatter g^-u mod p, u < p-1
+    uint32_t n;               // transform length
+    uint32_t m;               // convolution length
+};
+hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s);
+hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s);
+size_t conv_kernel_bytes(int prec, uint32_t m, uint32_t lpw);
+
+// Table generation (FP64 on the device, rounded once to the table type).
+hipError_t gen_twiddles(int prec, void* out, uint64_t n, uint64_t count, hipStream_t s);
+hipError_t gen_twiddles_f64_strided(double2* out, uint64_t n, uint64_t count, uint64_t mult,
+                                    hipStream_t s);
+hipError_t gen_chirp(int prec, void* out, uint64_t n, hipStream_t s);
+hipError_t gen_rader_seq(double2* out, uint64_t p, uint64_t m, uint64_t ginv, hipStream_t s);
+hipError_t gen_bluestein_seq(double2* out, uint64_t n, uint64_t m, hipStream_t s);
+hipError_t round_table(int prec, void* out, const double2* in, uint64_t count, double mul,
+                       hipStream_t s);
+
+// Elementwise helpers of the multi-pass Bluestein route and the direct DFT.
+hipError_t launch_chirp_pre(int prec, const LineArgs& a, const void* chirp, uint32_t n,
+                            uint32_t m, hipStream_t s);
+hipError_t launch_pointwise(int prec, void* buf, const void* filt, uint64_t m, uint64_t lines,
+                            hipStream_t s);
+hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uint32_t n,
+                             uint32_t m, hipStream_t s);
+hipError_t launch_scale_copy(int prec, const void* in, void* out, uint64_t count, double scale,
+                             hipStream_t s);
+
+// Synthetic data: SplitMix64 counter-based uniform[-1,1) f32 and N(0,1) f32.
+hipError_t fill_uniform_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s);
+hipError_t fill_normal_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s);
+
+DivMagic make_magic(uint32_t d);
+
+// Raises a kernel's dynamic-LDS limit to the full 160 KiB once per process.
+void ensure_lds_attr(const void* fn);
+
+}  // namespace mifft

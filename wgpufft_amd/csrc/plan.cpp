@@ -1,0 +1,918 @@
+// Planner and executor of the host runtime.
+//
+// Plan creation restates the reference's route choice (FftPlan::c2c ->
+// C2cPlan::new -> select_route / AxisPlan::new, src/runtime/c2c.rs:869-1275,
+// 7631-7650, src/runtime/axis_plan.rs:516-1144) for gfx950: each transformed
+// axis becomes one or more kernel launches ("ops") that read and write device
+// buffers named by role (input, output, workspace).  Tables (twiddles,
+// chirps, Rader/Bluestein filter spectra) are generated on the device in FP64
+// at plan creation and rounded once.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+namespace mifft {
+
+void fail(mifft_status code, const std::string& detail) { throw Error{code, detail}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(MIFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Plan::~Plan() {
+    for (DevBuf& b : owned)
+        if (b.p) (void)hipFree(b.p);
+    if (own_ws.p) (void)hipFree(own_ws.p);
+    if (in_copy.p) (void)hipFree(in_copy.p);
+}
+
+// ------------------------------------------------------------ integer helpers
+static const uint64_t kFactorOrder[8] = {13, 11, 8, 7, 5, 4, 3, 2};
+
+// factor_supported_length (src/runtime/mod.rs:33-54): greedy over [13,11,8,7,5,4,3,2].
+std::vector<uint64_t> factor_supported(uint64_t n, bool* ok) {
+    std::vector<uint64_t> f;
+    *ok = n > 0;
+    uint64_t r = n;
+    while (*ok && r > 1) {
+        bool found = false;
+        for (uint64_t c : kFactorOrder) {
+            if (r % c == 0) {
+                f.push_back(c);
+                r /= c;
+                found = true;
+                break;
+            }
+        }
+        if (!found) *ok = false;
+    }
+    return f;
+}
+
+static bool is_smooth(uint64_t n) {
+    if (n == 0) return false;
+    for (uint64_t p : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+bool is_prime_u64(uint64_t n) {
+    if (n < 2) return false;
+    if (n % 2 == 0) return n == 2;
+    for (uint64_t d = 3; d <= n / d; d += 2)
+        if (n % d == 0) return false;
+    return true;
+}
+
+uint64_t mod_pow_u64(uint64_t b, uint64_t e, uint64_t m) {
+    unsigned __int128 r = 1 % m, x = b % m;
+    while (e) {
+        if (e & 1) r = r * x % m;
+        x = x * x % m;
+        e >>= 1;
+    }
+    return (uint64_t)r;
+}
+
+// primitive_root_prime (src/runtime/axis_policy.rs:134-153).
+uint64_t primitive_root(uint64_t p) {
+    if (!is_prime_u64(p)) return 0;
+    if (p == 2) return 1;
+    std::vector<uint64_t> fs;
+    uint64_t n = p - 1;
+    for (uint64_t d = 2; d <= n / d; d += (d == 2 ? 1 : 2)) {
+        if (n % d == 0) {
+            fs.push_back(d);
+            while (n % d == 0) n /= d;
+        }
+    }
+    if (n > 1) fs.push_back(n);
+    for (uint64_t g = 2; g < p; g++) {
+        bool ok = true;
+        for (uint64_t f : fs)
+            if (mod_pow_u64(g, (p - 1) / f, p) == 1) {
+                ok = false;
+                break;
+            }
+        if (ok) return g;
+    }
+    return 0;
+}
+
+// next_smooth_at_least (src/runtime/axis_policy.rs:155-161).
+uint64_t next_smooth(uint64_t n) {
+    uint64_t c = n ? n : 1;
+    while (!is_smooth(c)) c++;
+    return c;
+}
+
+// rader_convolution_length (src/runtime/rader_axis.rs:1270-1289).
+uint64_t rader_conv_len(uint64_t p) {
+    const uint64_t l = p - 1;
+    if (l >= 2 && is_smooth(l)) return l;
+    return next_smooth(2 * l - 1);
+}
+
+// bluestein_convolution_length (src/runtime/bluestein_axis.rs:1035-1041).
+uint64_t bluestein_conv_len(uint64_t n) { return next_smooth(2 * n - 1); }
+
+// ------------------------------------------------------ config validation
+static void validate_tuning(const mifft_config& c) {
+    const mifft_tuning& t = c.tuning;
+    auto pow2 = [](uint32_t v) { return v != 0 && (v & (v - 1)) == 0; };
+    if (!pow2(t.workgroup_size))
+        fail(MIFFT_ERR_INVALID_TUNING, "workgroup_size must be a non-zero power of two");
+    if (!pow2(t.fused_workgroup_size))
+        fail(MIFFT_ERR_INVALID_TUNING, "fused_workgroup_size must be a non-zero power of two");
+    if (t.workgroup_size > 1024 || t.fused_workgroup_size > 1024)
+        fail(MIFFT_ERR_INVALID_TUNING, "workgroup size exceeds the 1024-thread device limit");
+    if (t.rader_max_prime < 2) fail(MIFFT_ERR_INVALID_TUNING, "rader_max_prime must be at least 2");
+    if (t.n_force_rader_axes < 0 || t.n_force_rader_axes > MIFFT_MAX_FORCED_AXES ||
+        t.n_force_bluestein_axes < 0 || t.n_force_bluestein_axes > MIFFT_MAX_FORCED_AXES)
+        fail(MIFFT_ERR_INVALID_TUNING, "forced axis count out of range");
+    for (int i = 0; i < t.n_force_rader_axes; i++)
+        for (int j = 0; j < t.n_force_bluestein_axes; j++)
+            if (t.force_rader_axes[i] == t.force_bluestein_axes[j])
+                fail(MIFFT_ERR_INVALID_TUNING,
+                     "the same axis cannot be forced to both Rader and Bluestein");
+    for (int pass = 0; pass < 2; pass++) {
+        const int32_t* ax = pass == 0 ? t.force_rader_axes : t.force_bluestein_axes;
+        const int n = pass == 0 ? t.n_force_rader_axes : t.n_force_bluestein_axes;
+        const char* field = pass == 0 ? "force_rader_axes" : "force_bluestein_axes";
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < i; j++)
+                if (ax[j] == ax[i]) fail(MIFFT_ERR_INVALID_TUNING, std::string(field) + ": axis appears more than once");
+            if (ax[i] < 0 || ax[i] >= c.rank)
+                fail(MIFFT_ERR_INVALID_TUNING, std::string(field) + ": axis is outside the configured shape rank");
+            bool selected = false;
+            for (int k = 0; k < c.naxes; k++) selected |= c.axes[k] == ax[i];
+            if (!selected)
+                fail(MIFFT_ERR_INVALID_TUNING, std::string(field) + ": axis is not present in FftConfig::axes");
+            const uint64_t len = c.shape[ax[i]];
+            if (pass == 0 && (len < 3 || !is_prime_u64(len)))
+                fail(MIFFT_ERR_INVALID_TUNING, "forced Rader axes must have prime length at least 3");
+            if (pass == 1 && len < 2)
+                fail(MIFFT_ERR_INVALID_TUNING, "forced Bluestein axes must have length at least 2");
+        }
+    }
+}
+
+uint64_t config_total(const mifft_config& c) {
+    unsigned __int128 t = c.batch;
+    for (int i = 0; i < c.rank; i++) {
+        t *= c.shape[i];
+        if (t > 0xFFFFFFFFull) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "prod(shape)*batch exceeds u32::MAX elements");
+    }
+    return (uint64_t)t;
+}
+
+// FftConfig::validate (src/config.rs:178-211).
+void validate_config(const mifft_config& c) {
+    if (c.rank < 0 || c.rank > MIFFT_MAX_RANK) fail(MIFFT_ERR_RANK_TOO_LARGE, "rank out of range 1..8");
+    if (c.rank == 0) fail(MIFFT_ERR_ZERO_LENGTH, "empty shape");
+    for (int i = 0; i < c.rank; i++)
+        if (c.shape[i] == 0) fail(MIFFT_ERR_ZERO_LENGTH, "zero-length axis");
+    if (c.batch == 0) fail(MIFFT_ERR_ZERO_BATCH, "batch must be at least 1");
+    if (c.naxes <= 0) fail(MIFFT_ERR_EMPTY_AXES, "no axes selected");
+    if (c.naxes > MIFFT_MAX_RANK) fail(MIFFT_ERR_RANK_TOO_LARGE, "too many axes");
+    if (c.direction < 0 || c.direction > 1 || c.normalization < 0 || c.normalization > 3 ||
+        c.precision < 0 || c.precision > 2)
+        fail(MIFFT_ERR_INVALID_ENUM, "direction/normalization/precision out of range");
+    bool seen[MIFFT_MAX_RANK] = {false};
+    for (int k = 0; k < c.naxes; k++) {
+        const int a = c.axes[k];
+        if (a < 0 || a >= c.rank)
+            fail(MIFFT_ERR_INVALID_AXIS, "axis " + std::to_string(a) + " >= rank " + std::to_string(c.rank));
+        if (seen[a]) fail(MIFFT_ERR_DUPLICATE_AXIS, "axis " + std::to_string(a) + " repeated");
+        seen[a] = true;
+        if (c.shape[a] == 1 && config_total(c) != 1)
+            fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "length-1 axis in a transform of more than one point");
+    }
+    config_total(c);
+    validate_tuning(c);
+}
+
+// FftConfig::scale_f64 (src/config.rs:274-286): 1/prod(shape), not of the selected axes.
+double config_scale(const mifft_config& c) {
+    double len = 1.0;
+    for (int i = 0; i < c.rank; i++) len *= (double)c.shape[i];
+    const bool inv = c.direction == MIFFT_INVERSE;
+    switch (c.normalization) {
+        case MIFFT_NORM_NONE: return 1.0;
+        case MIFFT_NORM_FORWARD: return inv ? 1.0 : 1.0 / len;
+        case MIFFT_NORM_INVERSE: return inv ? 1.0 / len : 1.0;
+        case MIFFT_NORM_ORTHOGONAL: return 1.0 / std::sqrt(len);
+        default: return 1.0;
+    }
+}
+
+// ------------------------------------------------------------------ builder
+static size_t elem_bytes(int prec) { return prec == P_F32 ? 8 : 16; }
+
+struct Layout {
+    int64_t si, sj, so, sn;
+};
+
+struct LineSet {
+    uint32_t lines, S1, S2;
+};
+
+// Radix schedule of the LDS engine for an n that is {2,3,5,7,11,13}-smooth:
+// radix-16 stages first, then the odd primes (3s paired to 9, a 3 and a 5 to 15).
+static std::vector<int> lds_schedule(uint64_t n) {
+    std::vector<int> r;
+    int e2 = 0, e3 = 0, e5 = 0, e7 = 0, e11 = 0, e13 = 0;
+    uint64_t m = n;
+    while (m % 2 == 0) { m /= 2; e2++; }
+    while (m % 3 == 0) { m /= 3; e3++; }
+    while (m % 5 == 0) { m /= 5; e5++; }
+    while (m % 7 == 0) { m /= 7; e7++; }
+    while (m % 11 == 0) { m /= 11; e11++; }
+    while (m % 13 == 0) { m /= 13; e13++; }
+    while (e2 >= 4) { r.push_back(16); e2 -= 4; }
+    if (e2 == 3) r.push_back(8);
+    if (e2 == 2) r.push_back(4);
+    if (e2 == 1) {
+        if (e3 > 0 && (e3 % 2 == 1)) { r.push_back(6); e3--; }
+        else r.push_back(2);
+    }
+    while (e3 >= 2) { r.push_back(9); e3 -= 2; }
+    if (e3 == 1) {
+        if (e5 > 0) { r.push_back(15); e5--; }
+        else r.push_back(3);
+    }
+    for (int i = 0; i < e5; i++) r.push_back(5);
+    for (int i = 0; i < e7; i++) r.push_back(7);
+    for (int i = 0; i < e11; i++) r.push_back(11);
+    for (int i = 0; i < e13; i++) r.push_back(13);
+    return r;
+}
+
+static Sched make_sched(uint64_t n, uint32_t lpw, bool ld_lf, bool st_lf) {
+    Sched sc;
+    std::memset(&sc, 0, sizeof(sc));
+    std::vector<int> rad = lds_schedule(n);
+    if (rad.size() > (size_t)kMaxStages) fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "too many stages");
+    sc.n = (uint32_t)n;
+    sc.ns = (uint32_t)rad.size();
+    sc.lpw = lpw;
+    sc.ld_lf = ld_lf;
+    sc.st_lf = st_lf;
+    sc.dn = make_magic((uint32_t)n);
+    sc.dlpw = make_magic(lpw);
+    uint32_t L = 1;
+    for (size_t s = 0; s < rad.size(); s++) {
+        sc.rad[s] = (uint8_t)rad[s];
+        sc.L[s] = L;
+        sc.dnr[s] = make_magic((uint32_t)(n / rad[s]));
+        sc.dl[s] = make_magic(L);
+        L *= rad[s];
+    }
+    return sc;
+}
+
+static std::string sched_str(const Sched& sc) {
+    std::string s;
+    for (uint32_t i = 0; i < sc.ns; i++) s += (i ? "x" : "") + std::to_string(sc.rad[i]);
+    return s.empty() ? "1" : s;
+}
+
+static bool adjacent_lines(const LineSet& ls, const Layout& l) {
+    if (l.sn == 1) return false;
+    if (ls.S1 > 1) return l.si == 1;
+    return ls.S2 > 1 && l.sj == 1;
+}
+
+struct Builder {
+    Plan& P;
+    hipStream_t s;
+    std::map<std::pair<int, uint64_t>, const void*> tw_cache;
+
+    void* alloc(size_t bytes) {
+        DevBuf b;
+        b.bytes = bytes;
+        hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
+        if (e != hipSuccess)
+            fail(MIFFT_ERR_ALLOCATION_FAILED, "plan table allocation of " + std::to_string(bytes) + " bytes");
+        P.owned.push_back(b);
+        return b.p;
+    }
+
+    // W_n^t for t < n, arithmetic precision, generated in FP64 on the device.
+    const void* twiddles(int arith, uint64_t n) {
+        auto key = std::make_pair(arith, n);
+        auto it = tw_cache.find(key);
+        if (it != tw_cache.end()) return it->second;
+        void* t = alloc(n * elem_bytes(arith));
+        hip_check(gen_twiddles(arith, t, n, n, s), "twiddle generation");
+        tw_cache[key] = t;
+        return t;
+    }
+
+    LineArgs base_args(const LineSet& ls, const Layout& in, const Layout& out, int arith,
+                       uint64_t n, bool swap_in, bool swap_out, double scale) {
+        LineArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.lines = ls.lines;
+        a.S1 = ls.S1;
+        a.S2 = ls.S2;
+        a.d1 = make_magic(ls.S1);
+        a.d2 = make_magic(ls.S2);
+        a.in_si = in.si; a.in_sj = in.sj; a.in_so = in.so; a.in_sn = in.sn;
+        a.out_si = out.si; a.out_sj = out.sj; a.out_so = out.so; a.out_sk = out.sn;
+        a.tw = n > 1 ? twiddles(arith, n) : nullptr;
+        a.scale = scale;
+        a.swap_in = swap_in;
+        a.swap_out = swap_out;
+        return a;
+    }
+
+    // One line kernel (register variant when compiled, else the LDS kernel).
+    // Returns false if the line does not fit on chip.
+    bool line_op(uint64_t n, const LineSet& ls, const Layout& in, const Layout& out, int src,
+                 uint64_t src_off, int dst, uint64_t dst_off, bool swap_in, bool swap_out,
+                 double scale, int ldm, int stm, const double2* post_hi = nullptr,
+                 const double2* post_lo = nullptr, uint32_t post_shift = 0, uint32_t post_mask = 0) {
+        const int post = post_hi != nullptr;
+        const bool ld_lf = adjacent_lines(ls, in);
+        const bool st_lf = adjacent_lines(ls, out);
+        Op op;
+        op.src = src;
+        op.src_off = src_off;
+        op.dst = dst;
+        op.dst_off = dst_off;
+        op.prec = P.prec;
+        op.ldm = ldm;
+        op.stm = stm;
+        op.post = post;
+        const RegVariant* rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post);
+        const size_t es = elem_bytes(P.arith);
+        const uint64_t lstr = lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30));
+        if (!rv && 2 * lstr * es > kLdsBudget) return false;
+        op.a = base_args(ls, in, out, P.arith, n, swap_in, swap_out, scale);
+        op.a.post_hi = post_hi;
+        op.a.post_lo = post_lo;
+        op.a.post_shift = post_shift;
+        op.a.post_mask = post_mask;
+        std::ostringstream d;
+        if (rv) {
+            op.kind = OP_REG;
+            op.rv = rv;
+            d << "reg N=" << n << " " << rv->radices << " lines=" << ls.lines << " tpl=" << rv->tpl
+              << " lpw=" << rv->lines;
+        } else {
+            op.kind = OP_LDS;
+            uint32_t fit = (uint32_t)(kLdsBudget / (2 * lstr * es));
+            uint32_t lpw;
+            if (ld_lf || st_lf) lpw = P.arith == P_F32 ? 16 : 8;
+            else lpw = (uint32_t)std::max<uint64_t>(1, 4096 / n);
+            lpw = std::max<uint32_t>(1, std::min(lpw, fit));
+            op.sc = make_sched(n, lpw, ld_lf, st_lf);
+            d << "lds N=" << n << " " << sched_str(op.sc) << " lines=" << ls.lines << " lpw=" << lpw;
+        }
+        if (ld_lf || st_lf) d << (ld_lf ? " ldLF" : "") << (st_lf ? " stLF" : "");
+        if (post) d << " post";
+        if (ldm != LD_COMPLEX || stm != ST_COMPLEX) d << " mode=" << ldm << "/" << stm;
+        op.desc = d.str();
+        P.ops.push_back(op);
+        return true;
+    }
+
+    // f64 device FFT of a length-m sequence in place (filter spectra).
+    void fft_f64_inplace(double2* seq, uint64_t m) {
+        mifft_config cc;
+        std::memset(&cc, 0, sizeof(cc));
+        cc.rank = 1;
+        cc.shape[0] = m;
+        cc.naxes = 1;
+        cc.axes[0] = 0;
+        cc.batch = 1;
+        cc.direction = MIFFT_FORWARD;
+        cc.normalization = MIFFT_NORM_NONE;
+        cc.precision = MIFFT_F64;
+        cc.tuning = P.cfg.tuning;
+        cc.tuning.n_force_rader_axes = 0;
+        cc.tuning.n_force_bluestein_axes = 0;
+        std::unique_ptr<Plan> child = build_plan(cc, MIFFT_C2C);
+        void* ws = nullptr;
+        if (child->ws_bytes) hip_check(hipMalloc(&ws, child->ws_bytes), "filter workspace");
+        run_plan(*child, s, seq, seq, ws);
+        hip_check(hipStreamSynchronize(s), "filter FFT");
+        if (ws) (void)hipFree(ws);
+    }
+
+    // FFT_m(seq)/m rounded once to the arithmetic precision.
+    const void* filter_spectrum(double2* seq, uint64_t m) {
+        fft_f64_inplace(seq, m);
+        void* f = alloc(m * elem_bytes(P.arith));
+        hip_check(round_table(P.arith, f, seq, m, 1.0 / (double)m, s), "filter rounding");
+        return f;
+    }
+
+    bool conv_fits(uint64_t m, uint32_t* lpw_out, bool lf) {
+        const uint32_t want = lf ? (P.arith == P_F32 ? 16u : 8u) : (uint32_t)std::max<uint64_t>(1, 2048 / m);
+        for (uint32_t lpw = want; lpw >= 1; lpw = lpw / 2) {
+            if (conv_kernel_bytes(P.arith, (uint32_t)m, lpw) <= kLdsBudget) {
+                *lpw_out = lpw;
+                return true;
+            }
+            if (lpw == 1) break;
+        }
+        return false;
+    }
+
+    void rader_op(uint64_t p, uint64_t m, uint32_t lpw, const LineSet& ls, const Layout& in,
+                  const Layout& out, int src, uint64_t src_off, int dst, uint64_t dst_off,
+                  bool swap_in, bool swap_out, double scale) {
+        const uint64_t g = primitive_root(p);
+        const uint64_t ginv = mod_pow_u64(g, p - 2, p);
+        std::vector<uint32_t> pin(p - 1), pout(p - 1);
+        for (uint64_t q = 0; q + 1 < p; q++) {
+            pin[q] = (uint32_t)mod_pow_u64(g, q, p);
+            pout[q] = (uint32_t)mod_pow_u64(ginv, q, p);
+        }
+        uint32_t* dpin = (uint32_t*)alloc(4 * (p - 1));
+        uint32_t* dpout = (uint32_t*)alloc(4 * (p - 1));
+        hip_check(hipMemcpyAsync(dpin, pin.data(), 4 * (p - 1), hipMemcpyHostToDevice, s), "perm upload");
+        hip_check(hipMemcpyAsync(dpout, pout.data(), 4 * (p - 1), hipMemcpyHostToDevice, s), "perm upload");
+        double2* seq = nullptr;
+        hip_check(hipMalloc(&seq, m * sizeof(double2)), "rader sequence");
+        hip_check(gen_rader_seq(seq, p, m, ginv, s), "rader sequence");
+        const void* filt = filter_spectrum(seq, m);
+        hip_check(hipStreamSynchronize(s), "rader filter");
+        (void)hipFree(seq);
+        Op op;
+        op.kind = OP_RADER;
+        op.src = src;
+        op.src_off = src_off;
+        op.dst = dst;
+        op.dst_off = dst_off;
+        op.prec = P.prec;
+        op.conv.io = base_args(ls, in, out, P.arith, m, swap_in, swap_out, scale);
+        op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
+        op.conv.filt = filt;
+        op.conv.perm_in = dpin;
+        op.conv.perm_out = dpout;
+        op.conv.n = (uint32_t)p;
+        op.conv.m = (uint32_t)m;
+        std::ostringstream d;
+        d << "rader p=" << p << " M=" << m << " " << sched_str(op.conv.sc) << " lines=" << ls.lines
+          << " lpw=" << lpw;
+        op.desc = d.str();
+        P.ops.push_back(op);
+    }
+
+    const void* chirp_table(uint64_t n) {
+        void* c = alloc(n * elem_bytes(P.arith));
+        hip_check(gen_chirp(P.arith, c, n, s), "chirp generation");
+        return c;
+    }
+
+    const void* bluestein_filter(uint64_t n, uint64_t m) {
+        double2* seq = nullptr;
+        hip_check(hipMalloc(&seq, m * sizeof(double2)), "bluestein sequence");
+        hip_check(gen_bluestein_seq(seq, n, m, s), "bluestein sequence");
+        const void* f = filter_spectrum(seq, m);
+        hip_check(hipStreamSynchronize(s), "bluestein filter");
+        (void)hipFree(seq);
+        return f;
+    }
+
+    void bluestein_fused_op(uint64_t n, uint64_t m, uint32_t lpw, const LineSet& ls,
+                            const Layout& in, const Layout& out, int src, uint64_t src_off,
+                            int dst, uint64_t dst_off, bool swap_in, bool swap_out, double scale) {
+        Op op;
+        op.kind = OP_BLUESTEIN;
+        op.src = src;
+        op.src_off = src_off;
+        op.dst = dst;
+        op.dst_off = dst_off;
+        op.prec = P.prec;
+        op.conv.io = base_args(ls, in, out, P.arith, m, swap_in, swap_out, scale);
+        op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
+        op.conv.filt = bluestein_filter(n, m);
+        op.conv.chirp = chirp_table(n);
+        op.conv.n = (uint32_t)n;
+        op.conv.m = (uint32_t)m;
+        std::ostringstream d;
+        d << "bluestein N=" << n << " M=" << m << " " << sched_str(op.conv.sc)
+          << " lines=" << ls.lines << " lpw=" << lpw;
+        op.desc = d.str();
+        P.ops.push_back(op);
+    }
+
+    // Multi-pass Bluestein for convolutions that do not fit LDS; returns the
+    // workspace bytes used from ws_off.
+    uint64_t bluestein_multi(uint64_t n, uint64_t m, const LineSet& ls, const Layout& in,
+                             const Layout& out, int src, uint64_t src_off, int dst,
+                             uint64_t dst_off, uint64_t ws_off, bool swap_in, bool swap_out,
+                             double scale) {
+        if (P.prec == P_DF64)
+            fail(MIFFT_ERR_PRECISION_UNSUPPORTED, "df64 Bluestein beyond the on-chip convolution length");
+        const uint64_t region = (uint64_t)ls.lines * m * elem_bytes(P.arith);
+        mifft_config cc;
+        std::memset(&cc, 0, sizeof(cc));
+        cc.rank = 1;
+        cc.shape[0] = m;
+        cc.naxes = 1;
+        cc.batch = ls.lines;
+        cc.direction = MIFFT_FORWARD;
+        cc.normalization = MIFFT_NORM_NONE;
+        cc.precision = P.arith == P_F32 ? MIFFT_F32 : MIFFT_F64;
+        cc.tuning = P.cfg.tuning;
+        cc.tuning.n_force_rader_axes = 0;
+        cc.tuning.n_force_bluestein_axes = 0;
+        P.children.push_back(build_plan(cc, MIFFT_C2C));
+        Plan* child = P.children.back().get();
+        const void* chirp = chirp_table(n);
+        const void* filt = bluestein_filter(n, m);
+        LineArgs a = base_args(ls, in, out, P.arith, 1, swap_in, swap_out, scale);
+        Op pre;
+        pre.kind = OP_CHIRP_PRE;
+        pre.src = src;
+        pre.src_off = src_off;
+        pre.dst = B_WS;
+        pre.dst_off = ws_off;
+        pre.prec = P.prec;
+        pre.a = a;
+        pre.a.swap_out = 0;
+        pre.a.scale = 1.0;
+        pre.chirp = chirp;
+        pre.n = (uint32_t)n;
+        pre.m = (uint32_t)m;
+        pre.desc = "bluestein-pre N=" + std::to_string(n) + " M=" + std::to_string(m);
+        P.ops.push_back(pre);
+        Op sub;
+        sub.kind = OP_SUBPLAN;
+        sub.dst = B_WS;
+        sub.dst_off = ws_off;
+        sub.sub = child;
+        sub.sub_ws_off = ws_off + region;
+        sub.desc = "subplan FFT M=" + std::to_string(m) + " x" + std::to_string(ls.lines);
+        P.ops.push_back(sub);
+        Op pw;
+        pw.kind = OP_POINTWISE;
+        pw.dst = B_WS;
+        pw.dst_off = ws_off;
+        pw.prec = P.prec;
+        pw.filt = filt;
+        pw.m = (uint32_t)m;
+        pw.count = ls.lines;
+        pw.desc = "bluestein-filter M=" + std::to_string(m);
+        P.ops.push_back(pw);
+        P.ops.push_back(sub);
+        Op post;
+        post.kind = OP_CHIRP_POST;
+        post.src = B_WS;
+        post.src_off = ws_off;
+        post.dst = dst;
+        post.dst_off = dst_off;
+        post.prec = P.prec;
+        post.a = a;
+        post.a.swap_in = 0;
+        post.chirp = chirp;
+        post.n = (uint32_t)n;
+        post.m = (uint32_t)m;
+        post.desc = "bluestein-post N=" + std::to_string(n);
+        P.ops.push_back(post);
+        return region + child->ws_bytes;
+    }
+
+    // Four-step split n = n1*n2 of an axis too long for one workgroup: pass 1
+    // transforms n1-point columns (stride n2) with W_n^(q2 k1) on store into the
+    // workspace, pass 2 transforms n2-point rows and stores transposed
+    // (k = k1 + n1 k2).  Returns false if no split fits.
+    bool four_step(uint64_t n, uint64_t S, uint64_t O, int src, uint64_t src_off, int dst,
+                   uint64_t dst_off, uint64_t ws_off, bool swap_in, bool swap_out, double scale,
+                   uint64_t* ws_used) {
+        // Prefer a balanced split with register variants on both sides.
+        uint64_t best1 = 0;
+        double best_cost = 1e30;
+        for (uint64_t n1 = 2; n1 * n1 <= n * 16 && n1 < n; n1++) {
+            if (n % n1) continue;
+            const uint64_t n2 = n / n1;
+            const size_t es = elem_bytes(P.arith);
+            auto fits = [&](uint64_t len) {
+                return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= kLdsBudget;
+            };
+            if (!fits(n1) || !fits(n2)) continue;
+            const double cost = std::fabs(std::log((double)n1 / (double)n2));
+            if (cost < best_cost) {
+                best_cost = cost;
+                best1 = n1;
+            }
+        }
+        if (!best1) return false;
+        const uint64_t n1 = best1, n2 = n / n1;
+        const uint64_t lines_total = S * O;
+        // post twiddle tables: W_n^t = hi[t >> sh] * lo[t & mask]
+        uint32_t sh = 0;
+        while ((1ull << (2 * sh)) < n) sh++;
+        const uint64_t lo_n = 1ull << sh;
+        const uint64_t hi_n = (n + lo_n - 1) / lo_n;
+        double2* lo = (double2*)alloc(lo_n * sizeof(double2));
+        double2* hi = (double2*)alloc(hi_n * sizeof(double2));
+        hip_check(gen_twiddles_f64_strided(lo, n, lo_n, 1, s), "post twiddles");
+        hip_check(gen_twiddles_f64_strided(hi, n, hi_n, lo_n, s), "post twiddles");
+        const uint64_t N = n;
+        // pass 1: lines (i < S, q2 < n2, o < O), points q1 at stride S*n2
+        LineSet l1{(uint32_t)(lines_total * n2), (uint32_t)S, (uint32_t)n2};
+        Layout in1{1, (int64_t)S, (int64_t)(S * N), (int64_t)(S * n2)};
+        Layout ws1 = in1;
+        if (!line_op(n1, l1, in1, ws1, src, src_off, B_WS, ws_off, swap_in, false, 1.0, LD_COMPLEX,
+                     ST_COMPLEX, hi, lo, sh, (uint32_t)(lo_n - 1)))
+            return false;
+        // pass 2: lines (i < S, k1 < n1, o < O), points q2 at stride S
+        LineSet l2{(uint32_t)(lines_total * n1), (uint32_t)S, (uint32_t)n1};
+        Layout in2{1, (int64_t)(S * n2), (int64_t)(S * N), (int64_t)S};
+        Layout out2{1, (int64_t)S, (int64_t)(S * N), (int64_t)(S * n1)};
+        if (!line_op(n2, l2, in2, out2, B_WS, ws_off, dst, dst_off, false, swap_out, scale,
+                     LD_COMPLEX, ST_COMPLEX))
+            return false;
+        *ws_used = lines_total * N * elem_bytes(P.prec);
+        return true;
+    }
+
+    // All ops of one C2C axis: `n` points at stride S, O outer repetitions.
+    // Returns the workspace bytes used beyond ws_off.
+    uint64_t axis_ops(uint64_t n, int kind, uint64_t S, uint64_t O, int src, uint64_t src_off,
+                      int dst, uint64_t dst_off, uint64_t ws_off, bool inverse, double scale) {
+        const LineSet ls{(uint32_t)(S * O), (uint32_t)S, 1};
+        const Layout lay{1, 0, (int64_t)(S * n), (int64_t)S};
+        const bool lf = adjacent_lines(ls, lay);
+        if (kind == MIFFT_AXIS_MIXED) {
+            if (line_op(n, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale,
+                        LD_COMPLEX, ST_COMPLEX))
+                return 0;
+            uint64_t used = 0;
+            if (four_step(n, S, O, src, src_off, dst, dst_off, ws_off, inverse, inverse, scale, &used))
+                return used;
+            fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "no on-chip split for length " + std::to_string(n));
+        }
+        if (kind == MIFFT_AXIS_RADER) {
+            const uint64_t m = rader_conv_len(n);
+            uint32_t lpw = 0;
+            if (conv_fits(m, &lpw, lf)) {
+                rader_op(n, m, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale);
+                return 0;
+            }
+            // Rader convolutions that do not fit on chip run as Bluestein.
+        }
+        const uint64_t m = bluestein_conv_len(n);
+        uint32_t lpw = 0;
+        if (conv_fits(m, &lpw, lf)) {
+            bluestein_fused_op(n, m, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale);
+            return 0;
+        }
+        return bluestein_multi(n, m, ls, lay, lay, src, src_off, dst, dst_off, ws_off, inverse,
+                               inverse, scale);
+    }
+};
+
+static int axis_kind_for(const mifft_config& c, int axis) {
+    const mifft_tuning& t = c.tuning;
+    for (int i = 0; i < t.n_force_rader_axes; i++)
+        if (t.force_rader_axes[i] == axis) return MIFFT_AXIS_RADER;
+    for (int i = 0; i < t.n_force_bluestein_axes; i++)
+        if (t.force_bluestein_axes[i] == axis) return MIFFT_AXIS_BLUESTEIN;
+    // axis_kind_for_len_with_rader_max (src/runtime/axis_policy.rs:72-80)
+    const uint64_t n = c.shape[axis];
+    bool ok = false;
+    factor_supported(n, &ok);
+    if (ok) return MIFFT_AXIS_MIXED;
+    if (is_prime_u64(n) && n <= t.rader_max_prime) return MIFFT_AXIS_RADER;
+    return MIFFT_AXIS_BLUESTEIN;
+}
+
+std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
+    validate_config(cfg);
+    if (kind != MIFFT_C2C && kind != MIFFT_R2C && kind != MIFFT_C2R)
+        fail(MIFFT_ERR_INVALID_ENUM, "plan kind out of range");
+    if (kind != MIFFT_C2C) {
+        // validate_real_config / validate_real_precision (src/runtime/real.rs:38-63, 3151-3174)
+        const int want = kind == MIFFT_R2C ? MIFFT_FORWARD : MIFFT_INVERSE;
+        if (cfg.direction != want)
+            fail(MIFFT_ERR_INVALID_REAL_DIRECTION,
+                 kind == MIFFT_R2C ? "r2c expects forward" : "c2r expects inverse");
+        bool all = cfg.naxes == cfg.rank;
+        for (int i = 0; i < cfg.naxes && all; i++) all = cfg.axes[i] == i;
+        if (!all) fail(MIFFT_ERR_UNSUPPORTED_REAL_AXES, "real transforms use every axis in order");
+        if (cfg.precision != MIFFT_F32)
+            fail(MIFFT_ERR_PRECISION_UNSUPPORTED,
+                 cfg.precision == MIFFT_F64 ? "real-f64-not-implemented" : "real-df64-not-implemented");
+    }
+    auto P = std::make_unique<Plan>();
+    P->cfg = cfg;
+    P->kind = kind;
+    P->prec = cfg.precision;
+    P->arith = cfg.precision == MIFFT_F32 ? P_F32 : P_F64;
+    hip_check(hipGetDevice(&P->device), "hipGetDevice");
+    const uint64_t logical = config_total(cfg) / cfg.batch;
+    const uint64_t total = config_total(cfg);
+    const bool inverse = cfg.direction == MIFFT_INVERSE;
+    const double scale = config_scale(cfg);
+    for (int k = 0; k < cfg.naxes; k++) {
+        AxisInfo ai;
+        ai.axis = cfg.axes[k];
+        ai.n = cfg.shape[ai.axis];
+        ai.kind = axis_kind_for(cfg, ai.axis);
+        bool ok = false;
+        ai.factors = factor_supported(ai.n, &ok);
+        if (!ok) ai.factors.clear();
+        P->axes.push_back(ai);
+    }
+    // select_route (src/runtime/c2c.rs:7631-7650)
+    if (total == 1) P->route = MIFFT_ROUTE_DIRECT_DFT;
+    else {
+        bool all_mixed = true;
+        for (auto& a : P->axes) all_mixed &= a.kind == MIFFT_AXIS_MIXED;
+        if (all_mixed) P->route = MIFFT_ROUTE_MIXED_RADIX;
+        else if (P->axes.size() == 1) P->route = P->axes[0].kind == MIFFT_AXIS_RADER ? MIFFT_ROUTE_RADER : MIFFT_ROUTE_BLUESTEIN;
+        else P->route = MIFFT_ROUTE_AXIS_SEQUENCE;
+    }
+    hipStream_t s;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "plan stream");
+    Builder B{*P, s, {}};
+    try {
+        const size_t cb = elem_bytes(P->prec);
+        if (kind == MIFFT_C2C) {
+            P->in_bytes = P->out_bytes = total * cb;
+            if (total == 1) {
+                Op op;
+                op.kind = OP_SCALE_COPY;
+                op.prec = P->prec;
+                op.count = 1;
+                op.scale = scale;
+                op.desc = "direct N=1";
+                P->ops.push_back(op);
+            } else {
+                uint64_t ws = 0;
+                int src = B_IN;
+                for (int k = 0; k < cfg.naxes; k++) {
+                    const int axis = cfg.axes[k];
+                    uint64_t S = 1;
+                    for (int d = 0; d < axis; d++) S *= cfg.shape[d];
+                    const uint64_t n = cfg.shape[axis];
+                    const uint64_t O = total / (S * n);
+                    const bool last = k + 1 == cfg.naxes;
+                    ws = std::max(ws, B.axis_ops(n, P->axes[k].kind, S, O, src, 0, B_OUT, 0, 0,
+                                                 inverse, last ? scale : 1.0));
+                    src = B_OUT;
+                }
+                P->ws_bytes = ws;
+            }
+        } else {
+            // Real transforms over shape [N0, N1, ...]: the packed spectrum is
+            // [N0/2+1, N1, ...] with axis 0 contiguous (src/math/mod.rs:444-502).
+            const uint64_t N0 = cfg.shape[0];
+            const uint64_t P0 = N0 / 2 + 1;
+            const uint64_t outer = total / N0;  // lines of axis 0 (all batches)
+            const uint64_t packed = P0 * outer;
+            if (kind == MIFFT_R2C) {
+                P->in_bytes = total * 4;
+                P->out_bytes = packed * 8;
+            } else {
+                P->in_bytes = packed * 8;
+                P->out_bytes = total * 4;
+            }
+            if (P->axes[0].kind != MIFFT_AXIS_MIXED)
+                fail(MIFFT_ERR_UNSUPPORTED_LENGTH,
+                     "real transforms need a {2,3,5,7,11,13}-smooth axis 0 (length " + std::to_string(N0) + ")");
+            const LineSet l0{(uint32_t)outer, 1, 1};
+            const Layout real_l{0, 0, (int64_t)N0, 1};
+            const Layout pack_l{0, 0, (int64_t)P0, 1};
+            uint64_t ws = 0;
+            if (kind == MIFFT_R2C) {
+                if (!B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false,
+                               cfg.rank == 1 ? scale : 1.0, LD_REAL, ST_PACKED))
+                    fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "axis 0 of a real transform must fit on chip");
+                uint64_t S = P0;
+                for (int axis = 1; axis < cfg.rank; axis++) {
+                    const uint64_t n = cfg.shape[axis];
+                    const uint64_t O = packed / (S * n);
+                    ws = std::max(ws, B.axis_ops(n, P->axes[axis].kind, S, O, B_OUT, 0, B_OUT, 0, 0,
+                                                 false, axis + 1 == cfg.rank ? scale : 1.0));
+                    S *= n;
+                }
+            } else {
+                int src = B_IN;
+                uint64_t src_off = 0;
+                const uint64_t pk_bytes = packed * 8;
+                if (cfg.rank > 1) {
+                    uint64_t S = P0;
+                    for (int axis = 1; axis < cfg.rank; axis++) {
+                        const uint64_t n = cfg.shape[axis];
+                        const uint64_t O = packed / (S * n);
+                        ws = std::max(ws, B.axis_ops(n, P->axes[axis].kind, S, O, src, src_off, B_WS,
+                                                     0, pk_bytes, true, 1.0));
+                        src = B_WS;
+                        src_off = 0;
+                        S *= n;
+                    }
+                    ws += pk_bytes;
+                }
+                if (!B.line_op(N0, l0, pack_l, real_l, src, src_off, B_OUT, 0, true, true, scale,
+                               LD_HERMITIAN, ST_REAL))
+                    fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "axis 0 of a real transform must fit on chip");
+            }
+            P->ws_bytes = ws;
+        }
+        hip_check(hipStreamSynchronize(s), "plan tables");
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    hip_check(hipStreamDestroy(s), "plan stream");
+    // In-place execution is safe when every op of the plan reads its whole
+    // line set before writing it (single-op axes) or goes through the
+    // workspace (four-step, multi-pass Bluestein); real plans change size.
+    P->in_place_kernels = kind == MIFFT_C2C;
+    return P;
+}
+
+void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws) {
+    auto ptr = [&](int ref, uint64_t off) -> char* {
+        char* b = ref == B_IN ? (char*)in : ref == B_OUT ? (char*)out : (char*)ws;
+        return b + off;
+    };
+    for (const Op& op : p.ops) {
+        switch (op.kind) {
+            case OP_REG: {
+                LineArgs a = op.a;
+                a.in = ptr(op.src, op.src_off);
+                a.out = ptr(op.dst, op.dst_off);
+                hip_check(launch_reg(op.rv, a, s), "register line kernel");
+                break;
+            }
+            case OP_LDS: {
+                LineArgs a = op.a;
+                a.in = ptr(op.src, op.src_off);
+                a.out = ptr(op.dst, op.dst_off);
+                hip_check(launch_lds(op.prec, op.ldm, op.stm, op.post, a, op.sc, s), "LDS line kernel");
+                break;
+            }
+            case OP_RADER:
+            case OP_BLUESTEIN: {
+                ConvArgs c = op.conv;
+                c.io.in = ptr(op.src, op.src_off);
+                c.io.out = ptr(op.dst, op.dst_off);
+                hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
+                          "convolution kernel");
+                break;
+            }
+            case OP_CHIRP_PRE: {
+                LineArgs a = op.a;
+                a.in = ptr(op.src, op.src_off);
+                a.out = ptr(op.dst, op.dst_off);
+                hip_check(launch_chirp_pre(op.prec, a, op.chirp, op.n, op.m, s), "chirp pre");
+                break;
+            }
+            case OP_POINTWISE:
+                hip_check(launch_pointwise(op.prec, ptr(op.dst, op.dst_off), op.filt, op.m, op.count, s),
+                          "pointwise");
+                break;
+            case OP_CHIRP_POST: {
+                LineArgs a = op.a;
+                a.in = ptr(op.src, op.src_off);
+                a.out = ptr(op.dst, op.dst_off);
+                hip_check(launch_chirp_post(op.prec, a, op.chirp, op.n, op.m, s), "chirp post");
+                break;
+            }
+            case OP_SUBPLAN: {
+                char* b = ptr(op.dst, op.dst_off);
+                run_plan(*op.sub, s, b, b, (char*)ws + op.sub_ws_off);
+                break;
+            }
+            case OP_SCALE_COPY:
+                hip_check(launch_scale_copy(op.prec, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off),
+                                            op.count, op.scale, s),
+                          "scale copy");
+                break;
+        }
+    }
+}
+
+static int count_launches(const Plan& p) {
+    int c = 0;
+    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? count_launches(*op.sub) : 1;
+    return c;
+}
+
+std::string describe_plan(const Plan& p) {
+    std::ostringstream d;
+    bool first = true;
+    for (const Op& op : p.ops) {
+        if (!first) d << "; ";
+        first = false;
+        d << op.desc;
+    }
+    (void)count_launches;
+    return d.str();
+}
+
+}  // namespace mifft

@@ -1,0 +1,107 @@
+// Internal plan representation of the host runtime.
+#pragma once
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mifft.h"
+#include "launch.h"
+
+namespace mifft {
+
+// Thrown inside the library, converted to mifft_status at the C boundary.
+struct Error {
+    mifft_status code;
+    std::string detail;
+};
+[[noreturn]] void fail(mifft_status code, const std::string& detail);
+void hip_check(hipError_t e, const char* what);
+
+enum BufRef { B_IN = 0, B_OUT = 1, B_WS = 2 };
+
+enum OpKind {
+    OP_REG,         // compiled register-resident line kernel
+    OP_LDS,         // runtime-schedule LDS line kernel
+    OP_RADER,       // fused Rader convolution
+    OP_BLUESTEIN,   // fused Bluestein convolution
+    OP_CHIRP_PRE,   // multi-pass Bluestein: chirp multiply + zero pad into workspace
+    OP_POINTWISE,   // multi-pass Bluestein: filter multiply (+ swap)
+    OP_CHIRP_POST,  // multi-pass Bluestein: chirp multiply + crop
+    OP_SUBPLAN,     // a child plan run in place on a workspace region
+    OP_SCALE_COPY,  // direct route of a one-point transform
+};
+
+struct Plan;
+
+struct Op {
+    OpKind kind;
+    int src = B_IN, dst = B_OUT;
+    uint64_t src_off = 0, dst_off = 0;  // byte offsets into the referenced buffers
+    int prec = P_F32;                   // kernel format
+    LineArgs a{};
+    Sched sc{};
+    ConvArgs conv{};
+    const RegVariant* rv = nullptr;
+    int ldm = LD_COMPLEX, stm = ST_COMPLEX, post = 0;
+    Plan* sub = nullptr;     // OP_SUBPLAN
+    uint64_t sub_ws_off = 0; // child workspace region
+    uint32_t n = 0, m = 0;
+    const void* chirp = nullptr;
+    const void* filt = nullptr;
+    uint64_t count = 0;
+    double scale = 1.0;
+    std::string desc;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct AxisInfo {
+    int axis;
+    uint64_t n;
+    int kind;  // mifft_axis_kind
+    std::vector<uint64_t> factors;
+};
+
+struct Plan {
+    mifft_config cfg{};
+    mifft_kind kind = MIFFT_C2C;
+    int prec = P_F32;   // storage precision of the plan
+    int arith = P_F32;  // P_F32 or P_F64 (df64 computes in f64)
+    int route = MIFFT_ROUTE_MIXED_RADIX;
+    int device = 0;
+    std::vector<AxisInfo> axes;
+    std::vector<Op> ops;
+    std::vector<DevBuf> owned;
+    std::vector<std::unique_ptr<Plan>> children;
+    uint64_t in_bytes = 0, out_bytes = 0, ws_bytes = 0;
+    bool in_place_kernels = true;
+    // Plan-owned workspace used when the caller passes none (one in-flight
+    // execution per plan, as the reference's plan-owned temporaries).
+    std::mutex ws_mu;
+    DevBuf own_ws;
+    DevBuf in_copy;  // lazily created input copy for in-place real transforms
+    ~Plan();
+};
+
+std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind);
+// Enqueues the plan on `s`.  `ws` must hold ws_bytes when ws_bytes > 0.
+void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws);
+std::string describe_plan(const Plan& p);
+
+// Host-side integer helpers (the reference's route-choice arithmetic).
+std::vector<uint64_t> factor_supported(uint64_t n, bool* ok);  // runtime/mod.rs:33-54
+bool is_prime_u64(uint64_t n);
+uint64_t mod_pow_u64(uint64_t b, uint64_t e, uint64_t m);
+uint64_t primitive_root(uint64_t p);
+uint64_t next_smooth(uint64_t n);
+uint64_t rader_conv_len(uint64_t p);
+uint64_t bluestein_conv_len(uint64_t n);
+void validate_config(const mifft_config& c);
+double config_scale(const mifft_config& c);
+uint64_t config_total(const mifft_config& c);
+
+}  // namespace mifft
