@@ -1,0 +1,255 @@
+"""Benchmark of the FFT hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2c2d|c1|r2c] [--gpus 1]
+
+Default workload: 2D [4096, 2048] f32 C2C forward then inverse, in place (one
+"step" = one forward+inverse pair).  Algorithmic traffic per pair is
+4 * S per FFT (one HBM read + write of the 64 MiB array per axis pass, two
+passes) = 536,870,912 bytes; GFLOP/s counts 5*N*log2(N) per FFT.
+
+The inputs rotate over 8 independent 64 MiB buffers (512 MiB > the 256 MB
+MALL), so every pass streams from HBM; `value` is that HBM-resident rate.  The
+single-buffer (cache-resident, as the reference's README figure) rate is
+reported beside it.  The roofline entry times each kernel of the dominant
+pass live with HIP events on the stream it is launched on.  The CPU baseline
+times the FP64 oracle (oracle/oracle.c, a restatement of the reference's
+CPU/host transforms) on the host cores.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c"])
+    ap.add_argument("--nbuf", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def workload_spec(name):
+    if name == "c2c2d":
+        shape, batch = [4096, 2048], 1
+        n_elems = 4096 * 2048
+        return dict(shape=shape, batch=batch, kind="c2c_pair", elems=n_elems,
+                    bytes_per_step=4 * 8 * n_elems * 2,  # 4*S per FFT, two FFTs
+                    flops_per_step=2 * 5 * n_elems * 23,
+                    label="2D C2C f32 [4096,2048] forward+inverse in place")
+    if name == "c1":
+        n, b = 1024, 4096
+        return dict(shape=[n], batch=b, kind="c2c_fwd", elems=n * b, bytes_per_step=2 * 8 * n * b,
+                    flops_per_step=5 * n * 10 * b,
+                    label="1D C2C f32 N=1024 batch=4096 forward")
+    if name == "r2c":
+        n = 2048
+        s_r, s_p = 4 * n * n, 8 * (n // 2 + 1) * n
+        return dict(shape=[n, n], batch=1, kind="r2c_c2r", elems=n * n,
+                    bytes_per_step=2 * (s_r + 3 * s_p),
+                    flops_per_step=2 * int(2.5 * n * n * 22),
+                    label="2D R2C then C2R f32 [2048,2048]")
+    raise ValueError(name)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from wgpufft_amd import FftConfig, FftPlan, fill_normal_f32, fill_uniform_f32
+
+    spec = workload_spec(args.workload)
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    shape, batch = spec["shape"], spec["batch"]
+    plans = []
+    if spec["kind"] == "c2c_pair":
+        fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch))
+        inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch))
+        plans = [fwd, inv]
+        nbytes = fwd.required_buffer_size_bytes()
+    elif spec["kind"] == "c2c_fwd":
+        fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch))
+        plans = [fwd]
+        nbytes = fwd.required_buffer_size_bytes()
+    else:
+        r2c = FftPlan.r2c(FftConfig.new_nd(shape))
+        c2r = FftPlan.c2r(FftConfig.inverse_nd(shape))
+        plans = [r2c, c2r]
+        nbytes = r2c.required_input_buffer_size_bytes()
+
+    nbuf = max(1, args.nbuf)
+    bufs = [torch.empty(nbytes, dtype=torch.uint8, device=dev) for _ in range(nbuf)]
+    for i, b in enumerate(bufs):
+        if spec["kind"] == "r2c_c2r":
+            fill_normal_f32(b, 3 + i)
+        else:
+            fill_uniform_f32(b, 2 + i)
+    aux = None
+    if spec["kind"] == "r2c_c2r":
+        aux = [torch.empty(plans[0].required_output_buffer_size_bytes(), dtype=torch.uint8, device=dev)
+               for _ in range(nbuf)]
+    torch.cuda.synchronize()
+
+    def step(i, single=False):
+        k = 0 if single else i % nbuf
+        b = bufs[k]
+        if spec["kind"] == "c2c_pair":
+            plans[0].execute_in_place(b, stream)
+            plans[1].execute_in_place(b, stream)
+        elif spec["kind"] == "c2c_fwd":
+            plans[0].execute_in_place(b, stream)
+        else:
+            plans[0].execute(b, aux[k], stream)
+            plans[1].execute(aux[k], b, stream)
+
+    def timed(steps, single):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(steps):
+            step(i, single)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps  # ms per step
+
+    # one normalised forward+inverse step must return its input (GPU-only check)
+    rt_err = None
+    if spec["kind"] in ("c2c_pair", "r2c_c2r"):
+        ref0 = bufs[0].clone()
+        step(0)
+        torch.cuda.synchronize()
+        a = ref0.view(torch.float32).double()
+        bcur = bufs[0].view(torch.float32).double()
+        rt_err = float(torch.linalg.norm(bcur - a) / torch.linalg.norm(a))
+        bufs[0].copy_(ref0)
+        del ref0
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ms = timed(args.steps, single=False)
+    ms_single = timed(args.steps, single=True)
+    value = spec["bytes_per_step"] / (ms * 1e-3) / 1e9
+    value_single = spec["bytes_per_step"] / (ms_single * 1e-3) / 1e9
+    gflops = spec["flops_per_step"] / (ms * 1e-3) / 1e9
+
+    # live per-kernel timing of every launch of the first plan (rotating buffers)
+    kernels = []
+    reps = max(20, min(200, args.steps))
+    p0 = plans[0]
+    for li in range(p0.launch_count()):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            k = r % nbuf
+            if spec["kind"] == "r2c_c2r":
+                p0.execute_launch(li, bufs[k], aux[k], None, stream)
+            else:
+                p0.execute_launch(li, bufs[k], bufs[k], None, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        kernels.append(dict(launch=li, desc=p0.launch_describe(li), us=e0.elapsed_time(e1) / reps * 1e3))
+    # algorithmic bytes of one axis pass: read + write of the array once
+    if spec["kind"] == "r2c_c2r":
+        n = shape[0]
+        pass_bytes = [4 * n * n + 8 * (n // 2 + 1) * n, 2 * 8 * (n // 2 + 1) * n]
+    else:
+        pass_bytes = [2 * 8 * spec["elems"]] * len(kernels)
+    for kinfo, pb in zip(kernels, pass_bytes):
+        kinfo["alg_bytes"] = pb
+        kinfo["gbs"] = pb / (kinfo["us"] * 1e-6) / 1e9
+    dom = max(kernels, key=lambda k: k["us"])
+    roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom["desc"], "kernel_us": round(dom["us"], 2),
+                "alg_bytes_per_launch": dom["alg_bytes"]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(spec)
+
+    line = {
+        "metric": "HBM GB/s (% of 8 TB/s) 4096x2048 f32 C2C; mixed-size serve p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "pct_of_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (SplitMix64 uniform[-1,1) generated on device)",
+        "config": {"workload": spec["label"], "shape": shape, "batch": batch,
+                   "buffers": nbuf, "buffer_mib": nbytes / 2**20,
+                   "alg_bytes_per_step": spec["bytes_per_step"]},
+        "value_single_buffer_cache_resident": round(value_single, 1),
+        "ms_per_step_single_buffer": round(ms_single, 5),
+        "gflops": round(gflops, 1),
+        "round_trip_rel_l2": rt_err,
+        "kernels": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(spec):
+    """FP64 oracle (oracle/oracle.c) on the host: one bounded sample of the workload."""
+    import numpy as np
+
+    import oracle
+    ncores = min(os.cpu_count() or 1, 16)
+    shape, batch = spec["shape"], spec["batch"]
+    t0 = time.perf_counter()
+    if spec["kind"] == "c2c_pair":
+        x = oracle.uniform_f32(2, 2 * spec["elems"]).view(np.complex64).astype(np.complex128)
+        t0 = time.perf_counter()
+        y = oracle.c2c_nd(x, shape, batch=batch, nthreads=ncores)
+        oracle.c2c_nd(y, shape, batch=batch, inverse=True, nthreads=ncores)
+        sample = "one forward+inverse pair of the same [4096,2048] workload in FP64"
+    elif spec["kind"] == "c2c_fwd":
+        x = oracle.uniform_f32(1, 2 * spec["elems"]).view(np.complex64).astype(np.complex128)
+        t0 = time.perf_counter()
+        oracle.c2c_nd(x, shape, batch=batch, nthreads=ncores)
+        sample = "one forward FFT of the same N=1024 x 4096 workload in FP64"
+    else:
+        x = oracle.normal_f32(3, spec["elems"]).astype(np.float64)
+        t0 = time.perf_counter()
+        p = oracle.r2c_nd(x, shape, nthreads=ncores)
+        oracle.c2r_nd(p, shape, nthreads=ncores)
+        sample = "one R2C+C2R pair of the same [2048,2048] workload in FP64"
+    dt = time.perf_counter() - t0
+    return {"value": round(spec["bytes_per_step"] / dt / 1e9, 3), "unit": "GB/s",
+            "ms_per_step": round(dt * 1e3, 1), "cores": ncores, "kind": "port",
+            "sample": sample + f" (oracle/oracle.c, {ncores} threads)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -52,7 +52,9 @@ typedef enum mifft_status {
     MIFFT_ERR_HIP = 20,                        /* a HIP runtime call failed */
     MIFFT_ERR_RECORDER_STATE = 21,             /* recorder used after end */
     MIFFT_ERR_OVER_BUDGET = 22,                /* server admission: request exceeds the memory budget */
-    MIFFT_ERR_BUSY = 23                        /* server: no free slot, try again */
+    MIFFT_ERR_BUSY = 23,                       /* server: no free slot, try again */
+    MIFFT_ERR_UNALIGNED = 24                   /* FftError::BufferViewOffsetUnaligned: device
+                                                  buffers must be 16-byte aligned */
 } mifft_status;
 
 /* FftTransformKind (plan.rs:24-29). */
@@ -154,6 +156,14 @@ int32_t mifft_plan_supports_in_place(const mifft_plan* p);
  * of them ("reg N=4096 16x16x16 lines=2048; ..."), for diagnostics. */
 int32_t mifft_plan_launch_count(const mifft_plan* p);
 mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap);
+/* Text of launch `index` (0 <= index < mifft_plan_launch_count). */
+mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char* buf, size_t cap);
+/* Enqueues only launch `index` of one execution (no counterpart in the
+ * reference: per-kernel timing for roofline measurement).  The buffers are
+ * those the full execution would use; `workspace` may be NULL to use the
+ * plan's own. */
+mifft_status mifft_execute_launch(const mifft_plan* p, hipStream_t stream, int32_t index, const void* in,
+                                  void* out, void* workspace);
 
 /* ------------------------------------------------------------ execution */
 /* FftPlan::execute_checked (plan.rs:510-518): input -> output on `stream`.

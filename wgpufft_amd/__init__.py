@@ -79,7 +79,7 @@ _STATUS_NAMES = {
     12: "BufferViewTooSmall", 13: "WorkspaceTooSmall", 14: "InPlaceUnsupported",
     15: "GpuPlanResourceAllocationFailed", 16: "HostBufferLength", 17: "NullArgument",
     18: "InvalidEnum", 19: "RankTooLarge", 20: "Hip", 21: "RecorderState", 22: "OverBudget",
-    23: "Busy",
+    23: "Busy", 24: "BufferViewOffsetUnaligned",
 }
 
 
@@ -362,6 +362,18 @@ class FftPlan:
         buf = ctypes.create_string_buffer(8192)
         _check(self._lib.mifft_plan_describe(self._h, buf, 8192))
         return buf.value.decode()
+
+    def launch_describe(self, index: int) -> str:
+        buf = ctypes.create_string_buffer(1024)
+        _check(self._lib.mifft_plan_launch_describe(self._h, int(index), buf, 1024))
+        return buf.value.decode()
+
+    def execute_launch(self, index: int, input, output, workspace=None, stream=None) -> None:
+        """Enqueue only launch `index` of one execution (per-kernel timing)."""
+        ip, _ = _buf(input)
+        op, _ = _buf(output)
+        wp = _buf(workspace)[0] if workspace is not None else None
+        _check(self._lib.mifft_execute_launch(self._h, _stream(stream), int(index), ip, op, wp))
 
     # execution (plan.rs:494-836)
     def execute_checked(self, input, output, stream=None) -> None:

@@ -156,12 +156,27 @@ int32_t mifft_plan_packed_shape(const mifft_plan* p, uint64_t* out, int32_t cap)
 
 int32_t mifft_plan_supports_in_place(const mifft_plan* p) { return p && p->p->in_place_kernels ? 1 : 0; }
 
-static int32_t launches(const Plan& p) {
-    int32_t c = 0;
-    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? launches(*op.sub) : 1;
-    return c;
+int32_t mifft_plan_launch_count(const mifft_plan* p) { return p ? plan_launch_count(*p->p) : 0; }
+
+mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char* buf, size_t cap) {
+    if (!p || !buf || cap == 0) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    std::string d;
+    if (!run_plan_launch(*p->p, nullptr, false, index, nullptr, nullptr, nullptr, &d))
+        return set_err(MIFFT_ERR_INVALID_ENUM, "launch index out of range");
+    std::strncpy(buf, d.c_str(), cap - 1);
+    buf[cap - 1] = '\0';
+    return MIFFT_SUCCESS;
 }
-int32_t mifft_plan_launch_count(const mifft_plan* p) { return p ? launches(*p->p) : 0; }
+
+mifft_status mifft_execute_launch(const mifft_plan* p, hipStream_t stream, int32_t index, const void* in,
+                                  void* out, void* workspace) {
+    if (!p || !in || !out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    return guard([&] {
+        if (!run_plan_launch(*p->p, stream, true, index, in, out, workspace ? workspace : p->p->own_ws.p,
+                             nullptr))
+            fail(MIFFT_ERR_INVALID_ENUM, "launch index out of range");
+    });
+}
 
 mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap) {
     if (!p || !buf || cap == 0) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
@@ -174,6 +189,8 @@ mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap) {
 static void check_sizes(const Plan& P, const void* in, uint64_t in_bytes, const void* out,
                         uint64_t out_bytes) {
     if (!in || !out) fail(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    if (((uintptr_t)in | (uintptr_t)out) & 15u)
+        fail(MIFFT_ERR_UNALIGNED, "device buffers must be 16-byte aligned");
     if (in_bytes < P.in_bytes)
         fail(MIFFT_ERR_BUFFER_TOO_SMALL, "input needs " + std::to_string(P.in_bytes) + " bytes, got " +
                                              std::to_string(in_bytes));
@@ -308,6 +325,7 @@ const char* mifft_status_string(mifft_status s) {
         case MIFFT_ERR_RECORDER_STATE: return "recorder state";
         case MIFFT_ERR_OVER_BUDGET: return "over memory budget";
         case MIFFT_ERR_BUSY: return "busy";
+        case MIFFT_ERR_UNALIGNED: return "buffer not 16-byte aligned";
     }
     return "unknown status";
 }

@@ -84,14 +84,12 @@ __device__ __forceinline__ cpx<typename Fmt::T> load_point(const LineArgs& a, in
         // Hermitian extension of a packed half line (n/2 + 1 points): the
         // imaginary part of the self-conjugate bins 0 and n/2 is dropped and the
         // upper half mirrors the conjugate (src/math/mod.rs:470-502).
-        uint32_t h = n >> 1;
-        if (p <= h) {
-            v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)p * a.in_sn));
-            if (p == 0 || 2 * p == n) v.y = T(0);
-        } else {
-            v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)(n - p) * a.in_sn));
-            v.y = -v.y;
-        }
+        // Branch-free so that consecutive loads stay in flight together.
+        const uint32_t h = n >> 1;
+        const bool up = p > h;
+        const uint32_t idx = up ? n - p : p;
+        v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)idx * a.in_sn));
+        v.y = (p == 0 || 2 * p == n) ? T(0) : (up ? -v.y : v.y);
     }
     if (a.swap_in) v = swap_ri(v);
     return v;
@@ -125,25 +123,99 @@ struct RadixList {
         return p;
     }
     static constexpr int prod() { return lprod(n); }
+    // Offset of stage s in the stage-twiddle table (see stage_twiddle_index).
+    static constexpr int twoff(int s) {
+        int o = 0;
+        for (int i = 1; i < s; i++) o += (r[i] - 1) * lprod(i);
+        return o;
+    }
 };
+
+// Stage-twiddle table of a radix schedule R_0..R_{S-1}: for s >= 1 a block of
+// (R_s - 1) * L_s entries, entry (m-1)*L_s + k = W_{L_s R_s}^(k m).  Adjacent
+// butterflies (adjacent k) read adjacent entries, so a wavefront's twiddle loads
+// are contiguous.  N - R_0 entries in all.
 
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 constexpr int lds_pad(int p) { return p + (p >> 4); }
 
+// Two adjacent lines (line g even and g+1, adjacent in memory) in one 16-byte
+// access per point; the host guarantees even strides and 16-byte alignment.
+template <class Fmt>
+__device__ __forceinline__ void load_pair(const LineArgs& a, int64_t base, uint32_t p,
+                                          cpx<typename Fmt::T>& v0, cpx<typename Fmt::T>& v1) {
+    const uint64_t i = (uint64_t)(base + (int64_t)p * a.in_sn);
+    if constexpr (Fmt::kId == 0) {
+        const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(a.in) + i);
+        v0 = mk<float>(q.x, q.y);
+        v1 = mk<float>(q.z, q.w);
+    } else {
+        v0 = Fmt::ld(a.in, i);
+        v1 = Fmt::ld(a.in, i + 1);
+    }
+    if (a.swap_in) {
+        v0 = swap_ri(v0);
+        v1 = swap_ri(v1);
+    }
+}
+
+template <class Fmt, bool POST>
+__device__ __forceinline__ void store_pair(const LineArgs& a, int64_t base, uint32_t jline,
+                                           uint32_t k, cpx<typename Fmt::T> v0,
+                                           cpx<typename Fmt::T> v1) {
+    using T = typename Fmt::T;
+    if constexpr (POST) {
+        const cpx<T> w = post_twiddle<T>(a, (uint64_t)jline * k);
+        v0 = cmul(v0, w);
+        v1 = cmul(v1, w);
+    }
+    if (a.swap_out) {
+        v0 = swap_ri(v0);
+        v1 = swap_ri(v1);
+    }
+    const T s = (T)a.scale;
+    v0 = scale(v0, s);
+    v1 = scale(v1, s);
+    const uint64_t i = (uint64_t)(base + (int64_t)k * a.out_sk);
+    if constexpr (Fmt::kId == 0) {
+        *reinterpret_cast<float4*>(reinterpret_cast<float2*>(a.out) + i) = make_float4(v0.x, v0.y, v1.x, v1.y);
+    } else {
+        Fmt::st(a.out, i, v0);
+        Fmt::st(a.out, i + 1, v1);
+    }
+}
+
 // Template parameters:
-//   N, TPL (threads per line), LINES (lines per workgroup);
+//   N, TPL (threads per line group), LINES (lines per workgroup);
 //   LD_LF / ST_LF: stage-0 loads / last-stage stores map adjacent lanes to
 //   adjacent lines (strided axes) instead of adjacent points (contiguous axes);
-//   LDM / STM: load and store modes; POST: four-step twiddle on store.
+//   LDM / STM: load and store modes; POST: four-step twiddle on store;
+//   VEC: adjacent lines per thread (2: one 16-byte access covers both lines;
+//   the LDS exchange then runs in VEC rounds through one slot per line group).
+//   PF: load every stage's twiddles into registers together with the data, so
+//   their latency overlaps the data loads instead of following each exchange.
 template <class Fmt, int N, int TPL, int LINES, bool LD_LF, bool ST_LF, int LDM, int STM, bool POST,
-          class RL>
+          class RL, int VEC = 1, bool PF = false>
 struct RegFft {
     using T = typename Fmt::T;
     static constexpr int NS = RL::n;
-    static constexpr int NT = TPL * LINES;
+    // Twiddle registers of stage s: NB_s butterflies x (R_s - 1).
+    static constexpr int tw_count(int s) {
+        return s >= NS ? 0 : (s == 0 ? 0 : ceil_div(N / RL::at(s), TPL) * (RL::at(s) - 1)) + tw_count(s + 1);
+    }
+    static constexpr int tw_base(int s) {
+        int b = 0;
+        for (int i = 1; i < s; i++) b += ceil_div(N / RL::at(i), TPL) * (RL::at(i) - 1);
+        return b;
+    }
+    static constexpr int TWREG = PF ? (tw_count(0) > 0 ? tw_count(0) : 1) : 1;
+    static constexpr int kLines = LINES;
+    static constexpr int GROUPS = LINES / VEC;
+    static constexpr int NT = TPL * GROUPS;
     static constexpr int LSTR = lds_pad(N) + 1;
+    static_assert(VEC == 1 || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
     static constexpr size_t lds_bytes() {
-        return NS > 1 ? (size_t)LINES * LSTR * sizeof(cpx<T>) : 0;
+        return NS > 1 ? (size_t)GROUPS * LSTR * sizeof(cpx<T>) : 0;
     }
     static constexpr int max_regs(int s) {
         return s >= NS ? 0
@@ -153,98 +225,154 @@ struct RegFft {
     }
     static constexpr int EMAX = max_regs(0);
 
+    struct Map {
+        uint32_t lg, t;  // line group within the workgroup, thread within the group
+    };
+    // Stage 0 follows the load layout, the last stage the store layout; the
+    // stages in between exchange through LDS, so their mapping is free.
+    template <int s>
+    static __device__ __forceinline__ Map map_of() {
+        constexpr bool LF = (s == 0) ? LD_LF : (s == NS - 1 ? ST_LF : false);
+        const uint32_t tid = threadIdx.x;
+        if constexpr (LF) return Map{tid % GROUPS, tid / GROUPS};
+        else return Map{tid / TPL, tid % TPL};
+    }
+
+    // Outputs of stage s -> LDS -> inputs of stage s+1, one round per line of
+    // the group (VEC rounds through GROUPS slots).
+    template <int s>
+    static __device__ __forceinline__ void exchange(cpx<T>* lds, cpx<T> (&w)[VEC][EMAX]) {
+        constexpr int R = RL::at(s), L = RL::lprod(s), NR = N / R, NB = ceil_div(NR, TPL);
+        constexpr int R2 = RL::at(s + 1), NR2 = N / R2, NB2 = ceil_div(NR2, TPL);
+        const Map mw = map_of<s>();
+        const Map mr = map_of<s + 1>();
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            cpx<T>* ln = lds + mw.lg * LSTR;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const uint32_t j = mw.t + q * TPL;
+                if (NR % TPL == 0 || j < (uint32_t)NR) {
+                    const uint32_t k = j % L;
+#pragma unroll
+                    for (int m = 0; m < R; m++) ln[lds_pad((j - k) * R + k + m * L)] = w[v][q * R + m];
+                }
+            }
+            __syncthreads();
+            const cpx<T>* lr = lds + mr.lg * LSTR;
+#pragma unroll
+            for (int q = 0; q < NB2; q++) {
+                const uint32_t j = mr.t + q * TPL;
+                if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
+#pragma unroll
+                    for (int m = 0; m < R2; m++) w[v][q * R2 + m] = lr[lds_pad(j + m * NR2)];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    template <int s>
+    static __device__ __forceinline__ void prefetch(const LineArgs& a, cpx<T> (&twr)[TWREG]) {
+        if constexpr (s < NS) {
+            constexpr int R = RL::at(s), L = RL::lprod(s), NR = N / R, NB = ceil_div(NR, TPL);
+            constexpr uint32_t off = RL::twoff(s);
+            const Map mp = map_of<s>();
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const uint32_t j = mp.t + q * TPL;
+                const uint32_t k = (NR % TPL == 0 || j < (uint32_t)NR) ? j % L : 0;
+#pragma unroll
+                for (int m = 1; m < R; m++)
+                    twr[tw_base(s) + q * (R - 1) + m - 1] = ld_tw<T>(a.tw, off + (m - 1) * L + k);
+            }
+            prefetch<s + 1>(a, twr);
+        }
+    }
+
     template <int s>
     static __device__ __forceinline__ void stage(const LineArgs& a, cpx<T>* lds, uint32_t g0,
-                                                 cpx<T> (&w)[EMAX]) {
+                                                 cpx<T> (&w)[VEC][EMAX], cpx<T> (&twr)[TWREG]) {
         if constexpr (s < NS) {
             constexpr int R = RL::at(s);
             constexpr int L = RL::lprod(s);
             constexpr int NR = N / R;
             constexpr int NB = ceil_div(NR, TPL);
             constexpr bool FIRST = (s == 0), LAST = (s == NS - 1);
-            constexpr bool LF = FIRST ? LD_LF : (LAST ? ST_LF : false);
-            const uint32_t tid = threadIdx.x;
-            const uint32_t l = LF ? (tid % LINES) : (tid / TPL);
-            const uint32_t t = LF ? (tid / LINES) : (tid % TPL);
-            const uint32_t g = g0 + l;
+            const Map mp = map_of<s>();
+            const uint32_t g = g0 + mp.lg * VEC;
             const bool valid = g < a.lines;
-            // gather
             if constexpr (FIRST) {
-                LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
+                // Branch-free loads: a line group past the end reads the last valid
+                // group and a butterfly past N/R reads point N/R-1; those values are
+                // never stored.  A conditional load per point would make the
+                // compiler wait for each load before issuing the next.
+                const uint32_t gl = valid ? g : (a.lines - VEC);
+                LineBase b = line_base(a, gl);
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
-                    const uint32_t j = t + q * TPL;
+                    uint32_t j = mp.t + q * TPL;
+                    if constexpr (NR % TPL != 0) j = j < (uint32_t)NR ? j : (uint32_t)(NR - 1);
 #pragma unroll
                     for (int m = 0; m < R; m++) {
-                        if (valid && (NR % TPL == 0 || j < (uint32_t)NR))
-                            w[q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
+                        if constexpr (VEC == 1)
+                            w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
                         else
-                            w[q * R + m] = mk<T>(T(0), T(0));
+                            load_pair<Fmt>(a, b.in, j + m * NR, w[0][q * R + m], w[1][q * R + m]);
                     }
                 }
-            } else {
-                const cpx<T>* ln = lds + l * LSTR;
-#pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    const uint32_t j = t + q * TPL;
-                    if (NR % TPL == 0 || j < (uint32_t)NR) {
-#pragma unroll
-                        for (int m = 0; m < R; m++) w[q * R + m] = ln[lds_pad(j + m * NR)];
-                    }
-                }
-                if constexpr (!LAST) __syncthreads();
+                if constexpr (PF) prefetch<1>(a, twr);
             }
-            // twiddle + butterfly
+            // twiddle + butterfly (one twiddle load serves the VEC lines)
 #pragma unroll
             for (int q = 0; q < NB; q++) {
-                const uint32_t j = t + q * TPL;
+                const uint32_t j = mp.t + q * TPL;
                 if (NR % TPL == 0 || j < (uint32_t)NR) {
                     if constexpr (L > 1) {
                         const uint32_t k = j % L;
-                        constexpr uint32_t step = N / (L * R);
+                        constexpr uint32_t off = RL::twoff(s);
 #pragma unroll
-                        for (int m = 1; m < R; m++)
-                            w[q * R + m] = cmul(w[q * R + m], ld_tw<T>(a.tw, k * m * step));
+                        for (int m = 1; m < R; m++) {
+                            cpx<T> tw;
+                            if constexpr (PF) tw = twr[tw_base(s) + q * (R - 1) + m - 1];
+                            else tw = ld_tw<T>(a.tw, off + (m - 1) * L + k);
+#pragma unroll
+                            for (int v = 0; v < VEC; v++) w[v][q * R + m] = cmul(w[v][q * R + m], tw);
+                        }
                     }
-                    Dft<R, T>::run(&w[q * R]);
+#pragma unroll
+                    for (int v = 0; v < VEC; v++) Dft<R, T>::run(&w[v][q * R]);
                 }
             }
-            // scatter
             if constexpr (LAST) {
                 LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
-                    const uint32_t j = t + q * TPL;
+                    const uint32_t j = mp.t + q * TPL;
                     if (valid && (NR % TPL == 0 || j < (uint32_t)NR)) {
                         const uint32_t k = j % L;
 #pragma unroll
-                        for (int m = 0; m < R; m++)
-                            store_point<Fmt, STM, POST>(a, b.out, b.j, (j - k) * R + k + m * L, N,
-                                                        w[q * R + m]);
+                        for (int m = 0; m < R; m++) {
+                            const uint32_t p = (j - k) * R + k + m * L;
+                            if constexpr (VEC == 1)
+                                store_point<Fmt, STM, POST>(a, b.out, b.j, p, N, w[0][q * R + m]);
+                            else
+                                store_pair<Fmt, POST>(a, b.out, b.j, p, w[0][q * R + m], w[1][q * R + m]);
+                        }
                     }
                 }
             } else {
-                cpx<T>* ln = lds + l * LSTR;
-#pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    const uint32_t j = t + q * TPL;
-                    if (NR % TPL == 0 || j < (uint32_t)NR) {
-                        const uint32_t k = j % L;
-#pragma unroll
-                        for (int m = 0; m < R; m++)
-                            ln[lds_pad((j - k) * R + k + m * L)] = w[q * R + m];
-                    }
-                }
-                __syncthreads();
-                stage<s + 1>(a, lds, g0, w);
+                exchange<s>(lds, w);
+                stage<s + 1>(a, lds, g0, w, twr);
             }
         }
     }
 
     static __device__ __forceinline__ void run(const LineArgs& a, cpx<T>* lds) {
         const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
-        cpx<T> w[EMAX];
-        stage<0>(a, lds, bid * LINES, w);
+        cpx<T> w[VEC][EMAX];
+        cpx<T> twr[TWREG];
+        stage<0>(a, lds, bid * LINES, w, twr);
     }
 };
 
@@ -266,8 +394,9 @@ struct Sched {
     DivMagic dlpw;  // divide by lpw
     uint8_t rad[kMaxStages];
     uint32_t L[kMaxStages];
-    DivMagic dnr[kMaxStages];  // divide by n / rad
-    DivMagic dl[kMaxStages];   // divide by L
+    uint32_t twoff[kMaxStages];  // stage offsets in the stage-twiddle table
+    DivMagic dnr[kMaxStages];    // divide by n / rad
+    DivMagic dl[kMaxStages];     // divide by L
 };
 
 // LDS line stride (elements) of the runtime-schedule kernels.
@@ -278,10 +407,9 @@ __device__ __forceinline__ uint32_t lpad(uint32_t p) { return p + (p >> 4); }
 template <int R, class T>
 __device__ __forceinline__ void lds_stage(const cpx<T>* src, cpx<T>* dst, uint32_t nl,
                                           uint32_t n, uint32_t lstr, uint32_t L, DivMagic dnr,
-                                          DivMagic dl, const void* tw) {
+                                          DivMagic dl, uint32_t twoff, const void* tw) {
     const uint32_t nr = n / R;
     const uint32_t total = nl * nr;
-    const uint32_t step = n / (L * R);
     for (uint32_t b = threadIdx.x; b < total; b += blockDim.x) {
         const uint32_t l = fdiv(b, dnr);
         const uint32_t j = b - l * nr;
@@ -293,7 +421,7 @@ __device__ __forceinline__ void lds_stage(const cpx<T>* src, cpx<T>* dst, uint32
         for (int m = 0; m < R; m++) w[m] = s[lpad(j + m * nr)];
         if (L > 1) {
 #pragma unroll
-            for (int m = 1; m < R; m++) w[m] = cmul(w[m], ld_tw<T>(tw, k * m * step));
+            for (int m = 1; m < R; m++) w[m] = cmul(w[m], ld_tw<T>(tw, twoff + (m - 1) * L + k));
         }
         Dft<R, T>::run(w);
         cpx<T>* d = dst + l * lstr;
@@ -317,7 +445,7 @@ __device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched&
         switch (R) {
 #define MIFFT_CASE(RR)                                                                     \
     case RR:                                                                               \
-        lds_stage<RR, T>(src, dst, nl, sc.n, lstr, sc.L[s], sc.dnr[s], sc.dl[s], tw); \
+        lds_stage<RR, T>(src, dst, nl, sc.n, lstr, sc.L[s], sc.dnr[s], sc.dl[s], sc.twoff[s], tw); \
         break;
             MIFFT_CASE(2)
             MIFFT_CASE(3)

@@ -297,6 +297,8 @@ struct Dft<16, T> : DftComp<4, 4, T> {};
 template <class T>
 struct Dft<32, T> : DftComp<4, 8, T> {};
 template <class T>
+struct Dft<64, T> : DftComp<8, 8, T> {};
+template <class T>
 struct Dft<6, T> : DftComp<2, 3, T> {};
 template <class T>
 struct Dft<9, T> : DftComp<3, 3, T> {};

@@ -22,7 +22,9 @@ void ensure_lds_attr(const void* fn) {
 }
 
 #define MIFFT_RL(...) RadixList<__VA_ARGS__>
-#define REGV(FMT, PREC, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, RADSTR, ...)                  \
+#define MIFFT_K(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, ...) \
+    RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__), VEC>
+#define REGV(FMT, PREC, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, RADSTR, ...)              \
     {PREC,                                                                                      \
      N,                                                                                         \
      TPL,                                                                                       \
@@ -32,43 +34,48 @@ void ensure_lds_attr(const void* fn) {
      LDM,                                                                                       \
      STM,                                                                                       \
      POST,                                                                                      \
-     TPL * LINES,                                                                               \
-     RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__)>::lds_bytes(), \
-     (const void*)&reg_fft_kernel<                                                              \
-         RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__)>>,        \
+     VEC,                                                                                       \
+     MIFFT_K(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, __VA_ARGS__)::NT,             \
+     MIFFT_K(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, __VA_ARGS__)::lds_bytes(),    \
+     (const void*)&reg_fft_kernel<MIFFT_K(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC,  \
+                                          __VA_ARGS__)>,                                        \
      RADSTR}
 
-// f32 power-of-two lines: 16 points per thread, radix-16 stages.
-#define F32_POW2(LDLF, STLF, LDM, STM, POST, L4096, L2048, L1024, L512, L256, L128, L64)          \
-    REGV(FmtF32, P_F32, 4096, 256, L4096, LDLF, STLF, LDM, STM, POST, "16x16x16", 16, 16, 16),    \
-        REGV(FmtF32, P_F32, 2048, 128, L2048, LDLF, STLF, LDM, STM, POST, "16x16x8", 16, 16, 8),  \
-        REGV(FmtF32, P_F32, 1024, 64, L1024, LDLF, STLF, LDM, STM, POST, "16x16x4", 16, 16, 4),   \
-        REGV(FmtF32, P_F32, 512, 32, L512, LDLF, STLF, LDM, STM, POST, "8x8x8", 8, 8, 8),         \
-        REGV(FmtF32, P_F32, 256, 16, L256, LDLF, STLF, LDM, STM, POST, "16x16", 16, 16),          \
-        REGV(FmtF32, P_F32, 128, 8, L128, LDLF, STLF, LDM, STM, POST, "16x8", 16, 8),             \
-        REGV(FmtF32, P_F32, 64, 4, L64, LDLF, STLF, LDM, STM, POST, "16x4", 16, 4)
+// f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
+#define F32_POW2(LDLF, STLF, LDM, STM, POST, VEC, L4096, L2048, L1024, L512, L256, L128, L64)          \
+    REGV(FmtF32, P_F32, 4096, 256, L4096, LDLF, STLF, LDM, STM, POST, VEC, "16x16x16", 16, 16, 16),    \
+        REGV(FmtF32, P_F32, 2048, 128, L2048, LDLF, STLF, LDM, STM, POST, VEC, "16x16x8", 16, 16, 8),  \
+        REGV(FmtF32, P_F32, 1024, 64, L1024, LDLF, STLF, LDM, STM, POST, VEC, "16x16x4", 16, 16, 4),   \
+        REGV(FmtF32, P_F32, 512, 32, L512, LDLF, STLF, LDM, STM, POST, VEC, "8x8x8", 8, 8, 8),         \
+        REGV(FmtF32, P_F32, 256, 16, L256, LDLF, STLF, LDM, STM, POST, VEC, "16x16", 16, 16),          \
+        REGV(FmtF32, P_F32, 128, 8, L128, LDLF, STLF, LDM, STM, POST, VEC, "16x8", 16, 8),             \
+        REGV(FmtF32, P_F32, 64, 4, L64, LDLF, STLF, LDM, STM, POST, VEC, "16x4", 16, 4)
 
 static const RegVariant kRegistry[] = {
     // contiguous axes (axis 0)
-    F32_POW2(0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, 4, 8, 16, 32, 64),
-    // strided axes: adjacent lanes on adjacent lines, >= 128 B per row segment
-    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 0, 4, 8, 16, 16, 16, 32, 64),
+    F32_POW2(0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, 2, 4, 8, 16, 32, 64),
+    // strided axes, two adjacent lines per thread: 16-byte accesses and
+    // >= 128-byte row segments per workgroup
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 8, 16, 32, 32, 32, 64, 128),
+    // strided axes whose line count is odd (one line per thread)
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, 4, 8, 16, 16, 16, 32, 64),
     // R2C axis 0: real in, packed half spectrum out
-    F32_POW2(0, 0, LD_REAL, ST_PACKED, 0, 1, 2, 4, 8, 16, 32, 64),
+    F32_POW2(0, 0, LD_REAL, ST_PACKED, 0, 1, 1, 2, 4, 8, 16, 32, 64),
     // C2R axis 0: Hermitian-extended packed in, real out
-    F32_POW2(0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, 4, 8, 16, 32, 64),
+    F32_POW2(0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 1, 2, 4, 8, 16, 32, 64),
     // four-step split passes: columns with twiddle on store; rows with transposed store
-    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 4, 8, 16, 16, 16, 32, 64),
-    F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 4, 8, 16, 16, 16, 32, 64),
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, 8, 16, 32, 32, 32, 64, 128),
+    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, 4, 8, 16, 16, 16, 32, 64),
+    F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, 4, 8, 16, 16, 16, 32, 64),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
-    REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "15x8", 15, 8),
-    REGV(FmtF32, P_F32, 343, 49, 5, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "7x7x7", 7, 7, 7),
-    REGV(FmtF32, P_F32, 1001, 77, 3, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "13x11x7", 13, 11, 7),
+    REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "15x8", 15, 8),
+    REGV(FmtF32, P_F32, 343, 49, 5, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),
+    REGV(FmtF32, P_F32, 1001, 77, 3, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "13x11x7", 13, 11, 7),
     // f64 / df64 256-point lines (256^3 volumes)
-    REGV(FmtF64, P_F64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
-    REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
-    REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
-    REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
 };
 
 int reg_variant_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }
@@ -77,10 +84,10 @@ const RegVariant* reg_variant_at(int i) {
 }
 
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                   int post) {
+                                   int post, int vec) {
     for (const RegVariant& v : kRegistry) {
         if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm &&
-            v.stm == stm && v.post == post)
+            v.stm == stm && v.post == post && v.vec == vec)
             return &v;
     }
     return nullptr;

@@ -7,6 +7,8 @@
 // table type.  The reference builds the same values on the host in f64
 // (src/runtime/twiddle.rs:198-217, src/runtime/bluestein_axis.rs:818-867,
 // src/runtime/rader_axis.rs:1318-1336).
+#include <cstring>
+
 #include "launch.h"
 
 namespace mifft {
@@ -53,6 +55,28 @@ __global__ void k_gen_tw(cpx<T>* out, uint64_t n, uint64_t count) {
     GRID_STRIDE(i, count) {
         double2 w = root_f64(i, n);
         out[i] = mk<T>((T)w.x, (T)w.y);
+    }
+}
+
+// Stage-twiddle table of a radix schedule (layout in fft_kernels.hpp):
+// entry off_s + (m-1)*L_s + k = W_{L_s R_s}^(k m).
+struct StageTab {
+    uint32_t ns;
+    uint32_t L[kMaxStages];
+    uint32_t R[kMaxStages];
+    uint64_t off[kMaxStages + 1];
+};
+
+template <class T>
+__global__ void k_gen_stage_tw(cpx<T>* out, StageTab st, uint64_t total) {
+    GRID_STRIDE(e, total) {
+        uint32_t s = 1;
+        while (s + 1 < st.ns && e >= st.off[s + 1]) s++;
+        const uint64_t r = e - st.off[s];
+        const uint64_t L = st.L[s];
+        const uint64_t m = r / L + 1, k = r % L;
+        double2 w = root_f64(k * m, L * st.R[s]);
+        out[e] = mk<T>((T)w.x, (T)w.y);
     }
 }
 
@@ -202,6 +226,37 @@ hipError_t gen_twiddles(int prec, void* out, uint64_t n, uint64_t count, hipStre
         k_gen_tw<float><<<grid_for(count), 256, 0, s>>>((cpx<float>*)out, n, count);
     else
         k_gen_tw<double><<<grid_for(count), 256, 0, s>>>((cpx<double>*)out, n, count);
+    return hipGetLastError();
+}
+
+uint64_t stage_twiddle_count(const int* rad, int ns) {
+    uint64_t L = 1, total = 0;
+    for (int s = 0; s < ns; s++) {
+        if (s >= 1) total += (uint64_t)(rad[s] - 1) * L;
+        L *= (uint64_t)rad[s];
+    }
+    return total;
+}
+
+hipError_t gen_stage_twiddles(int prec, void* out, const int* rad, int ns, hipStream_t s) {
+    if (ns > kMaxStages) return hipErrorInvalidValue;
+    StageTab st;
+    std::memset(&st, 0, sizeof(st));
+    st.ns = (uint32_t)ns;
+    uint64_t L = 1, off = 0;
+    for (int i = 0; i < ns; i++) {
+        st.L[i] = (uint32_t)L;
+        st.R[i] = (uint32_t)rad[i];
+        st.off[i] = off;
+        if (i >= 1) off += (uint64_t)(rad[i] - 1) * L;
+        L *= (uint64_t)rad[i];
+    }
+    st.off[ns] = off;
+    if (off == 0) return hipSuccess;
+    if (prec == P_F32)
+        k_gen_stage_tw<float><<<grid_for(off), 256, 0, s>>>((cpx<float>*)out, st, off);
+    else
+        k_gen_stage_tw<double><<<grid_for(off), 256, 0, s>>>((cpx<double>*)out, st, off);
     return hipGetLastError();
 }
 

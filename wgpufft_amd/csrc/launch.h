@@ -21,6 +21,7 @@ struct RegVariant {
     int ld_lf, st_lf;
     int ldm, stm;
     int post;
+    int vec;  // adjacent lines per thread (2: 16-byte pair accesses)
     int nthreads;
     size_t lds_bytes;
     const void* fn;
@@ -29,7 +30,7 @@ struct RegVariant {
 
 // Registry lookup; nullptr when no compiled variant matches.
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                   int post);
+                                   int post, int vec);
 int reg_variant_count();
 const RegVariant* reg_variant_at(int i);
 
@@ -58,6 +59,9 @@ size_t conv_kernel_bytes(int prec, uint32_t m, uint32_t lpw);
 
 // Table generation (FP64 on the device, rounded once to the table type).
 hipError_t gen_twiddles(int prec, void* out, uint64_t n, uint64_t count, hipStream_t s);
+// Stage-twiddle table of a radix schedule (entries: stage_twiddle_count).
+uint64_t stage_twiddle_count(const int* rad, int ns);
+hipError_t gen_stage_twiddles(int prec, void* out, const int* rad, int ns, hipStream_t s);
 hipError_t gen_twiddles_f64_strided(double2* out, uint64_t n, uint64_t count, uint64_t mult,
                                     hipStream_t s);
 hipError_t gen_chirp(int prec, void* out, uint64_t n, hipStream_t s);

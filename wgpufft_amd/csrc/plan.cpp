@@ -265,11 +265,14 @@ static Sched make_sched(uint64_t n, uint32_t lpw, bool ld_lf, bool st_lf) {
     sc.dn = make_magic((uint32_t)n);
     sc.dlpw = make_magic(lpw);
     uint32_t L = 1;
+    uint64_t twoff = 0;
     for (size_t s = 0; s < rad.size(); s++) {
         sc.rad[s] = (uint8_t)rad[s];
         sc.L[s] = L;
         sc.dnr[s] = make_magic((uint32_t)(n / rad[s]));
         sc.dl[s] = make_magic(L);
+        sc.twoff[s] = (uint32_t)twoff;
+        if (s >= 1) twoff += (uint64_t)(rad[s] - 1) * L;
         L *= rad[s];
     }
     return sc;
@@ -290,7 +293,7 @@ static bool adjacent_lines(const LineSet& ls, const Layout& l) {
 struct Builder {
     Plan& P;
     hipStream_t s;
-    std::map<std::pair<int, uint64_t>, const void*> tw_cache;
+    std::map<std::string, const void*> tw_cache;
 
     void* alloc(size_t bytes) {
         DevBuf b;
@@ -302,19 +305,42 @@ struct Builder {
         return b.p;
     }
 
-    // W_n^t for t < n, arithmetic precision, generated in FP64 on the device.
-    const void* twiddles(int arith, uint64_t n) {
-        auto key = std::make_pair(arith, n);
+    // Stage-twiddle table of a radix schedule, arithmetic precision, generated
+    // in FP64 on the device and rounded once.
+    const void* stage_twiddles(int arith, const std::vector<int>& rad) {
+        std::string key = std::to_string(arith) + ":";
+        for (int r : rad) key += std::to_string(r) + "x";
         auto it = tw_cache.find(key);
         if (it != tw_cache.end()) return it->second;
-        void* t = alloc(n * elem_bytes(arith));
-        hip_check(gen_twiddles(arith, t, n, n, s), "twiddle generation");
+        const uint64_t cnt = stage_twiddle_count(rad.data(), (int)rad.size());
+        void* t = alloc(std::max<uint64_t>(cnt, 1) * elem_bytes(arith));
+        hip_check(gen_stage_twiddles(arith, t, rad.data(), (int)rad.size(), s), "twiddle generation");
         tw_cache[key] = t;
         return t;
     }
 
-    LineArgs base_args(const LineSet& ls, const Layout& in, const Layout& out, int arith,
-                       uint64_t n, bool swap_in, bool swap_out, double scale) {
+    static std::vector<int> parse_radices(const char* s) {
+        std::vector<int> r;
+        int v = 0;
+        for (const char* c = s;; c++) {
+            if (*c >= '0' && *c <= '9') v = v * 10 + (*c - '0');
+            else {
+                if (v) r.push_back(v);
+                v = 0;
+                if (!*c) break;
+            }
+        }
+        return r;
+    }
+
+    static std::vector<int> sched_radices(const Sched& sc) {
+        std::vector<int> r;
+        for (uint32_t i = 0; i < sc.ns; i++) r.push_back(sc.rad[i]);
+        return r;
+    }
+
+    LineArgs base_args(const LineSet& ls, const Layout& in, const Layout& out, bool swap_in,
+                       bool swap_out, double scale) {
         LineArgs a;
         std::memset(&a, 0, sizeof(a));
         a.lines = ls.lines;
@@ -324,7 +350,7 @@ struct Builder {
         a.d2 = make_magic(ls.S2);
         a.in_si = in.si; a.in_sj = in.sj; a.in_so = in.so; a.in_sn = in.sn;
         a.out_si = out.si; a.out_sj = out.sj; a.out_so = out.so; a.out_sk = out.sn;
-        a.tw = n > 1 ? twiddles(arith, n) : nullptr;
+        a.tw = nullptr;
         a.scale = scale;
         a.swap_in = swap_in;
         a.swap_out = swap_out;
@@ -349,11 +375,19 @@ struct Builder {
         op.ldm = ldm;
         op.stm = stm;
         op.post = post;
-        const RegVariant* rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post);
+        // Pairs of adjacent lines (16-byte accesses) need both sides to place line
+        // g+1 right after line g with every stride even (16-byte alignment).
+        const bool pairable = ld_lf && st_lf && ldm == LD_COMPLEX && stm == ST_COMPLEX &&
+                              ls.lines % 2 == 0 && ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 &&
+                              in.sj % 2 == 0 && in.so % 2 == 0 && in.sn % 2 == 0 &&
+                              out.sj % 2 == 0 && out.so % 2 == 0 && out.sn % 2 == 0;
+        const RegVariant* rv =
+            pairable ? find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 2) : nullptr;
+        if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
         const size_t es = elem_bytes(P.arith);
         const uint64_t lstr = lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30));
         if (!rv && 2 * lstr * es > kLdsBudget) return false;
-        op.a = base_args(ls, in, out, P.arith, n, swap_in, swap_out, scale);
+        op.a = base_args(ls, in, out, swap_in, swap_out, scale);
         op.a.post_hi = post_hi;
         op.a.post_lo = post_lo;
         op.a.post_shift = post_shift;
@@ -362,7 +396,9 @@ struct Builder {
         if (rv) {
             op.kind = OP_REG;
             op.rv = rv;
+            op.a.tw = stage_twiddles(P.arith, parse_radices(rv->radices));
             d << "reg N=" << n << " " << rv->radices << " lines=" << ls.lines << " tpl=" << rv->tpl
+              << " vec=" << rv->vec
               << " lpw=" << rv->lines;
         } else {
             op.kind = OP_LDS;
@@ -372,6 +408,7 @@ struct Builder {
             else lpw = (uint32_t)std::max<uint64_t>(1, 4096 / n);
             lpw = std::max<uint32_t>(1, std::min(lpw, fit));
             op.sc = make_sched(n, lpw, ld_lf, st_lf);
+            op.a.tw = stage_twiddles(P.arith, sched_radices(op.sc));
             d << "lds N=" << n << " " << sched_str(op.sc) << " lines=" << ls.lines << " lpw=" << lpw;
         }
         if (ld_lf || st_lf) d << (ld_lf ? " ldLF" : "") << (st_lf ? " stLF" : "");
@@ -452,8 +489,9 @@ struct Builder {
         op.dst = dst;
         op.dst_off = dst_off;
         op.prec = P.prec;
-        op.conv.io = base_args(ls, in, out, P.arith, m, swap_in, swap_out, scale);
+        op.conv.io = base_args(ls, in, out, swap_in, swap_out, scale);
         op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
+        op.conv.io.tw = stage_twiddles(P.arith, sched_radices(op.conv.sc));
         op.conv.filt = filt;
         op.conv.perm_in = dpin;
         op.conv.perm_out = dpout;
@@ -492,8 +530,9 @@ struct Builder {
         op.dst = dst;
         op.dst_off = dst_off;
         op.prec = P.prec;
-        op.conv.io = base_args(ls, in, out, P.arith, m, swap_in, swap_out, scale);
+        op.conv.io = base_args(ls, in, out, swap_in, swap_out, scale);
         op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
+        op.conv.io.tw = stage_twiddles(P.arith, sched_radices(op.conv.sc));
         op.conv.filt = bluestein_filter(n, m);
         op.conv.chirp = chirp_table(n);
         op.conv.n = (uint32_t)n;
@@ -530,7 +569,7 @@ struct Builder {
         Plan* child = P.children.back().get();
         const void* chirp = chirp_table(n);
         const void* filt = bluestein_filter(n, m);
-        LineArgs a = base_args(ls, in, out, P.arith, 1, swap_in, swap_out, scale);
+        LineArgs a = base_args(ls, in, out, swap_in, swap_out, scale);
         Op pre;
         pre.kind = OP_CHIRP_PRE;
         pre.src = src;
@@ -750,7 +789,15 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
                 op.desc = "direct N=1";
                 P->ops.push_back(op);
             } else {
-                uint64_t ws = 0;
+                // Ping-pong: with several axes, consecutive passes alternate between
+                // the output and a workspace copy of the array (the last pass lands
+                // in the output), so no pass reads and writes the same buffer;
+                // in-place streams run measurably slower on HBM than
+                // out-of-place ones.  Scratch of multi-pass axes sits after it.
+                const uint64_t arr_bytes = total * cb;
+                const bool pp = cfg.naxes >= 2;
+                const uint64_t scratch = pp ? arr_bytes : 0;
+                uint64_t ws = scratch;
                 int src = B_IN;
                 for (int k = 0; k < cfg.naxes; k++) {
                     const int axis = cfg.axes[k];
@@ -759,9 +806,10 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
                     const uint64_t n = cfg.shape[axis];
                     const uint64_t O = total / (S * n);
                     const bool last = k + 1 == cfg.naxes;
-                    ws = std::max(ws, B.axis_ops(n, P->axes[k].kind, S, O, src, 0, B_OUT, 0, 0,
-                                                 inverse, last ? scale : 1.0));
-                    src = B_OUT;
+                    const int dst = (!pp || (cfg.naxes - 1 - k) % 2 == 0) ? B_OUT : B_WS;
+                    ws = std::max(ws, scratch + B.axis_ops(n, P->axes[k].kind, S, O, src, 0, dst, 0,
+                                                           scratch, inverse, last ? scale : 1.0));
+                    src = dst;
                 }
                 P->ws_bytes = ws;
             }
@@ -835,72 +883,99 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
     return P;
 }
 
-void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws) {
+static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void* ws) {
     auto ptr = [&](int ref, uint64_t off) -> char* {
         char* b = ref == B_IN ? (char*)in : ref == B_OUT ? (char*)out : (char*)ws;
         return b + off;
     };
-    for (const Op& op : p.ops) {
-        switch (op.kind) {
-            case OP_REG: {
-                LineArgs a = op.a;
-                a.in = ptr(op.src, op.src_off);
-                a.out = ptr(op.dst, op.dst_off);
-                hip_check(launch_reg(op.rv, a, s), "register line kernel");
-                break;
-            }
-            case OP_LDS: {
-                LineArgs a = op.a;
-                a.in = ptr(op.src, op.src_off);
-                a.out = ptr(op.dst, op.dst_off);
-                hip_check(launch_lds(op.prec, op.ldm, op.stm, op.post, a, op.sc, s), "LDS line kernel");
-                break;
-            }
-            case OP_RADER:
-            case OP_BLUESTEIN: {
-                ConvArgs c = op.conv;
-                c.io.in = ptr(op.src, op.src_off);
-                c.io.out = ptr(op.dst, op.dst_off);
-                hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
-                          "convolution kernel");
-                break;
-            }
-            case OP_CHIRP_PRE: {
-                LineArgs a = op.a;
-                a.in = ptr(op.src, op.src_off);
-                a.out = ptr(op.dst, op.dst_off);
-                hip_check(launch_chirp_pre(op.prec, a, op.chirp, op.n, op.m, s), "chirp pre");
-                break;
-            }
-            case OP_POINTWISE:
-                hip_check(launch_pointwise(op.prec, ptr(op.dst, op.dst_off), op.filt, op.m, op.count, s),
-                          "pointwise");
-                break;
-            case OP_CHIRP_POST: {
-                LineArgs a = op.a;
-                a.in = ptr(op.src, op.src_off);
-                a.out = ptr(op.dst, op.dst_off);
-                hip_check(launch_chirp_post(op.prec, a, op.chirp, op.n, op.m, s), "chirp post");
-                break;
-            }
-            case OP_SUBPLAN: {
-                char* b = ptr(op.dst, op.dst_off);
-                run_plan(*op.sub, s, b, b, (char*)ws + op.sub_ws_off);
-                break;
-            }
-            case OP_SCALE_COPY:
-                hip_check(launch_scale_copy(op.prec, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off),
-                                            op.count, op.scale, s),
-                          "scale copy");
-                break;
+    switch (op.kind) {
+        case OP_REG: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_reg(op.rv, a, s), "register line kernel");
+            break;
         }
+        case OP_LDS: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_lds(op.prec, op.ldm, op.stm, op.post, a, op.sc, s), "LDS line kernel");
+            break;
+        }
+        case OP_RADER:
+        case OP_BLUESTEIN: {
+            ConvArgs c = op.conv;
+            c.io.in = ptr(op.src, op.src_off);
+            c.io.out = ptr(op.dst, op.dst_off);
+            hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
+                      "convolution kernel");
+            break;
+        }
+        case OP_CHIRP_PRE: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_chirp_pre(op.prec, a, op.chirp, op.n, op.m, s), "chirp pre");
+            break;
+        }
+        case OP_POINTWISE:
+            hip_check(launch_pointwise(op.prec, ptr(op.dst, op.dst_off), op.filt, op.m, op.count, s),
+                      "pointwise");
+            break;
+        case OP_CHIRP_POST: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_chirp_post(op.prec, a, op.chirp, op.n, op.m, s), "chirp post");
+            break;
+        }
+        case OP_SUBPLAN: {
+            char* b = ptr(op.dst, op.dst_off);
+            run_plan(*op.sub, s, b, b, (char*)ws + op.sub_ws_off);
+            break;
+        }
+        case OP_SCALE_COPY:
+            hip_check(launch_scale_copy(op.prec, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off),
+                                        op.count, op.scale, s),
+                      "scale copy");
+            break;
     }
 }
 
-static int count_launches(const Plan& p) {
+void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws) {
+    for (const Op& op : p.ops) run_op(op, s, in, out, ws);
+}
+
+int plan_launch_count(const Plan& p) {
     int c = 0;
-    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? count_launches(*op.sub) : 1;
+    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? plan_launch_count(*op.sub) : 1;
     return c;
+}
+
+static bool launch_at(const Plan& p, hipStream_t s, bool run, int* index, const void* in, void* out, void* ws,
+                      std::string* desc) {
+    for (const Op& op : p.ops) {
+        if (op.kind == OP_SUBPLAN) {
+            char* b = (op.dst == B_IN ? (char*)in : op.dst == B_OUT ? (char*)out : (char*)ws) + op.dst_off;
+            if (launch_at(*op.sub, s, run, index, b, b, (char*)ws + op.sub_ws_off, desc)) return true;
+            continue;
+        }
+        if (*index == 0) {
+            if (desc) *desc = op.desc;
+            if (run) run_op(op, s, in, out, ws);
+            return true;
+        }
+        (*index)--;
+    }
+    return false;
+}
+
+bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const void* in, void* out, void* ws,
+                     std::string* desc) {
+    if (index < 0) return false;
+    int i = index;
+    return launch_at(p, s, run, &i, in, out, ws, desc);
 }
 
 std::string describe_plan(const Plan& p) {
@@ -911,7 +986,6 @@ std::string describe_plan(const Plan& p) {
         first = false;
         d << op.desc;
     }
-    (void)count_launches;
     return d.str();
 }
 

@@ -90,6 +90,11 @@ struct Plan {
 std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind);
 // Enqueues the plan on `s`.  `ws` must hold ws_bytes when ws_bytes > 0.
 void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws);
+// Runs only launch `index` of the flattened launch sequence (child plans
+// expanded); returns false if index is out of range.
+bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const void* in, void* out, void* ws,
+                     std::string* desc);
+int plan_launch_count(const Plan& p);
 std::string describe_plan(const Plan& p);
 
 // Host-side integer helpers (the reference's route-choice arithmetic).
