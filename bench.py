@@ -37,9 +37,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c"])
+    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "batched"])
+    ap.add_argument("--transforms", type=int, default=2000, help="batched workload: transform count")
     ap.add_argument("--nbuf", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="profiling driver: 5 plain 64 MiB device copies (torch), then 5 launches "
+                         "of every kernel of the workload; prints nothing else")
     return ap.parse_args()
 
 
@@ -77,6 +81,9 @@ def main():
 
     from wgpufft_amd import FftConfig, FftPlan, fill_normal_f32, fill_uniform_f32
 
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args)), flush=True)
+        return
     spec = workload_spec(args.workload)
     torch.cuda.init()
     dev = torch.device("cuda:0")
@@ -135,6 +142,25 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps  # ms per step
 
+    if args.calibrate:
+        # Counter calibration: a plain copy reads and writes exactly 64 MiB.
+        src = torch.empty(64 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        fill_uniform_f32(src, 99)
+        for _ in range(5):
+            dst.copy_(src)
+        for li in range(plans[0].launch_count()):
+            for r in range(5):
+                k = r % nbuf
+                if spec["kind"] == "r2c_c2r":
+                    plans[0].execute_launch(li, bufs[k], aux[k], None, stream)
+                else:
+                    plans[0].execute_launch(li, bufs[k], bufs[k], None, stream)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate": True, "launches": [plans[0].launch_describe(i)
+                                                         for i in range(plans[0].launch_count())]}))
+        return
+
     # one normalised forward+inverse step must return its input (GPU-only check)
     rt_err = None
     if spec["kind"] in ("c2c_pair", "r2c_c2r"):
@@ -183,8 +209,9 @@ def main():
         kinfo["alg_bytes"] = pb
         kinfo["gbs"] = pb / (kinfo["us"] * 1e-6) / 1e9
     dom = max(kernels, key=lambda k: k["us"])
+    traffic, traffic_src = profiled_traffic(dom["desc"])
     roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom["desc"], "kernel_us": round(dom["us"], 2),
                 "alg_bytes_per_launch": dom["alg_bytes"]}
 
@@ -218,6 +245,128 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+BATCH_LENGTHS = [120, 343, 1001, 127, 1009, 4096, 65536]
+
+
+def batched_items(count, seed=4):
+    """The uneven-batch workload: lengths uniform over BATCH_LENGTHS, line
+    counts uniform in [1, 256], 1009 forced through Bluestein."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(0, len(BATCH_LENGTHS), count)
+    bs = rng.integers(1, 257, count)
+    return [(BATCH_LENGTHS[int(k)], int(b)) for k, b in zip(ns, bs)]
+
+
+def run_batched(args):
+    """One persistent batched launch (hipGraph) over `--transforms` transforms
+    against sequential per-plan dispatch of the same transforms on one stream."""
+    import math
+
+    import torch
+
+    from wgpufft_amd import FftBatch, FftConfig, FftPlan, FftTuning, fill_uniform_f32
+    spec = batched_items(args.transforms)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    bufs, items = [], []
+    for k, (n, b) in enumerate(spec):
+        t = torch.empty(n * b * 8, dtype=torch.uint8, device=dev)
+        bufs.append(t)
+        items.append(dict(n=n, batch=b, input=t, output=t, algorithm=2 if n == 1009 else 0))
+    total_elems = sum(n * b for n, b in spec)
+
+    def refill():
+        for k, t in enumerate(bufs):
+            fill_uniform_f32(t, 1000 + k)
+
+    refill()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch = FftBatch(items)
+    t_build = time.perf_counter() - t0
+    batch.capture(stream)
+    info = batch.info()
+    alg_bytes = sum(2 * 8 * n * b * (2 if n > 16384 else 1) for n, b in spec)
+    flops = sum(5 * n * math.log2(n) * b for n, b in spec)
+
+    def time_it(fn, reps):
+        refill()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    reps = max(1, min(args.steps, 10))
+    batch.launch(stream)  # warm-up
+    ms_batch = time_it(lambda: batch.launch(stream), reps)
+    # sequential per-plan dispatch: one standalone plan per transform, one after another
+    plans = {}
+    t0 = time.perf_counter()
+    for it in items:
+        key = (it["n"], it["batch"])
+        if key not in plans:
+            cfg = FftConfig.new(it["n"]).with_batch(it["batch"])
+            if it["n"] == 1009:
+                cfg = cfg.with_tuning(FftTuning().with_force_bluestein_axes([0]))
+            plans[key] = FftPlan.c2c(cfg)
+    t_plans = time.perf_counter() - t0
+
+    def sequential():
+        for it in items:
+            plans[(it["n"], it["batch"])].execute_in_place(it["input"], stream)
+
+    sequential()
+    ms_seq = time_it(sequential, max(1, reps // 2))
+    per_len = {}
+    for n, b in spec:
+        per_len.setdefault(n, [0, 0])
+        per_len[n][0] += 1
+        per_len[n][1] += b
+    return {
+        "metric": "batched uneven 1D f32 C2C: persistent launch vs sequential per-plan dispatch",
+        "value": round(alg_bytes / (ms_batch * 1e-3) / 1e9, 1),
+        "unit": "GB/s",
+        "higher_is_better": True,
+        "ms_per_launch": round(ms_batch, 3),
+        "ms_sequential": round(ms_seq, 3),
+        "speedup_vs_sequential": round(ms_seq / ms_batch, 2),
+        "gflops": round(flops / (ms_batch * 1e-3) / 1e9, 1),
+        "transforms": len(spec),
+        "complex_elements": total_elems,
+        "alg_bytes": alg_bytes,
+        "tiles": info["tiles"], "phases": info["phases"], "grid": info["grid"],
+        "lds_bytes": info["lds_bytes"], "workspace_bytes": info["workspace_bytes"],
+        "batch_build_s": round(t_build, 2), "standalone_plans": len(plans), "plans_build_s": round(t_plans, 2),
+        "per_length_transforms_lines": {str(k): v for k, v in sorted(per_len.items())},
+        "dtype": "f32", "data": "synthetic (SplitMix64 uniform[-1,1) on device)", "n_gpus": 1,
+    }
+
+
+def profiled_traffic(desc):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on length,
+    radices and lines per thread; (None, None) when no summary covers it."""
+    import glob
+    import re
+    m = re.search(r"N=(\d+) (\S+) .*vec=(\d)", desc)
+    if not m:
+        return None, None
+    n, rad, vec = m.groups()
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if not line.startswith("traffic_bytes_per_launch:"):
+                continue
+            if f"N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line:
+                return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(spec):

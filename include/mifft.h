@@ -194,6 +194,37 @@ mifft_status mifft_recorder_end(mifft_recorder* r);
 mifft_status mifft_recorder_launch(mifft_recorder* r, hipStream_t stream);
 void mifft_recorder_destroy(mifft_recorder* r);
 
+/* ------------------------------------------------------- batched launch */
+/* Many 1D f32 C2C transforms of uneven lengths and line counts in one
+ * persistent launch (two when a length needs a four-step split).  No
+ * counterpart entry point in the reference: it replaces recording one plan per
+ * transform into an FftRecorder (plan.rs:38-95).  A transform's result does
+ * not depend on the other items of the batch (bit for bit).  Buffers are
+ * bound at creation. */
+typedef struct mifft_batch_item {
+    uint64_t n;            /* transform length */
+    uint64_t batch;        /* lines (contiguous, n apart) */
+    int32_t direction;     /* mifft_direction */
+    int32_t normalization; /* mifft_normalization */
+    int32_t algorithm;     /* 0 route choice, 1 force Rader, 2 force Bluestein */
+    int32_t reserved;
+    const void* in;        /* device, interleaved complex f32, 16-byte aligned */
+    void* out;             /* may equal in */
+} mifft_batch_item;
+typedef struct mifft_batch mifft_batch;
+
+mifft_status mifft_batch_create(const mifft_batch_item* items, int32_t count, const mifft_tuning* tuning,
+                                mifft_batch** out);
+/* Enqueues the persistent launch(es) on `stream`. */
+mifft_status mifft_batch_execute(mifft_batch* b, hipStream_t stream);
+/* Captures one execution into a hipGraph kept by the batch (first call only). */
+mifft_status mifft_batch_capture(mifft_batch* b, hipStream_t stream);
+/* Replays the captured graph on `stream`. */
+mifft_status mifft_batch_launch(mifft_batch* b, hipStream_t stream);
+mifft_status mifft_batch_info(const mifft_batch* b, uint64_t* tiles, uint64_t* phases, uint64_t* lds_bytes,
+                              uint64_t* workspace_bytes, uint32_t* grid);
+void mifft_batch_destroy(mifft_batch* b);
+
 /* --------------------------------------------------------------- errors */
 const char* mifft_status_string(mifft_status s);
 /* Thread-local detail text of the last failing call on this thread. */

@@ -21,7 +21,7 @@ from ._lib import LibraryMissing
 
 __all__ = [
     "FftPrecision", "FftDirection", "Normalization", "FftTransformKind", "AxisKind", "C2cRoute",
-    "FftTuning", "FftConfig", "FftPlan", "FftRecorder", "FftError", "LibraryMissing",
+    "FftTuning", "FftConfig", "FftPlan", "FftRecorder", "FftBatch", "FftError", "LibraryMissing",
     "create_plan", "create_r2c_plan", "create_c2r_plan", "fill_uniform_f32", "fill_normal_f32",
 ]
 
@@ -441,6 +441,60 @@ class FftRecorder:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._lib.mifft_recorder_destroy(h)
+            self._h = None
+
+
+class FftBatch:
+    """Many 1D f32 C2C transforms of uneven lengths in one persistent launch.
+
+    ``items``: sequence of dicts with keys ``n``, ``batch``, ``input``,
+    ``output`` (device buffers, may be the same) and optional ``direction``,
+    ``normalization`` (default Inverse) and ``algorithm`` (0 route choice,
+    1 force Rader, 2 force Bluestein).  A transform's result does not depend
+    on the other items (bit for bit).
+    """
+
+    def __init__(self, items, tuning: Optional[FftTuning] = None):
+        self._lib = _lib.load()
+        arr = (_lib.BatchItem * len(items))()
+        self._keep = []
+        for k, it in enumerate(items):
+            ip, _ = _buf(it["input"])
+            op, _ = _buf(it["output"])
+            arr[k].n = int(it["n"])
+            arr[k].batch = int(it["batch"])
+            arr[k].direction = int(it.get("direction", FftDirection.Forward))
+            arr[k].normalization = int(it.get("normalization", Normalization.Inverse))
+            arr[k].algorithm = int(it.get("algorithm", 0))
+            arr[k].in_ptr = ip
+            arr[k].out_ptr = op
+            self._keep.append((it["input"], it["output"]))
+        t = (tuning or FftTuning())._to_c()
+        h = ctypes.c_void_p()
+        _check(self._lib.mifft_batch_create(arr, len(items), ctypes.byref(t), ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, stream=None) -> None:
+        _check(self._lib.mifft_batch_execute(self._h, _stream(stream)))
+
+    def capture(self, stream=None) -> None:
+        _check(self._lib.mifft_batch_capture(self._h, _stream(stream)))
+
+    def launch(self, stream=None) -> None:
+        _check(self._lib.mifft_batch_launch(self._h, _stream(stream)))
+
+    def info(self) -> dict:
+        t, p, l, w = (ctypes.c_uint64() for _ in range(4))
+        g = ctypes.c_uint32()
+        _check(self._lib.mifft_batch_info(self._h, ctypes.byref(t), ctypes.byref(p), ctypes.byref(l),
+                                          ctypes.byref(w), ctypes.byref(g)))
+        return {"tiles": t.value, "phases": p.value, "lds_bytes": l.value, "workspace_bytes": w.value,
+                "grid": g.value}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mifft_batch_destroy(h)
             self._h = None
 
 

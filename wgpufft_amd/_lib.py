@@ -44,6 +44,19 @@ class Config(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("n", u64),
+        ("batch", u64),
+        ("direction", i32),
+        ("normalization", i32),
+        ("algorithm", i32),
+        ("reserved", i32),
+        ("in_ptr", vp),
+        ("out_ptr", vp),
+    ]
+
+
 # (name, restype, argtypes) for every exported function of include/mifft.h.
 SIGNATURES = [
     ("mifft_tuning_init", None, [ctypes.POINTER(Tuning)]),
@@ -79,6 +92,13 @@ SIGNATURES = [
     ("mifft_recorder_end", i32, [vp]),
     ("mifft_recorder_launch", i32, [vp, vp]),
     ("mifft_recorder_destroy", None, [vp]),
+    ("mifft_batch_create", i32, [ctypes.POINTER(BatchItem), i32, ctypes.POINTER(Tuning), ctypes.POINTER(vp)]),
+    ("mifft_batch_execute", i32, [vp, vp]),
+    ("mifft_batch_capture", i32, [vp, vp]),
+    ("mifft_batch_launch", i32, [vp, vp]),
+    ("mifft_batch_info", i32, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                               ctypes.POINTER(u64), ctypes.POINTER(u32)]),
+    ("mifft_batch_destroy", None, [vp]),
     ("mifft_status_string", cp, [i32]),
     ("mifft_last_error_detail", cp, []),
     ("mifft_version", cp, []),

@@ -22,6 +22,8 @@ struct mifft_recorder {
 
 static thread_local std::string g_detail;
 
+void mifft::set_last_detail(const std::string& d) { g_detail = d; }
+
 static mifft_status set_err(mifft_status s, const std::string& d) {
     g_detail = d;
     return s;

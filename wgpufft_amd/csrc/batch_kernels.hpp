@@ -1,0 +1,55 @@
+// Shared declarations of the persistent batched kernel (kern_batch.hip) and its
+// host planner (batch.cpp).
+#pragma once
+#include "conv_kernels.hpp"
+#include "launch.h"
+
+namespace mifft {
+
+// Tile classes: the device function a tile runs.
+enum BatchClass : uint32_t {
+    BC_LDS = 0,
+    BC_LDS_POST,
+    BC_RADER,
+    BC_BLUESTEIN,
+    BC_REG_C4096,
+    BC_REG_C2048,
+    BC_REG_C1024,
+    BC_REG_C512,
+    BC_REG_C256,
+    BC_REG_C128,
+    BC_REG_C64,
+    BC_REG_C120,
+    BC_REG_C343,
+    BC_REG_C1001,
+    BC_REG_P256,    // four-step column pass with twiddle, 256 points, line pairs
+    BC_REG_P256V1,  // the same, one line per thread
+    BC_REG_T256,    // four-step row pass with transposed store, 256 points
+    BC_REG_P128,
+    BC_REG_T128,
+    BC_COUNT
+};
+
+struct BatchTile {
+    uint32_t cls;  // BatchClass
+    uint32_t arg;  // index into the OpDesc array
+    uint32_t g0;   // first line of the tile within its op
+    uint32_t cost; // scheduling weight (host only)
+};
+
+// Arguments of one op of one transform, device pointers already bound.
+struct OpDesc {
+    LineArgs a;   // line ops (register and LDS classes)
+    Sched sc;     // LDS classes
+    ConvArgs conv;  // Rader / Bluestein classes
+};
+
+// Class of a register variant usable inside the batched kernel (256 threads,
+// f32, complex in and out), or -1.
+int batch_class_for(const RegVariant* v);
+
+hipError_t launch_batch(const BatchTile* tiles, uint32_t ntiles, uint32_t* counter, const OpDesc* ops,
+                        uint32_t grid, size_t lds_bytes, hipStream_t s);
+hipError_t launch_batch_reset(uint32_t* counters, uint32_t n, hipStream_t s);
+
+}  // namespace mifft

@@ -368,11 +368,16 @@ struct RegFft {
         }
     }
 
-    static __device__ __forceinline__ void run(const LineArgs& a, cpx<T>* lds) {
-        const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    // Transforms lines [g0, g0 + LINES) with the first NT threads of the block.
+    static __device__ __forceinline__ void tile(const LineArgs& a, cpx<T>* lds, uint32_t g0) {
         cpx<T> w[VEC][EMAX];
         cpx<T> twr[TWREG];
-        stage<0>(a, lds, bid * LINES, w, twr);
+        stage<0>(a, lds, g0, w, twr);
+    }
+
+    static __device__ __forceinline__ void run(const LineArgs& a, cpx<T>* lds) {
+        const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+        tile(a, lds, bid * LINES);
     }
 };
 
@@ -474,15 +479,14 @@ __device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched&
     return src;
 }
 
+// Lines [g0, g0 + sc.lpw) through LDS with the runtime schedule `sc`.
 template <class Fmt, int LDM, int STM, bool POST>
-__global__ void __launch_bounds__(256) lds_fft_kernel(LineArgs a, Sched sc) {
+__device__ void lds_fft_tile(const LineArgs& a, const Sched& sc, unsigned char* smem_raw,
+                             uint32_t g0) {
     using T = typename Fmt::T;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     const uint32_t lstr = lds_line_stride(sc.n);
     cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* bufB = bufA + sc.lpw * lstr;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t g0 = bid * sc.lpw;
     const uint32_t total = sc.lpw * sc.n;
     // load
     for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
@@ -520,6 +524,13 @@ __global__ void __launch_bounds__(256) lds_fft_kernel(LineArgs a, Sched sc) {
             store_point<Fmt, STM, POST>(a, b.out, b.j, p, sc.n, res[l * lstr + lpad(p)]);
         }
     }
+}
+
+template <class Fmt, int LDM, int STM, bool POST>
+__global__ void __launch_bounds__(256) lds_fft_kernel(LineArgs a, Sched sc) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    lds_fft_tile<Fmt, LDM, STM, POST>(a, sc, smem_raw, bid * sc.lpw);
 }
 
 }  // namespace mifft

@@ -1,0 +1,128 @@
+// Persistent batched kernel: many transforms of uneven lengths and batch
+// counts in one launch.
+//
+// The host splits every transform's line passes into tiles (a line range of
+// one pass, sized for one 256-thread workgroup) and sorts them by cost.
+// Resident workgroups pull tile indices from a device counter and run the
+// tile with the device function of its class (register-resident line kernel,
+// runtime-schedule LDS kernel, fused Rader or Bluestein).  A tile computes
+// exactly what the same op computes in a standalone plan, so every transform
+// is bit-identical alone and in the mix.  Passes that depend on a previous
+// pass of the same transform (the second half of a four-step split) go into a
+// second phase, launched after the first on the same stream.
+//
+// This is the gfx950 counterpart of recording many plans into one shared
+// compute pass with FftRecorder (src/plan.rs:38-95, src/runtime/recorder.rs).
+#include "batch_kernels.hpp"
+
+namespace mifft {
+
+// Every class is a separate (not inlined) function: inlined into one switch,
+// the register allocation of the kernel would be the worst path's, and the
+// whole persistent kernel would run at one wave per SIMD.
+template <class K>
+__device__ __noinline__ void reg_tile(const LineArgs& a, unsigned char* smem, uint32_t g0) {
+    K::tile(a, reinterpret_cast<cpx<typename K::T>*>(smem), g0);
+}
+template <bool POST>
+__device__ __noinline__ void lds_tile_nc(const LineArgs& a, const Sched& sc, unsigned char* smem, uint32_t g0) {
+    lds_fft_tile<FmtF32, LD_COMPLEX, ST_COMPLEX, POST>(a, sc, smem, g0);
+}
+__device__ __noinline__ void rader_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
+    rader_tile<FmtF32>(c, smem, g0);
+}
+__device__ __noinline__ void bluestein_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
+    bluestein_tile<FmtF32>(c, smem, g0);
+}
+
+#define MIFFT_BK(N, TPL, LINES, LDLF, STLF, POST, VEC, ...) \
+    RegFft<FmtF32, N, TPL, LINES, LDLF, STLF, LD_COMPLEX, ST_COMPLEX, POST, RadixList<__VA_ARGS__>, VEC>
+
+// Four workgroups per CU (the LDS budget of batched plans is 40 KiB): at most
+// 128 VGPRs per lane.
+__global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ tiles, uint32_t ntiles,
+                                               uint32_t* counter, const OpDesc* __restrict__ ops) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t >= ntiles) break;  // uniform: every wave of the block leaves together
+        const BatchTile tl = tiles[t];
+        const OpDesc& d = ops[tl.arg];
+        switch (tl.cls) {
+            case BC_LDS: lds_tile_nc<false>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_POST: lds_tile_nc<true>(d.a, d.sc, smem, tl.g0); break;
+            case BC_RADER: rader_tile_nc(d.conv, smem, tl.g0); break;
+            case BC_BLUESTEIN: bluestein_tile_nc(d.conv, smem, tl.g0); break;
+            // register-resident variants with 256 threads (the registry entries of the same shape)
+            case BC_REG_C4096: reg_tile<MIFFT_BK(4096, 256, 1, 0, 0, 0, 1, 16, 16, 16)>(d.a, smem, tl.g0); break;
+            case BC_REG_C2048: reg_tile<MIFFT_BK(2048, 128, 2, 0, 0, 0, 1, 16, 16, 8)>(d.a, smem, tl.g0); break;
+            case BC_REG_C1024: reg_tile<MIFFT_BK(1024, 64, 4, 0, 0, 0, 1, 16, 16, 4)>(d.a, smem, tl.g0); break;
+            case BC_REG_C512: reg_tile<MIFFT_BK(512, 32, 8, 0, 0, 0, 1, 8, 8, 8)>(d.a, smem, tl.g0); break;
+            case BC_REG_C256: reg_tile<MIFFT_BK(256, 16, 16, 0, 0, 0, 1, 16, 16)>(d.a, smem, tl.g0); break;
+            case BC_REG_C128: reg_tile<MIFFT_BK(128, 8, 32, 0, 0, 0, 1, 16, 8)>(d.a, smem, tl.g0); break;
+            case BC_REG_C64: reg_tile<MIFFT_BK(64, 4, 64, 0, 0, 0, 1, 16, 4)>(d.a, smem, tl.g0); break;
+            case BC_REG_C120: reg_tile<MIFFT_BK(120, 8, 32, 0, 0, 0, 1, 15, 8)>(d.a, smem, tl.g0); break;
+            case BC_REG_C343: reg_tile<MIFFT_BK(343, 64, 4, 0, 0, 0, 1, 7, 7, 7)>(d.a, smem, tl.g0); break;
+            case BC_REG_C1001: reg_tile<MIFFT_BK(1001, 128, 2, 0, 0, 0, 1, 13, 11, 7)>(d.a, smem, tl.g0); break;
+            case BC_REG_P256: reg_tile<MIFFT_BK(256, 16, 32, 1, 1, 1, 2, 16, 16)>(d.a, smem, tl.g0); break;
+            case BC_REG_P256V1: reg_tile<MIFFT_BK(256, 16, 16, 1, 1, 1, 1, 16, 16)>(d.a, smem, tl.g0); break;
+            case BC_REG_T256: reg_tile<MIFFT_BK(256, 16, 16, 0, 1, 0, 1, 16, 16)>(d.a, smem, tl.g0); break;
+            case BC_REG_P128: reg_tile<MIFFT_BK(128, 8, 64, 1, 1, 1, 2, 16, 8)>(d.a, smem, tl.g0); break;
+            case BC_REG_T128: reg_tile<MIFFT_BK(128, 8, 32, 0, 1, 0, 1, 16, 8)>(d.a, smem, tl.g0); break;
+            default: break;
+        }
+        __syncthreads();  // the next tile reuses LDS
+    }
+}
+
+__global__ void k_batch_reset(uint32_t* counters, uint32_t n) {
+    if (threadIdx.x < n) counters[threadIdx.x] = 0u;
+}
+
+hipError_t launch_batch(const BatchTile* tiles, uint32_t ntiles, uint32_t* counter, const OpDesc* ops,
+                        uint32_t grid, size_t lds_bytes, hipStream_t s) {
+    hipError_t e = ensure_lds_attr((const void*)&k_batch, lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), lds_bytes, s, tiles, ntiles, counter, ops);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(uint32_t* counters, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters, n);
+    return hipGetLastError();
+}
+
+int batch_class_for(const RegVariant* v) {
+    if (!v || v->prec != P_F32 || v->nthreads != 256 || v->ldm != LD_COMPLEX || v->stm != ST_COMPLEX)
+        return -1;
+    const int key = v->n * 1000 + v->ld_lf * 100 + v->st_lf * 10 + v->post;
+    if (v->vec == 1) {
+        switch (key) {
+            case 4096000: return BC_REG_C4096;
+            case 2048000: return BC_REG_C2048;
+            case 1024000: return BC_REG_C1024;
+            case 512000: return BC_REG_C512;
+            case 256000: return BC_REG_C256;
+            case 128000: return BC_REG_C128;
+            case 64000: return BC_REG_C64;
+            case 120000: return BC_REG_C120;
+            case 343000: return BC_REG_C343;
+            case 1001000: return BC_REG_C1001;
+            case 256111: return BC_REG_P256V1;
+            case 256010: return BC_REG_T256;
+            case 128010: return BC_REG_T128;
+            default: return -1;
+        }
+    }
+    switch (key) {
+        case 256111: return BC_REG_P256;
+        case 128111: return BC_REG_P128;
+        default: return -1;
+    }
+}
+
+}  // namespace mifft

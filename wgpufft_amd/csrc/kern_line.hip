@@ -6,19 +6,23 @@
 // at build time and looked up by (precision, N, layout, load/store mode).
 // Lengths without a variant run the runtime-schedule LDS kernel.
 #include <mutex>
-#include <unordered_set>
+#include <unordered_map>
 
 #include "launch.h"
 
 namespace mifft {
 
-void ensure_lds_attr(const void* fn) {
+hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
+    if (bytes <= 64 * 1024) return hipSuccess;
     static std::mutex mu;
-    static std::unordered_set<const void*> done;
+    static std::unordered_map<const void*, size_t> done;
     std::lock_guard<std::mutex> g(mu);
-    if (done.count(fn)) return;
-    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
-    done.insert(fn);
+    auto it = done.find(fn);
+    if (it != done.end() && it->second >= bytes) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    done[fn] = bytes;
+    return hipSuccess;
 }
 
 #define MIFFT_RL(...) RadixList<__VA_ARGS__>
@@ -69,8 +73,8 @@ static const RegVariant kRegistry[] = {
     F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, 4, 8, 16, 16, 16, 32, 64),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
     REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "15x8", 15, 8),
-    REGV(FmtF32, P_F32, 343, 49, 5, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),
-    REGV(FmtF32, P_F32, 1001, 77, 3, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "13x11x7", 13, 11, 7),
+    REGV(FmtF32, P_F32, 343, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),
+    REGV(FmtF32, P_F32, 1001, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "13x11x7", 13, 11, 7),
     // f64 / df64 256-point lines (256^3 volumes)
     REGV(FmtF64, P_F64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
@@ -94,7 +98,8 @@ const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ld
 }
 
 hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s) {
-    if (v->lds_bytes > 64 * 1024) ensure_lds_attr(v->fn);
+    hipError_t e = ensure_lds_attr(v->fn, v->lds_bytes);
+    if (e != hipSuccess) return e;
     const uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
     LineArgs copy = a;
     void* args[] = {&copy};
@@ -121,7 +126,8 @@ static hipError_t launch_lds_fmt(int ldm, int stm, int post, const LineArgs& a, 
     else
         return hipErrorInvalidValue;
     const size_t bytes = lds_kernel_bytes(Fmt::kId == 0 ? P_F32 : P_F64, sc.n, sc.lpw);
-    if (bytes > 64 * 1024) ensure_lds_attr(fn);
+    hipError_t e = ensure_lds_attr(fn, bytes);
+    if (e != hipSuccess) return e;
     const uint32_t blocks = (a.lines + sc.lpw - 1) / sc.lpw;
     LineArgs ca = a;
     Sched cs = sc;

@@ -86,7 +86,7 @@ hipError_t fill_normal_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s)
 
 DivMagic make_magic(uint32_t d);
 
-// Raises a kernel's dynamic-LDS limit to the full 160 KiB once per process.
-void ensure_lds_attr(const void* fn);
+// Raises a kernel's dynamic-LDS limit to `bytes` when above the 64 KiB default.
+hipError_t ensure_lds_attr(const void* fn, size_t bytes);
 
 }  // namespace mifft

@@ -10,6 +10,8 @@
 #include "plan.hpp"
 
 #include <algorithm>
+
+#include "batch_kernels.hpp"
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -386,7 +388,8 @@ struct Builder {
         if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
         const size_t es = elem_bytes(P.arith);
         const uint64_t lstr = lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30));
-        if (!rv && 2 * lstr * es > kLdsBudget) return false;
+        if (rv && P.batch_mode && batch_class_for(rv) < 0) rv = nullptr;
+        if (!rv && 2 * lstr * es > P.lds_budget) return false;
         op.a = base_args(ls, in, out, swap_in, swap_out, scale);
         op.a.post_hi = post_hi;
         op.a.post_lo = post_lo;
@@ -402,7 +405,7 @@ struct Builder {
               << " lpw=" << rv->lines;
         } else {
             op.kind = OP_LDS;
-            uint32_t fit = (uint32_t)(kLdsBudget / (2 * lstr * es));
+            uint32_t fit = (uint32_t)(P.lds_budget / (2 * lstr * es));
             uint32_t lpw;
             if (ld_lf || st_lf) lpw = P.arith == P_F32 ? 16 : 8;
             else lpw = (uint32_t)std::max<uint64_t>(1, 4096 / n);
@@ -453,7 +456,7 @@ struct Builder {
     bool conv_fits(uint64_t m, uint32_t* lpw_out, bool lf) {
         const uint32_t want = lf ? (P.arith == P_F32 ? 16u : 8u) : (uint32_t)std::max<uint64_t>(1, 2048 / m);
         for (uint32_t lpw = want; lpw >= 1; lpw = lpw / 2) {
-            if (conv_kernel_bytes(P.arith, (uint32_t)m, lpw) <= kLdsBudget) {
+            if (conv_kernel_bytes(P.arith, (uint32_t)m, lpw) <= P.lds_budget) {
                 *lpw_out = lpw;
                 return true;
             }
@@ -552,6 +555,9 @@ struct Builder {
                              double scale) {
         if (P.prec == P_DF64)
             fail(MIFFT_ERR_PRECISION_UNSUPPORTED, "df64 Bluestein beyond the on-chip convolution length");
+        if (P.batch_mode)
+            fail(MIFFT_ERR_UNSUPPORTED_LENGTH,
+                 "Bluestein length " + std::to_string(n) + " needs a multi-pass convolution (not batched)");
         const uint64_t region = (uint64_t)ls.lines * m * elem_bytes(P.arith);
         mifft_config cc;
         std::memset(&cc, 0, sizeof(cc));
@@ -636,7 +642,7 @@ struct Builder {
             const uint64_t n2 = n / n1;
             const size_t es = elem_bytes(P.arith);
             auto fits = [&](uint64_t len) {
-                return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= kLdsBudget;
+                return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= P.lds_budget;
             };
             if (!fits(n1) || !fits(n2)) continue;
             const double cost = std::fabs(std::log((double)n1 / (double)n2));
@@ -727,7 +733,7 @@ static int axis_kind_for(const mifft_config& c, int axis) {
     return MIFFT_AXIS_BLUESTEIN;
 }
 
-std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
+std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const PlanOptions& opt) {
     validate_config(cfg);
     if (kind != MIFFT_C2C && kind != MIFFT_R2C && kind != MIFFT_C2R)
         fail(MIFFT_ERR_INVALID_ENUM, "plan kind out of range");
@@ -745,6 +751,8 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind) {
                  cfg.precision == MIFFT_F64 ? "real-f64-not-implemented" : "real-df64-not-implemented");
     }
     auto P = std::make_unique<Plan>();
+    P->lds_budget = opt.lds_budget;
+    P->batch_mode = opt.batch_mode;
     P->cfg = cfg;
     P->kind = kind;
     P->prec = cfg.precision;

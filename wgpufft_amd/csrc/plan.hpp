@@ -17,6 +17,8 @@ struct Error {
 };
 [[noreturn]] void fail(mifft_status code, const std::string& detail);
 void hip_check(hipError_t e, const char* what);
+// Sets the thread-local text returned by mifft_last_error_detail().
+void set_last_detail(const std::string& detail);
 
 enum BufRef { B_IN = 0, B_OUT = 1, B_WS = 2 };
 
@@ -79,6 +81,8 @@ struct Plan {
     std::vector<std::unique_ptr<Plan>> children;
     uint64_t in_bytes = 0, out_bytes = 0, ws_bytes = 0;
     bool in_place_kernels = true;
+    size_t lds_budget = kLdsBudget;
+    bool batch_mode = false;
     // Plan-owned workspace used when the caller passes none (one in-flight
     // execution per plan, as the reference's plan-owned temporaries).
     std::mutex ws_mu;
@@ -87,7 +91,15 @@ struct Plan {
     ~Plan();
 };
 
-std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind);
+// Planner options: the LDS budget per workgroup, and the batched mode, which
+// restricts kernels to those the persistent batched kernel can run
+// (256 threads, see batch_kernels.hpp).
+struct PlanOptions {
+    size_t lds_budget = kLdsBudget;
+    bool batch_mode = false;
+};
+std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind,
+                                 const PlanOptions& opt = PlanOptions());
 // Enqueues the plan on `s`.  `ws` must hold ws_bytes when ws_bytes > 0.
 void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws);
 // Runs only launch `index` of the flattened launch sequence (child plans
