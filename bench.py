@@ -37,7 +37,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "batched"])
+    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "batched", "serve"])
+    ap.add_argument("--requests", type=int, default=6000, help="serve workload: trace length")
+    ap.add_argument("--rate", type=float, default=20000.0, help="serve workload: Poisson arrival rate (req/s)")
+    ap.add_argument("--no-serve", action="store_true", help="default workload: skip the serving summary")
     ap.add_argument("--transforms", type=int, default=2000, help="batched workload: transform count")
     ap.add_argument("--nbuf", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -83,6 +86,9 @@ def main():
 
     if args.workload == "batched":
         print(json.dumps(run_batched(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args)), flush=True)
         return
     spec = workload_spec(args.workload)
     torch.cuda.init()
@@ -270,7 +276,7 @@ def run_batched(args):
     from wgpufft_amd import FftBatch, FftConfig, FftPlan, FftTuning, fill_uniform_f32
     spec = batched_items(args.transforms)
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # graph capture needs a non-default stream
     bufs, items = [], []
     for k, (n, b) in enumerate(spec):
         t = torch.empty(n * b * 8, dtype=torch.uint8, device=dev)
@@ -280,7 +286,7 @@ def run_batched(args):
 
     def refill():
         for k, t in enumerate(bufs):
-            fill_uniform_f32(t, 1000 + k)
+            fill_uniform_f32(t, 1000 + k, stream)
 
     refill()
     torch.cuda.synchronize()
@@ -347,6 +353,47 @@ def run_batched(args):
         "batch_build_s": round(t_build, 2), "standalone_plans": len(plans), "plans_build_s": round(t_plans, 2),
         "per_length_transforms_lines": {str(k): v for k, v in sorted(per_len.items())},
         "dtype": "f32", "data": "synthetic (SplitMix64 uniform[-1,1) on device)", "n_gpus": 1,
+    }
+
+
+def run_serve(args):
+    """Serving workload: a seeded request mix (1D f32/f64 C2C with smooth,
+    prime and Bluestein lengths in [64, 2^20], 2D f32 C2C up to 1024^2, R2C)
+    with Poisson arrivals at --rate; continuous batching against sequential
+    per-plan dispatch, latency at the given rate and throughput at saturation
+    (the whole trace arriving at once)."""
+    import numpy as np
+    import torch
+
+    from wgpufft_amd import serving as S
+    specs = S.request_mix(args.requests)
+    bufs = S.allocate(specs)
+    torch.cuda.synchronize()
+    arrivals = S.poisson_arrivals(len(specs), args.rate)
+    srv = S.FftServer(budget_bytes=64 << 30, streams=4, max_tick_requests=256)
+    t0 = time.perf_counter()
+    srv.prepare(specs, bufs)
+    t_prep = time.perf_counter() - t0
+    srv.replay(specs[:200], bufs[:200], np.zeros(200), mode=0)  # warm-up
+    out = {}
+    for mode, name in ((0, "continuous_batching"), (1, "sequential_per_plan")):
+        lat = S.latency_summary(srv.replay(specs, bufs, arrivals, mode=mode))
+        sat = S.latency_summary(srv.replay(specs, bufs, np.zeros(len(specs)), mode=mode))
+        out[name] = {"poisson": lat, "saturation_throughput_rps": sat["throughput_rps"],
+                     "saturation_makespan_s": sat["makespan_s"]}
+    cb, sq = out["continuous_batching"], out["sequential_per_plan"]
+    tot_bytes = sum(sum(S.request_bytes(s)) for s in specs)
+    return {
+        "metric": "mixed-size FFT serving: p50/p99 latency at Poisson arrivals, throughput at saturation",
+        "value": round(cb["poisson"]["p50_ms"], 4), "unit": "ms (p50 latency)", "higher_is_better": False,
+        "p50_ms": round(cb["poisson"]["p50_ms"], 4), "p99_ms": round(cb["poisson"]["p99_ms"], 4),
+        "throughput_rps": round(cb["poisson"]["throughput_rps"], 1),
+        "saturation_throughput_rps": round(cb["saturation_throughput_rps"], 1),
+        "saturation_speedup_vs_sequential": round(cb["saturation_throughput_rps"] / sq["saturation_throughput_rps"], 2),
+        "p99_ratio_sequential_over_batching": round(sq["poisson"]["p99_ms"] / cb["poisson"]["p99_ms"], 2),
+        "arrival_rate_rps": args.rate, "requests": len(specs), "bytes_in_out": tot_bytes,
+        "budget_bytes": 64 << 30, "streams": 4, "prepare_s": round(t_prep, 2),
+        "modes": out, "dtype": "f32/f64", "data": "synthetic (SplitMix64 uniform[-1,1) on device)", "n_gpus": 1,
     }
 
 

@@ -225,6 +225,50 @@ mifft_status mifft_batch_info(const mifft_batch* b, uint64_t* tiles, uint64_t* p
                               uint64_t* workspace_bytes, uint32_t* grid);
 void mifft_batch_destroy(mifft_batch* b);
 
+/* -------------------------------------------------------------- serving */
+/* Continuous-batching request loop on one GPU (no counterpart in the
+ * reference, which leaves submission to the caller).  A tick packs the
+ * admitted 1D f32 C2C requests into one persistent batched launch and runs the
+ * others through resident plans; up to `streams` ticks overlap on the device;
+ * requests are admitted while input + output + workspace bytes of the work on
+ * the GPU stay within the budget.  Request results do not depend on what else
+ * is in flight. */
+typedef struct mifft_server_request {
+    int32_t kind;          /* mifft_kind */
+    int32_t precision;     /* mifft_precision */
+    int32_t rank;          /* 1 or 2 */
+    int32_t direction;     /* mifft_direction */
+    int32_t normalization; /* mifft_normalization */
+    int32_t reserved;
+    uint64_t shape[2];
+    uint64_t batch;
+    const void* in;
+    void* out;
+    uint64_t in_bytes;
+    uint64_t out_bytes;
+} mifft_server_request;
+
+typedef struct mifft_server_result {
+    double arrival_s; /* trace time of arrival */
+    double admit_s;   /* when its tick was enqueued */
+    double done_s;    /* when its completion was observed */
+    int32_t status;   /* MIFFT_SUCCESS or MIFFT_ERR_OVER_BUDGET */
+    int32_t path;     /* 0 batched launch, 1 resident plan, 2 rejected */
+} mifft_server_result;
+
+typedef struct mifft_server mifft_server;
+
+mifft_status mifft_server_create(uint64_t memory_budget_bytes, int32_t streams, int32_t max_tick_requests,
+                                 mifft_server** out);
+/* Builds and keeps resident the plans of every distinct request shape, plus
+ * the per-stream workspaces, descriptor arenas and pinned staging. */
+mifft_status mifft_server_prepare(mifft_server* s, const mifft_server_request* reqs, int32_t n);
+/* Replays an arrival trace (seconds from start).  mode 0: continuous
+ * batching; mode 1: sequential per-plan dispatch on one stream (the baseline). */
+mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* reqs, const double* arrival_s,
+                                 int32_t n, int32_t mode, mifft_server_result* results);
+void mifft_server_destroy(mifft_server* s);
+
 /* --------------------------------------------------------------- errors */
 const char* mifft_status_string(mifft_status s);
 /* Thread-local detail text of the last failing call on this thread. */

@@ -57,6 +57,33 @@ class BatchItem(ctypes.Structure):
     ]
 
 
+class ServerRequest(ctypes.Structure):
+    _fields_ = [
+        ("kind", i32),
+        ("precision", i32),
+        ("rank", i32),
+        ("direction", i32),
+        ("normalization", i32),
+        ("reserved", i32),
+        ("shape", u64 * 2),
+        ("batch", u64),
+        ("in_ptr", vp),
+        ("out_ptr", vp),
+        ("in_bytes", u64),
+        ("out_bytes", u64),
+    ]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("status", i32),
+        ("path", i32),
+    ]
+
+
 # (name, restype, argtypes) for every exported function of include/mifft.h.
 SIGNATURES = [
     ("mifft_tuning_init", None, [ctypes.POINTER(Tuning)]),
@@ -99,6 +126,11 @@ SIGNATURES = [
     ("mifft_batch_info", i32, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
                                ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     ("mifft_batch_destroy", None, [vp]),
+    ("mifft_server_create", i32, [u64, i32, i32, ctypes.POINTER(vp)]),
+    ("mifft_server_prepare", i32, [vp, ctypes.POINTER(ServerRequest), i32]),
+    ("mifft_server_replay", i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(ctypes.c_double), i32, i32,
+                                  ctypes.POINTER(ServerResult)]),
+    ("mifft_server_destroy", None, [vp]),
     ("mifft_status_string", cp, [i32]),
     ("mifft_last_error_detail", cp, []),
     ("mifft_version", cp, []),

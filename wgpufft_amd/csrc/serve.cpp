@@ -1,0 +1,340 @@
+// Serving loop: continuous batching of FFT requests on one GPU.
+//
+// The reference has no serving layer; its closest mechanism is recording many
+// plans into one FftRecorder (src/plan.rs:38-95).  Here a single host event
+// loop replays an arrival trace:
+//   * arrivals join a FIFO queue;
+//   * whenever a stream slot is free, the loop forms a tick from the head of
+//     the queue, admitting requests while their memory (input + output +
+//     workspace) fits the budget next to the work already on the GPU;
+//   * a tick packs its 1D f32 C2C requests into one persistent batched launch
+//     (kern_batch.hip) and runs the others (f64, 2D, R2C, lengths the batched
+//     kernel does not take) through their resident plans, on the slot's stream;
+//   * ticks on different slots overlap on the device; an event per tick marks
+//     completion, which frees the slot and the memory.
+// Plans, descriptor arenas, pinned staging and workspaces are allocated once
+// (mifft_server_prepare); the loop itself allocates nothing.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+
+#include "batch_plan.hpp"
+
+using namespace mifft;
+
+namespace {
+
+struct PlanKey {
+    int kind, prec, rank, dir, norm;
+    uint64_t s0, s1, batch;
+    bool operator<(const PlanKey& o) const {
+        return std::tie(kind, prec, rank, dir, norm, s0, s1, batch) <
+               std::tie(o.kind, o.prec, o.rank, o.dir, o.norm, o.s0, o.s1, o.batch);
+    }
+};
+
+PlanKey key_of(const mifft_server_request& r) {
+    return PlanKey{r.kind, r.precision, r.rank, r.direction, r.normalization, r.shape[0],
+                   r.rank > 1 ? r.shape[1] : 1, r.batch};
+}
+
+struct Slot {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<int> reqs;
+    uint64_t mem = 0;
+    char* ws = nullptr;
+    uint64_t ws_cap = 0;
+    char* desc = nullptr;       // device arena: ops, tiles
+    char* staging = nullptr;    // pinned host copy of the arena
+    size_t desc_cap = 0;
+    uint32_t* counters = nullptr;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct mifft_server {
+    uint64_t budget = 0;
+    int nslots = 0;
+    int max_items = 0;
+    BatchPlanCache bcache;
+    std::map<PlanKey, std::unique_ptr<Plan>> plans;  // standalone plans
+    std::vector<Slot> slots;
+    char* seq_ws = nullptr;  // sequential mode: one workspace, stream order serialises its users
+    uint64_t seq_ws_cap = 0;
+    hipStream_t seq_stream = nullptr;
+    std::vector<hipEvent_t> seq_events;
+    ~mifft_server() {
+        for (Slot& s : slots) {
+            if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.s) (void)hipStreamDestroy(s.s);
+            if (s.ws) (void)hipFree(s.ws);
+            if (s.desc) (void)hipFree(s.desc);
+            if (s.staging) (void)hipHostFree(s.staging);
+            if (s.counters) (void)hipFree(s.counters);
+        }
+        for (hipEvent_t e : seq_events) (void)hipEventDestroy(e);
+        if (seq_stream) (void)hipStreamDestroy(seq_stream);
+        if (seq_ws) (void)hipFree(seq_ws);
+    }
+};
+
+static bool batchable(mifft_server& S, const mifft_server_request& r) {
+    if (r.kind != MIFFT_C2C || r.precision != MIFFT_F32 || r.rank != 1) return false;
+    mifft_batch_item it = {r.shape[0], r.batch, r.direction, r.normalization, 0, 0, r.in, r.out};
+    return batch_item_supported(S.bcache, it);
+}
+
+static mifft_batch_item item_of(const mifft_server_request& r) {
+    return mifft_batch_item{r.shape[0], r.batch, r.direction, r.normalization, 0, 0, r.in, r.out};
+}
+
+static Plan& standalone(mifft_server& S, const mifft_server_request& r) {
+    PlanKey k = key_of(r);
+    auto f = S.plans.find(k);
+    if (f != S.plans.end()) return *f->second;
+    mifft_config c;
+    std::memset(&c, 0, sizeof(c));
+    if (r.rank < 1 || r.rank > 2) fail(MIFFT_ERR_RANK_TOO_LARGE, "server requests are 1D or 2D");
+    c.rank = r.rank;
+    c.naxes = r.rank;
+    for (int i = 0; i < r.rank; i++) {
+        c.shape[i] = r.shape[i];
+        c.axes[i] = i;
+    }
+    c.batch = r.batch;
+    c.direction = r.direction;
+    c.normalization = r.normalization;
+    c.precision = r.precision;
+    mifft_tuning_init(&c.tuning);
+    std::unique_ptr<Plan> p = build_plan(c, (mifft_kind)r.kind);
+    Plan& ref = *p;
+    S.plans[k] = std::move(p);
+    return ref;
+}
+
+static uint64_t request_ws(mifft_server& S, const mifft_server_request& r, bool batched) {
+    if (batched) return batch_item_workspace(S.bcache, item_of(r));
+    return standalone(S, r).ws_bytes;
+}
+
+static void prepare(mifft_server& S, const mifft_server_request* reqs, int32_t n) {
+    uint64_t max_plan_ws = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const mifft_server_request& r = reqs[i];
+        if (batchable(S, r)) request_ws(S, r, true);
+        // the sequential baseline runs every request through a standalone plan
+        max_plan_ws = std::max(max_plan_ws, standalone(S, r).ws_bytes);
+    }
+    if (S.slots.empty()) {
+        S.slots.resize(S.nslots);
+        const uint64_t ws_cap = S.budget / (2 * (uint64_t)S.nslots);
+        const size_t desc_cap = (size_t)S.max_items * 4096 * sizeof(BatchTile) + (size_t)S.max_items * 2 * sizeof(OpDesc);
+        for (Slot& sl : S.slots) {
+            hip_check(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking), "server stream");
+            hip_check(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming), "server event");
+            sl.ws_cap = std::max(ws_cap, max_plan_ws);
+            hip_check(hipMalloc(&sl.ws, sl.ws_cap), "server workspace");
+            sl.desc_cap = desc_cap;
+            hip_check(hipMalloc(&sl.desc, desc_cap), "server descriptors");
+            hip_check(hipHostMalloc(&sl.staging, desc_cap, hipHostMallocDefault), "server staging");
+            hip_check(hipMalloc(&sl.counters, 64), "server counters");
+        }
+        hip_check(hipStreamCreateWithFlags(&S.seq_stream, hipStreamNonBlocking), "sequential stream");
+    }
+    if (max_plan_ws > S.seq_ws_cap) {
+        if (S.seq_ws) (void)hipFree(S.seq_ws);
+        S.seq_ws = nullptr;
+        hip_check(hipMalloc(&S.seq_ws, max_plan_ws), "sequential workspace");
+        S.seq_ws_cap = max_plan_ws;
+    }
+    hip_check(hipDeviceSynchronize(), "server prepare");
+}
+
+// Launches one tick on a free slot.  Returns the number of requests taken.
+static void launch_tick(mifft_server& S, Slot& sl, const mifft_server_request* reqs, const std::vector<int>& take,
+                        const std::vector<char>& is_batched) {
+    std::vector<mifft_batch_item> items;
+    std::vector<int> plan_reqs;
+    for (int r : take) {
+        if (is_batched[r]) items.push_back(item_of(reqs[r]));
+        else plan_reqs.push_back(r);
+    }
+    BatchBuild b;
+    if (!items.empty()) batch_build(S.bcache, items.data(), (int32_t)items.size(), sl.ws, b);
+    char* plan_ws = sl.ws + b.ws_bytes;
+    const size_t ops_bytes = b.ops.size() * sizeof(OpDesc);
+    const size_t t0_bytes = b.tiles[0].size() * sizeof(BatchTile);
+    const size_t t1_bytes = b.tiles[1].size() * sizeof(BatchTile);
+    const size_t total = ops_bytes + t0_bytes + t1_bytes;
+    if (!items.empty()) {
+        if (total > sl.desc_cap) fail(MIFFT_ERR_OVER_BUDGET, "tick descriptors exceed the arena");
+        std::memcpy(sl.staging, b.ops.data(), ops_bytes);
+        std::memcpy(sl.staging + ops_bytes, b.tiles[0].data(), t0_bytes);
+        std::memcpy(sl.staging + ops_bytes + t0_bytes, b.tiles[1].data(), t1_bytes);
+        hip_check(hipMemcpyAsync(sl.desc, sl.staging, total, hipMemcpyHostToDevice, sl.s), "descriptor upload");
+        batch_enqueue((const BatchTile*)(sl.desc + ops_bytes), (uint32_t)b.tiles[0].size(),
+                      (const BatchTile*)(sl.desc + ops_bytes + t0_bytes), (uint32_t)b.tiles[1].size(),
+                      (const OpDesc*)sl.desc, sl.counters, batch_grid(b.lds_bytes), b.lds_bytes, sl.s);
+    }
+    for (int r : plan_reqs) run_plan(standalone(S, reqs[r]), sl.s, reqs[r].in, reqs[r].out, plan_ws);
+    hip_check(hipEventRecord(sl.ev, sl.s), "tick event");
+}
+
+static void replay(mifft_server& S, const mifft_server_request* reqs, const double* arrival, int32_t n,
+                   int32_t mode, mifft_server_result* res) {
+    for (int32_t i = 0; i < n; i++) {
+        res[i] = mifft_server_result{arrival[i], -1.0, -1.0, 0, 0};
+    }
+    std::vector<char> is_batched(n, 0);
+    std::vector<uint64_t> mem(n, 0);
+    for (int32_t i = 0; i < n; i++) {
+        is_batched[i] = mode == 0 && batchable(S, reqs[i]);
+        mem[i] = reqs[i].in_bytes + reqs[i].out_bytes + request_ws(S, reqs[i], is_batched[i]);
+        res[i].path = is_batched[i] ? 0 : 1;
+    }
+    hip_check(hipDeviceSynchronize(), "replay start");
+    const auto t0 = std::chrono::steady_clock::now();
+    int32_t next = 0, done = 0;
+    std::deque<int> pending;
+    if (mode == 1) {
+        // Sequential per-plan dispatch: each arrival is enqueued at once on one
+        // stream through its standalone plan; completion in stream order.
+        while ((int32_t)S.seq_events.size() < n) {
+            hipEvent_t e;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            S.seq_events.push_back(e);
+        }
+        int32_t head = 0;
+        while (done < n) {
+            const double t = now_s(t0);
+            while (next < n && arrival[next] <= t) {
+                run_plan(standalone(S, reqs[next]), S.seq_stream, reqs[next].in, reqs[next].out, S.seq_ws);
+                hip_check(hipEventRecord(S.seq_events[next], S.seq_stream), "event");
+                res[next].admit_s = now_s(t0);
+                next++;
+            }
+            while (head < next && hipEventQuery(S.seq_events[head]) == hipSuccess) {
+                res[head].done_s = now_s(t0);
+                head++;
+                done++;
+            }
+        }
+        return;
+    }
+    uint64_t inflight = 0;
+    while (done < n) {
+        double t = now_s(t0);
+        while (next < n && arrival[next] <= t) pending.push_back(next++);
+        for (Slot& sl : S.slots) {
+            if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
+                t = now_s(t0);
+                for (int r : sl.reqs) {
+                    res[r].done_s = t;
+                    done++;
+                }
+                inflight -= sl.mem;
+                sl.busy = false;
+                sl.reqs.clear();
+                sl.mem = 0;
+            }
+        }
+        if (pending.empty()) continue;
+        Slot* free_slot = nullptr;
+        for (Slot& sl : S.slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (!free_slot) continue;
+        std::vector<int> take;
+        uint64_t tick_mem = 0, batch_ws = 0, plan_ws = 0;
+        while (!pending.empty() && (int)take.size() < S.max_items) {
+            const int r = pending.front();
+            const uint64_t ws = mem[r] - reqs[r].in_bytes - reqs[r].out_bytes;
+            if (mem[r] > S.budget || ws > free_slot->ws_cap) {
+                pending.pop_front();  // can never be admitted
+                res[r].status = MIFFT_ERR_OVER_BUDGET;
+                res[r].admit_s = res[r].done_s = now_s(t0);
+                res[r].path = 2;
+                done++;
+                continue;
+            }
+            const uint64_t nb = is_batched[r] ? batch_ws + ws : batch_ws;
+            const uint64_t np = is_batched[r] ? plan_ws : std::max(plan_ws, ws);
+            if (inflight + tick_mem + mem[r] > S.budget || nb + np > free_slot->ws_cap) break;
+            batch_ws = nb;
+            plan_ws = np;
+            tick_mem += mem[r];
+            take.push_back(r);
+            pending.pop_front();
+        }
+        if (take.empty()) continue;
+        const double ta = now_s(t0);
+        for (int r : take) res[r].admit_s = ta;
+        launch_tick(S, *free_slot, reqs, take, is_batched);
+        free_slot->busy = true;
+        free_slot->reqs = take;
+        free_slot->mem = tick_mem;
+        inflight += tick_mem;
+    }
+}
+
+template <class F>
+static mifft_status sguard(F&& f) {
+    try {
+        f();
+        return MIFFT_SUCCESS;
+    } catch (const Error& e) {
+        set_last_detail(e.detail);
+        return e.code;
+    } catch (...) {
+        set_last_detail("unexpected internal error");
+        return MIFFT_ERR_HIP;
+    }
+}
+
+extern "C" {
+
+mifft_status mifft_server_create(uint64_t memory_budget_bytes, int32_t streams, int32_t max_tick_requests,
+                                 mifft_server** out) {
+    if (!out) return MIFFT_ERR_NULL_ARGUMENT;
+    *out = nullptr;
+    if (streams < 1 || streams > 4 || max_tick_requests < 1 || memory_budget_bytes == 0) {
+        set_last_detail("streams must be 1..4, max_tick_requests >= 1, budget > 0");
+        return MIFFT_ERR_INVALID_ENUM;
+    }
+    return sguard([&] {
+        auto s = std::make_unique<mifft_server>();
+        s->budget = memory_budget_bytes;
+        s->nslots = streams;
+        s->max_items = max_tick_requests;
+        *out = s.release();
+    });
+}
+
+mifft_status mifft_server_prepare(mifft_server* s, const mifft_server_request* reqs, int32_t n) {
+    if (!s || (!reqs && n)) return MIFFT_ERR_NULL_ARGUMENT;
+    return sguard([&] { prepare(*s, reqs, n); });
+}
+
+mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* reqs, const double* arrival_s,
+                                 int32_t n, int32_t mode, mifft_server_result* results) {
+    if (!s || !reqs || !arrival_s || !results) return MIFFT_ERR_NULL_ARGUMENT;
+    if (mode < 0 || mode > 1) return MIFFT_ERR_INVALID_ENUM;
+    return sguard([&] {
+        prepare(*s, reqs, n);
+        replay(*s, reqs, arrival_s, n, mode, results);
+        hip_check(hipDeviceSynchronize(), "replay end");
+    });
+}
+
+void mifft_server_destroy(mifft_server* s) { delete s; }
+
+}  // extern "C"
