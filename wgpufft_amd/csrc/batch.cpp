@@ -1,12 +1,11 @@
 // Host planner of the persistent batched launch (mifft_batch_*).
 //
-// Each item (a 1D f32 transform of `n` points over `batch` lines) reuses the
-// planner of standalone plans in batched mode (256-thread kernels, a 40 KiB
-// LDS budget so that four workgroups share a CU); plans are cached by
-// (n, direction, normalization, algorithm), so the twiddle, chirp and filter
-// tables of equal lengths are shared.  Every op of every item becomes a run of
-// tiles; tiles are sorted by cost (longest first) inside their phase and
-// pulled by the persistent kernel from a device counter.
+// Each transform reuses the planner of standalone plans in batched mode
+// (256-thread kernels, a 40 KiB LDS budget so that four workgroups share a
+// CU).  Plans are cached, so the twiddle, chirp and filter tables of equal
+// shapes are shared.  Op k of every transform becomes a run of tiles in phase
+// k; tiles are sorted by cost (longest first) inside their phase and pulled by
+// the persistent kernel from a device counter.
 #include <algorithm>
 #include <cstring>
 
@@ -16,10 +15,34 @@ namespace mifft {
 
 BatchPlanCache::BatchPlanCache() { mifft_tuning_init(&tuning); }
 
-const Plan* BatchPlanCache::get(uint64_t n, int direction, int normalization, int algorithm) {
+static std::string batch_reject_reason(const Plan& p) {
+    if (p.ops.size() > (size_t)kMaxPhases) return "more than " + std::to_string(kMaxPhases) + " dependent passes";
+    for (const Op& op : p.ops) {
+        bool ok = false;
+        switch (op.kind) {
+            case OP_REG: ok = batch_class_for(op.rv) >= 0; break;
+            case OP_LDS: ok = batch_lds_class(op.prec, op.ldm, op.stm, op.post) >= 0; break;
+            case OP_RADER:
+            case OP_BLUESTEIN: ok = op.prec == P_F32 || op.prec == P_F64; break;
+            default: ok = false;
+        }
+        if (!ok) return "no batched class for " + op.desc;
+    }
+    return "";
+}
+
+static std::unique_ptr<Plan> build_batched(const mifft_config& c, mifft_kind kind) {
+    PlanOptions opt;
+    opt.lds_budget = kBatchLdsBudget;
+    opt.batch_mode = true;
+    return build_plan(c, kind, opt);
+}
+
+const Plan* BatchPlanCache::get_1d(uint64_t n, int direction, int normalization, int algorithm,
+                                   std::string* why) {
     auto key = std::make_tuple(n, direction, normalization, algorithm);
-    auto f = plans.find(key);
-    if (f != plans.end()) return f->second.get();
+    auto f = plans_1d.find(key);
+    if (f != plans_1d.end()) return f->second.get();
     mifft_config c;
     std::memset(&c, 0, sizeof(c));
     c.rank = 1;
@@ -39,74 +62,87 @@ const Plan* BatchPlanCache::get(uint64_t n, int direction, int normalization, in
         c.tuning.force_bluestein_axes[0] = 0;
         c.tuning.n_force_bluestein_axes = 1;
     }
-    PlanOptions opt;
-    opt.lds_budget = kBatchLdsBudget;
-    opt.batch_mode = true;
-    std::unique_ptr<Plan> p = build_plan(c, MIFFT_C2C, opt);
-    if (p->ops.size() > 2)
-        fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "length " + std::to_string(n) + " needs more than two passes");
-    for (const Op& op : p->ops) {
-        const bool ok = (op.kind == OP_REG && batch_class_for(op.rv) >= 0) ||
-                        (op.kind == OP_LDS && op.ldm == LD_COMPLEX && op.stm == ST_COMPLEX) ||
-                        op.kind == OP_RADER || op.kind == OP_BLUESTEIN;
-        if (!ok)
-            fail(MIFFT_ERR_UNSUPPORTED_LENGTH,
-                 "length " + std::to_string(n) + " needs a route the batched kernel does not run (" + op.desc + ")");
-    }
-    const Plan* r = p.get();
-    plans[key] = std::move(p);
-    return r;
-}
-
-static void check_item(const mifft_batch_item& it) {
-    if (it.n == 0) fail(MIFFT_ERR_ZERO_LENGTH, "batch item: zero length");
-    if (it.batch == 0) fail(MIFFT_ERR_ZERO_BATCH, "batch item: zero batch");
-    if (!it.in || !it.out) fail(MIFFT_ERR_NULL_ARGUMENT, "batch item: null buffer");
-    if (((uintptr_t)it.in | (uintptr_t)it.out) & 15u)
-        fail(MIFFT_ERR_UNALIGNED, "batch item: buffers must be 16-byte aligned");
-    if (it.algorithm < 0 || it.algorithm > 2 || it.direction < 0 || it.direction > 1 || it.normalization < 0 ||
-        it.normalization > 3)
-        fail(MIFFT_ERR_INVALID_ENUM, "batch item: enum out of range");
-    if ((unsigned __int128)it.n * it.batch > 0xFFFFFFFFull)
-        fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batch item exceeds u32::MAX elements");
-}
-
-uint64_t batch_item_workspace(BatchPlanCache& cache, const mifft_batch_item& it) {
-    check_item(it);
-    const Plan* P = cache.get(it.n, it.direction, it.normalization, it.algorithm);
-    return P->ws_bytes * it.batch;
-}
-
-bool batch_item_supported(BatchPlanCache& cache, const mifft_batch_item& it) {
+    std::unique_ptr<Plan> p;
     try {
-        batch_item_workspace(cache, it);
-        return true;
-    } catch (const Error&) {
-        return false;
+        p = build_batched(c, MIFFT_C2C);
+    } catch (const Error& e) {
+        if (e.code != MIFFT_ERR_UNSUPPORTED_LENGTH) throw;
+        if (why) *why = e.detail;
+        return nullptr;
     }
+    const std::string r = batch_reject_reason(*p);
+    if (!r.empty()) {
+        if (why) *why = "length " + std::to_string(n) + ": " + r;
+        return nullptr;
+    }
+    const Plan* out = p.get();
+    plans_1d[key] = std::move(p);
+    return out;
 }
 
-void batch_build(BatchPlanCache& cache, const mifft_batch_item* items, int32_t count, char* ws_base,
-                 BatchBuild& out) {
-    const size_t first[2] = {out.tiles[0].size(), out.tiles[1].size()};
+static std::string config_key(const mifft_config& c, mifft_kind kind) {
+    std::string k = std::to_string((int)kind) + "|";
+    for (int i = 0; i < c.rank; i++) k += std::to_string(c.shape[i]) + ",";
+    k += "|";
+    for (int i = 0; i < c.naxes; i++) k += std::to_string(c.axes[i]) + ",";
+    k += "|" + std::to_string(c.batch) + "|" + std::to_string(c.direction) + "|" + std::to_string(c.normalization) +
+         "|" + std::to_string(c.precision);
+    for (int i = 0; i < c.tuning.n_force_rader_axes; i++) k += "|r" + std::to_string(c.tuning.force_rader_axes[i]);
+    for (int i = 0; i < c.tuning.n_force_bluestein_axes; i++)
+        k += "|b" + std::to_string(c.tuning.force_bluestein_axes[i]);
+    return k;
+}
+
+const Plan* BatchPlanCache::get(const mifft_config& c, mifft_kind kind, std::string* why) {
+    const std::string key = config_key(c, kind);
+    auto f = plans.find(key);
+    if (f != plans.end()) return f->second.get();
+    auto rj = rejected.find(key);
+    if (rj != rejected.end()) {
+        if (why) *why = rj->second;
+        return nullptr;
+    }
+    std::unique_ptr<Plan> p;
+    try {
+        p = build_batched(c, kind);
+    } catch (const Error& e) {
+        if (e.code != MIFFT_ERR_UNSUPPORTED_LENGTH && e.code != MIFFT_ERR_PRECISION_UNSUPPORTED) throw;
+        rejected[key] = e.detail;
+        if (why) *why = e.detail;
+        return nullptr;
+    }
+    const std::string r = batch_reject_reason(*p);
+    if (!r.empty()) {
+        rejected[key] = r;
+        if (why) *why = r;
+        return nullptr;
+    }
+    const Plan* out = p.get();
+    plans[key] = std::move(p);
+    return out;
+}
+
+void batch_build(const BatchEntry* entries, int32_t count, char* ws_base, BatchBuild& out) {
+    size_t first[kMaxPhases];
+    for (int k = 0; k < kMaxPhases; k++) first[k] = out.tiles[k].size();
     for (int32_t i = 0; i < count; i++) {
-        const mifft_batch_item& it = items[i];
-        check_item(it);
-        const Plan& P = *cache.get(it.n, it.direction, it.normalization, it.algorithm);
+        const BatchEntry& e = entries[i];
+        const Plan& P = *e.plan;
         char* ws = ws_base + out.ws_bytes;
-        out.ws_bytes += P.ws_bytes * it.batch;
+        out.ws_bytes += P.ws_bytes * e.mult;
         for (size_t j = 0; j < P.ops.size(); j++) {
             const Op& op = P.ops[j];
-            // Plans are built for one line: workspace offsets scale with the line count.
+            // Workspace offsets of a plan built for one line scale with the line count.
             auto bind = [&](int ref, uint64_t off) -> char* {
-                if (ref == B_IN) return (char*)it.in + off;
-                if (ref == B_OUT) return (char*)it.out + off;
-                return ws + off * it.batch;
+                if (ref == B_IN) return (char*)e.in + off;
+                if (ref == B_OUT) return (char*)e.out + off;
+                return ws + off * e.mult;
             };
             OpDesc d;
             std::memset(&d, 0, sizeof(d));
             uint32_t cls = 0, lpw = 1;
             size_t op_lds = 0;
+            const size_t es = P.arith == P_F32 ? 8 : 16;
             switch (op.kind) {
                 case OP_REG:
                     cls = (uint32_t)batch_class_for(op.rv);
@@ -115,22 +151,23 @@ void batch_build(BatchPlanCache& cache, const mifft_batch_item* items, int32_t c
                     op_lds = op.rv->lds_bytes;
                     break;
                 case OP_LDS:
-                    cls = op.post ? BC_LDS_POST : BC_LDS;
+                    cls = (uint32_t)batch_lds_class(op.prec, op.ldm, op.stm, op.post);
                     d.a = op.a;
                     d.sc = op.sc;
                     lpw = op.sc.lpw;
-                    op_lds = lds_kernel_bytes(P_F32, op.sc.n, op.sc.lpw);
+                    op_lds = lds_kernel_bytes(op.prec, op.sc.n, op.sc.lpw);
                     break;
-                default:  // OP_RADER, OP_BLUESTEIN (BatchPlanCache::get admits nothing else)
-                    cls = op.kind == OP_RADER ? BC_RADER : BC_BLUESTEIN;
+                default:  // OP_RADER, OP_BLUESTEIN (the caches admit nothing else)
+                    if (op.prec == P_F64) cls = op.kind == OP_RADER ? BC_RADER_F64 : BC_BLUESTEIN_F64;
+                    else cls = op.kind == OP_RADER ? BC_RADER : BC_BLUESTEIN;
                     d.conv = op.conv;
                     lpw = op.conv.sc.lpw;
-                    op_lds = conv_kernel_bytes(P_F32, op.conv.m, op.conv.sc.lpw);
+                    op_lds = conv_kernel_bytes(op.prec, op.conv.m, op.conv.sc.lpw);
                     break;
             }
             LineArgs& la = (op.kind == OP_RADER || op.kind == OP_BLUESTEIN) ? d.conv.io : d.a;
-            const unsigned __int128 L = (unsigned __int128)la.lines * it.batch;
-            if (L > 0x7FFFFFFFu) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batch item lines exceed 2^31");
+            const unsigned __int128 L = (unsigned __int128)la.lines * e.mult;
+            if (L > 0x7FFFFFFFu) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batched transform lines exceed 2^31");
             la.lines = (uint32_t)L;
             la.in = bind(op.src, op.src_off);
             la.out = bind(op.dst, op.dst_off);
@@ -138,13 +175,17 @@ void batch_build(BatchPlanCache& cache, const mifft_batch_item* items, int32_t c
             const uint32_t arg = (uint32_t)out.ops.size();
             out.ops.push_back(d);
             const uint32_t ntiles = (la.lines + lpw - 1) / lpw;
-            const uint64_t c64 = (uint64_t)lpw * it.n * (op.kind == OP_BLUESTEIN || op.kind == OP_RADER ? 3 : 1);
+            // scheduling weight: bytes the tile touches (convolutions count 3x)
+            uint64_t c64 = (uint64_t)lpw * es;
+            c64 *= (op.kind == OP_RADER || op.kind == OP_BLUESTEIN) ? 3ull * op.conv.m
+                   : op.kind == OP_LDS                             ? op.sc.n
+                                                                   : (uint64_t)op.rv->n;
             const uint32_t cost = (uint32_t)std::min<uint64_t>(c64, 0xFFFFFFFFu);
             for (uint32_t t = 0; t < ntiles; t++) out.tiles[j].push_back(BatchTile{cls, arg, t * lpw, cost});
         }
     }
-    for (int p = 0; p < 2; p++)
-        std::stable_sort(out.tiles[p].begin() + first[p], out.tiles[p].end(),
+    for (int k = 0; k < kMaxPhases; k++)
+        std::stable_sort(out.tiles[k].begin() + first[k], out.tiles[k].end(),
                          [](const BatchTile& a, const BatchTile& b) { return a.cost > b.cost; });
 }
 
@@ -159,13 +200,14 @@ uint32_t batch_grid(size_t lds_bytes) {
     return (uint32_t)(cus * per_cu);
 }
 
-void batch_enqueue(const BatchTile* d_tiles0, uint32_t n0, const BatchTile* d_tiles1, uint32_t n1,
-                   const OpDesc* d_ops, uint32_t* d_counters, uint32_t grid, size_t lds_bytes,
-                   hipStream_t s) {
-    hip_check(launch_batch_reset(d_counters, 2, s), "batch reset");
-    if (n0) hip_check(launch_batch(d_tiles0, n0, d_counters, d_ops, std::min(grid, n0), lds_bytes, s), "batch");
-    if (n1)
-        hip_check(launch_batch(d_tiles1, n1, d_counters + 1, d_ops, std::min(grid, n1), lds_bytes, s), "batch");
+void batch_enqueue(const BatchTile* const* d_tiles, const uint32_t* counts, const OpDesc* d_ops,
+                   uint32_t* d_counters, uint32_t grid, size_t lds_bytes, hipStream_t s) {
+    hip_check(launch_batch_reset(d_counters, kMaxPhases, s), "batch reset");
+    for (int k = 0; k < kMaxPhases; k++)
+        if (counts[k])
+            hip_check(launch_batch(d_tiles[k], counts[k], d_counters + k, d_ops, std::min(grid, counts[k]),
+                                   lds_bytes, s),
+                      "batch");
 }
 
 }  // namespace mifft
@@ -176,7 +218,7 @@ struct mifft_batch {
     BatchPlanCache cache;
     BatchBuild build;
     OpDesc* d_ops = nullptr;
-    BatchTile* d_tiles[2] = {nullptr, nullptr};
+    BatchTile* d_tiles[kMaxPhases] = {};
     uint32_t* d_counters = nullptr;
     void* ws = nullptr;
     uint32_t grid = 0;
@@ -193,27 +235,50 @@ struct mifft_batch {
     }
 };
 
+static void check_item(const mifft_batch_item& it) {
+    if (it.n == 0) fail(MIFFT_ERR_ZERO_LENGTH, "batch item: zero length");
+    if (it.batch == 0) fail(MIFFT_ERR_ZERO_BATCH, "batch item: zero batch");
+    if (!it.in || !it.out) fail(MIFFT_ERR_NULL_ARGUMENT, "batch item: null buffer");
+    if (((uintptr_t)it.in | (uintptr_t)it.out) & 15u)
+        fail(MIFFT_ERR_UNALIGNED, "batch item: buffers must be 16-byte aligned");
+    if (it.algorithm < 0 || it.algorithm > 2 || it.direction < 0 || it.direction > 1 || it.normalization < 0 ||
+        it.normalization > 3)
+        fail(MIFFT_ERR_INVALID_ENUM, "batch item: enum out of range");
+    if ((unsigned __int128)it.n * it.batch > 0xFFFFFFFFull)
+        fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batch item exceeds u32::MAX elements");
+}
+
 static void create_batch(mifft_batch& B, const mifft_batch_item* items, int32_t count, const mifft_tuning* t) {
     if (t) B.cache.tuning = *t;
+    std::vector<BatchEntry> entries;
     uint64_t ws = 0;
-    for (int32_t i = 0; i < count; i++) ws += batch_item_workspace(B.cache, items[i]);
+    for (int32_t i = 0; i < count; i++) {
+        const mifft_batch_item& it = items[i];
+        check_item(it);
+        std::string why;
+        const Plan* p = B.cache.get_1d(it.n, it.direction, it.normalization, it.algorithm, &why);
+        if (!p) fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "batch item " + std::to_string(i) + ": " + why);
+        entries.push_back(BatchEntry{p, it.batch, it.in, it.out});
+        ws += p->ws_bytes * it.batch;
+    }
     if (ws) hip_check(hipMalloc(&B.ws, ws), "batch workspace");
-    batch_build(B.cache, items, count, (char*)B.ws, B.build);
+    batch_build(entries.data(), (int32_t)entries.size(), (char*)B.ws, B.build);
     B.grid = batch_grid(B.build.lds_bytes);
     auto upload = [](void** dst, const void* src, size_t bytes, const char* what) {
         hip_check(hipMalloc(dst, std::max<size_t>(bytes, 16)), what);
         if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
     };
     upload((void**)&B.d_ops, B.build.ops.data(), B.build.ops.size() * sizeof(OpDesc), "batch ops");
-    for (int p = 0; p < 2; p++)
-        upload((void**)&B.d_tiles[p], B.build.tiles[p].data(), B.build.tiles[p].size() * sizeof(BatchTile),
+    for (int k = 0; k < kMaxPhases; k++)
+        upload((void**)&B.d_tiles[k], B.build.tiles[k].data(), B.build.tiles[k].size() * sizeof(BatchTile),
                "batch tiles");
     hip_check(hipMalloc(&B.d_counters, 64), "batch counters");
 }
 
 static void run_batch(const mifft_batch& B, hipStream_t s) {
-    batch_enqueue(B.d_tiles[0], (uint32_t)B.build.tiles[0].size(), B.d_tiles[1], (uint32_t)B.build.tiles[1].size(),
-                  B.d_ops, B.d_counters, B.grid, B.build.lds_bytes, s);
+    uint32_t counts[kMaxPhases];
+    for (int k = 0; k < kMaxPhases; k++) counts[k] = (uint32_t)B.build.tiles[k].size();
+    batch_enqueue(B.d_tiles, counts, B.d_ops, B.d_counters, B.grid, B.build.lds_bytes, s);
 }
 
 template <class F>
@@ -275,8 +340,8 @@ mifft_status mifft_batch_launch(mifft_batch* b, hipStream_t stream) {
 mifft_status mifft_batch_info(const mifft_batch* b, uint64_t* tiles, uint64_t* phases, uint64_t* lds_bytes,
                               uint64_t* workspace_bytes, uint32_t* grid) {
     if (!b) return MIFFT_ERR_NULL_ARGUMENT;
-    if (tiles) *tiles = b->build.tiles[0].size() + b->build.tiles[1].size();
-    if (phases) *phases = b->build.tiles[1].empty() ? 1 : 2;
+    if (tiles) *tiles = b->build.tile_count();
+    if (phases) *phases = (uint64_t)b->build.phases();
     if (lds_bytes) *lds_bytes = b->build.lds_bytes;
     if (workspace_bytes) *workspace_bytes = b->build.ws_bytes;
     if (grid) *grid = b->grid;

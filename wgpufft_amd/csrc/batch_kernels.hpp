@@ -27,8 +27,20 @@ enum BatchClass : uint32_t {
     BC_REG_T256,    // four-step row pass with transposed store, 256 points
     BC_REG_P128,
     BC_REG_T128,
+    BC_LDS_R2C,       // f32 real load, packed store (R2C axis 0)
+    BC_LDS_C2R,       // f32 Hermitian load, real store (C2R axis 0)
+    BC_LDS_F64,
+    BC_LDS_POST_F64,
+    BC_RADER_F64,
+    BC_BLUESTEIN_F64,
+    BC_REG_C256_F64,  // f64 256-point lines, contiguous
+    BC_REG_S256_F64,  // f64 256-point lines, strided
     BC_COUNT
 };
+
+// Dependent passes of one transform run in successive phases (one persistent
+// launch each); ND transforms and four-step splits need more than one.
+constexpr int kMaxPhases = 4;
 
 struct BatchTile {
     uint32_t cls;  // BatchClass
@@ -45,8 +57,11 @@ struct OpDesc {
 };
 
 // Class of a register variant usable inside the batched kernel (256 threads,
-// f32, complex in and out), or -1.
+// complex in and out), or -1.
 int batch_class_for(const RegVariant* v);
+// Class of a runtime-schedule LDS op (precision, load/store modes, twiddle on
+// store), or -1.
+int batch_lds_class(int prec, int ldm, int stm, int post);
 
 hipError_t launch_batch(const BatchTile* tiles, uint32_t ntiles, uint32_t* counter, const OpDesc* ops,
                         uint32_t grid, size_t lds_bytes, hipStream_t s);

@@ -24,15 +24,17 @@ template <class K>
 __device__ __noinline__ void reg_tile(const LineArgs& a, unsigned char* smem, uint32_t g0) {
     K::tile(a, reinterpret_cast<cpx<typename K::T>*>(smem), g0);
 }
-template <bool POST>
+template <class Fmt, int LDM, int STM, bool POST>
 __device__ __noinline__ void lds_tile_nc(const LineArgs& a, const Sched& sc, unsigned char* smem, uint32_t g0) {
-    lds_fft_tile<FmtF32, LD_COMPLEX, ST_COMPLEX, POST>(a, sc, smem, g0);
+    lds_fft_tile<Fmt, LDM, STM, POST>(a, sc, smem, g0);
 }
+template <class Fmt>
 __device__ __noinline__ void rader_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
-    rader_tile<FmtF32>(c, smem, g0);
+    rader_tile<Fmt>(c, smem, g0);
 }
+template <class Fmt>
 __device__ __noinline__ void bluestein_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
-    bluestein_tile<FmtF32>(c, smem, g0);
+    bluestein_tile<Fmt>(c, smem, g0);
 }
 
 #define MIFFT_BK(N, TPL, LINES, LDLF, STLF, POST, VEC, ...) \
@@ -53,10 +55,24 @@ __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ 
         const BatchTile tl = tiles[t];
         const OpDesc& d = ops[tl.arg];
         switch (tl.cls) {
-            case BC_LDS: lds_tile_nc<false>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_POST: lds_tile_nc<true>(d.a, d.sc, smem, tl.g0); break;
-            case BC_RADER: rader_tile_nc(d.conv, smem, tl.g0); break;
-            case BC_BLUESTEIN: bluestein_tile_nc(d.conv, smem, tl.g0); break;
+            case BC_LDS: lds_tile_nc<FmtF32, LD_COMPLEX, ST_COMPLEX, false>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_POST: lds_tile_nc<FmtF32, LD_COMPLEX, ST_COMPLEX, true>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_R2C: lds_tile_nc<FmtF32, LD_REAL, ST_PACKED, false>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_C2R: lds_tile_nc<FmtF32, LD_HERMITIAN, ST_REAL, false>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_F64: lds_tile_nc<FmtF64, LD_COMPLEX, ST_COMPLEX, false>(d.a, d.sc, smem, tl.g0); break;
+            case BC_LDS_POST_F64: lds_tile_nc<FmtF64, LD_COMPLEX, ST_COMPLEX, true>(d.a, d.sc, smem, tl.g0); break;
+            case BC_RADER: rader_tile_nc<FmtF32>(d.conv, smem, tl.g0); break;
+            case BC_BLUESTEIN: bluestein_tile_nc<FmtF32>(d.conv, smem, tl.g0); break;
+            case BC_RADER_F64: rader_tile_nc<FmtF64>(d.conv, smem, tl.g0); break;
+            case BC_BLUESTEIN_F64: bluestein_tile_nc<FmtF64>(d.conv, smem, tl.g0); break;
+            case BC_REG_C256_F64:
+                reg_tile<RegFft<FmtF64, 256, 16, 16, false, false, LD_COMPLEX, ST_COMPLEX, false, RadixList<16, 16>, 1>>(
+                    d.a, smem, tl.g0);
+                break;
+            case BC_REG_S256_F64:
+                reg_tile<RegFft<FmtF64, 256, 16, 16, true, true, LD_COMPLEX, ST_COMPLEX, false, RadixList<16, 16>, 1>>(
+                    d.a, smem, tl.g0);
+                break;
             // register-resident variants with 256 threads (the registry entries of the same shape)
             case BC_REG_C4096: reg_tile<MIFFT_BK(4096, 256, 1, 0, 0, 0, 1, 16, 16, 16)>(d.a, smem, tl.g0); break;
             case BC_REG_C2048: reg_tile<MIFFT_BK(2048, 128, 2, 0, 0, 0, 1, 16, 16, 8)>(d.a, smem, tl.g0); break;
@@ -96,9 +112,22 @@ hipError_t launch_batch_reset(uint32_t* counters, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-int batch_class_for(const RegVariant* v) {
-    if (!v || v->prec != P_F32 || v->nthreads != 256 || v->ldm != LD_COMPLEX || v->stm != ST_COMPLEX)
+int batch_lds_class(int prec, int ldm, int stm, int post) {
+    if (prec == P_F32) {
+        if (ldm == LD_COMPLEX && stm == ST_COMPLEX) return post ? BC_LDS_POST : BC_LDS;
+        if (!post && ldm == LD_REAL && stm == ST_PACKED) return BC_LDS_R2C;
+        if (!post && ldm == LD_HERMITIAN && stm == ST_REAL) return BC_LDS_C2R;
         return -1;
+    }
+    if (prec == P_F64 && ldm == LD_COMPLEX && stm == ST_COMPLEX) return post ? BC_LDS_POST_F64 : BC_LDS_F64;
+    return -1;
+}
+
+int batch_class_for(const RegVariant* v) {
+    if (!v || v->nthreads != 256 || v->ldm != LD_COMPLEX || v->stm != ST_COMPLEX) return -1;
+    if (v->prec == P_F64 && v->n == 256 && v->vec == 1 && !v->post)
+        return v->ld_lf == 0 && v->st_lf == 0 ? BC_REG_C256_F64 : (v->ld_lf && v->st_lf ? BC_REG_S256_F64 : -1);
+    if (v->prec != P_F32) return -1;
     const int key = v->n * 1000 + v->ld_lf * 100 + v->st_lf * 10 + v->post;
     if (v->vec == 1) {
         switch (key) {

@@ -7,9 +7,11 @@
 //   * whenever a stream slot is free, the loop forms a tick from the head of
 //     the queue, admitting requests while their memory (input + output +
 //     workspace) fits the budget next to the work already on the GPU;
-//   * a tick packs its 1D f32 C2C requests into one persistent batched launch
-//     (kern_batch.hip) and runs the others (f64, 2D, R2C, lengths the batched
-//     kernel does not take) through their resident plans, on the slot's stream;
+//   * a tick packs every request whose plan the batched kernel can run (1D and
+//     2D C2C in f32 and f64, R2C; up to four dependent passes) into one
+//     persistent batched launch per pass (kern_batch.hip) and runs the others
+//     (multi-pass Bluestein lengths, df64) through their resident plans, on
+//     the slot's stream;
 //   * ticks on different slots overlap on the device; an event per tick marks
 //     completion, which frees the slot and the memory.
 // Plans, descriptor arenas, pinned staging and workspaces are allocated once
@@ -85,20 +87,7 @@ struct mifft_server {
     }
 };
 
-static bool batchable(mifft_server& S, const mifft_server_request& r) {
-    if (r.kind != MIFFT_C2C || r.precision != MIFFT_F32 || r.rank != 1) return false;
-    mifft_batch_item it = {r.shape[0], r.batch, r.direction, r.normalization, 0, 0, r.in, r.out};
-    return batch_item_supported(S.bcache, it);
-}
-
-static mifft_batch_item item_of(const mifft_server_request& r) {
-    return mifft_batch_item{r.shape[0], r.batch, r.direction, r.normalization, 0, 0, r.in, r.out};
-}
-
-static Plan& standalone(mifft_server& S, const mifft_server_request& r) {
-    PlanKey k = key_of(r);
-    auto f = S.plans.find(k);
-    if (f != S.plans.end()) return *f->second;
+static mifft_config config_of(const mifft_server_request& r) {
     mifft_config c;
     std::memset(&c, 0, sizeof(c));
     if (r.rank < 1 || r.rank > 2) fail(MIFFT_ERR_RANK_TOO_LARGE, "server requests are 1D or 2D");
@@ -113,14 +102,26 @@ static Plan& standalone(mifft_server& S, const mifft_server_request& r) {
     c.normalization = r.normalization;
     c.precision = r.precision;
     mifft_tuning_init(&c.tuning);
-    std::unique_ptr<Plan> p = build_plan(c, (mifft_kind)r.kind);
+    return c;
+}
+
+// The batched-mode plan of a request, or nullptr if the batched kernel cannot run it.
+static const Plan* batched_plan(mifft_server& S, const mifft_server_request& r) {
+    return S.bcache.get(config_of(r), (mifft_kind)r.kind);
+}
+
+static Plan& standalone(mifft_server& S, const mifft_server_request& r) {
+    PlanKey k = key_of(r);
+    auto f = S.plans.find(k);
+    if (f != S.plans.end()) return *f->second;
+    std::unique_ptr<Plan> p = build_plan(config_of(r), (mifft_kind)r.kind);
     Plan& ref = *p;
     S.plans[k] = std::move(p);
     return ref;
 }
 
 static uint64_t request_ws(mifft_server& S, const mifft_server_request& r, bool batched) {
-    if (batched) return batch_item_workspace(S.bcache, item_of(r));
+    if (batched) return batched_plan(S, r)->ws_bytes;
     return standalone(S, r).ws_bytes;
 }
 
@@ -128,7 +129,7 @@ static void prepare(mifft_server& S, const mifft_server_request* reqs, int32_t n
     uint64_t max_plan_ws = 0;
     for (int32_t i = 0; i < n; i++) {
         const mifft_server_request& r = reqs[i];
-        if (batchable(S, r)) request_ws(S, r, true);
+        batched_plan(S, r);
         // the sequential baseline runs every request through a standalone plan
         max_plan_ws = std::max(max_plan_ws, standalone(S, r).ws_bytes);
     }
@@ -160,28 +161,34 @@ static void prepare(mifft_server& S, const mifft_server_request* reqs, int32_t n
 // Launches one tick on a free slot.  Returns the number of requests taken.
 static void launch_tick(mifft_server& S, Slot& sl, const mifft_server_request* reqs, const std::vector<int>& take,
                         const std::vector<char>& is_batched) {
-    std::vector<mifft_batch_item> items;
+    std::vector<BatchEntry> entries;
     std::vector<int> plan_reqs;
     for (int r : take) {
-        if (is_batched[r]) items.push_back(item_of(reqs[r]));
+        if (is_batched[r]) entries.push_back(BatchEntry{batched_plan(S, reqs[r]), 1, reqs[r].in, reqs[r].out});
         else plan_reqs.push_back(r);
     }
     BatchBuild b;
-    if (!items.empty()) batch_build(S.bcache, items.data(), (int32_t)items.size(), sl.ws, b);
+    if (!entries.empty()) batch_build(entries.data(), (int32_t)entries.size(), sl.ws, b);
     char* plan_ws = sl.ws + b.ws_bytes;
     const size_t ops_bytes = b.ops.size() * sizeof(OpDesc);
-    const size_t t0_bytes = b.tiles[0].size() * sizeof(BatchTile);
-    const size_t t1_bytes = b.tiles[1].size() * sizeof(BatchTile);
-    const size_t total = ops_bytes + t0_bytes + t1_bytes;
-    if (!items.empty()) {
+    size_t total = ops_bytes;
+    for (auto& v : b.tiles) total += v.size() * sizeof(BatchTile);
+    if (!entries.empty()) {
         if (total > sl.desc_cap) fail(MIFFT_ERR_OVER_BUDGET, "tick descriptors exceed the arena");
         std::memcpy(sl.staging, b.ops.data(), ops_bytes);
-        std::memcpy(sl.staging + ops_bytes, b.tiles[0].data(), t0_bytes);
-        std::memcpy(sl.staging + ops_bytes + t0_bytes, b.tiles[1].data(), t1_bytes);
+        const BatchTile* d_tiles[kMaxPhases];
+        uint32_t counts[kMaxPhases];
+        size_t off = ops_bytes;
+        for (int k = 0; k < kMaxPhases; k++) {
+            const size_t tb = b.tiles[k].size() * sizeof(BatchTile);
+            std::memcpy(sl.staging + off, b.tiles[k].data(), tb);
+            d_tiles[k] = (const BatchTile*)(sl.desc + off);
+            counts[k] = (uint32_t)b.tiles[k].size();
+            off += tb;
+        }
         hip_check(hipMemcpyAsync(sl.desc, sl.staging, total, hipMemcpyHostToDevice, sl.s), "descriptor upload");
-        batch_enqueue((const BatchTile*)(sl.desc + ops_bytes), (uint32_t)b.tiles[0].size(),
-                      (const BatchTile*)(sl.desc + ops_bytes + t0_bytes), (uint32_t)b.tiles[1].size(),
-                      (const OpDesc*)sl.desc, sl.counters, batch_grid(b.lds_bytes), b.lds_bytes, sl.s);
+        batch_enqueue(d_tiles, counts, (const OpDesc*)sl.desc, sl.counters, batch_grid(b.lds_bytes), b.lds_bytes,
+                      sl.s);
     }
     for (int r : plan_reqs) run_plan(standalone(S, reqs[r]), sl.s, reqs[r].in, reqs[r].out, plan_ws);
     hip_check(hipEventRecord(sl.ev, sl.s), "tick event");
@@ -195,7 +202,7 @@ static void replay(mifft_server& S, const mifft_server_request* reqs, const doub
     std::vector<char> is_batched(n, 0);
     std::vector<uint64_t> mem(n, 0);
     for (int32_t i = 0; i < n; i++) {
-        is_batched[i] = mode == 0 && batchable(S, reqs[i]);
+        is_batched[i] = mode == 0 && batched_plan(S, reqs[i]) != nullptr;
         mem[i] = reqs[i].in_bytes + reqs[i].out_bytes + request_ws(S, reqs[i], is_batched[i]);
         res[i].path = is_batched[i] ? 0 : 1;
     }
