@@ -15,18 +15,32 @@ namespace mifft {
 
 BatchPlanCache::BatchPlanCache() { mifft_tuning_init(&tuning); }
 
-static std::string batch_reject_reason(const Plan& p) {
-    if (p.ops.size() > (size_t)kMaxPhases) return "more than " + std::to_string(kMaxPhases) + " dependent passes";
+// Ops of a plan in launch order, child plans (the padded transforms of a
+// multi-pass Bluestein axis) expanded in place.
+static void flat_ops(const Plan& p, std::vector<const Op*>& out) {
     for (const Op& op : p.ops) {
+        if (op.kind == OP_SUBPLAN) flat_ops(*op.sub, out);
+        else out.push_back(&op);
+    }
+}
+
+static std::string batch_reject_reason(const Plan& p) {
+    std::vector<const Op*> ops;
+    flat_ops(p, ops);
+    if (ops.size() > (size_t)kMaxPhases) return "more than " + std::to_string(kMaxPhases) + " dependent passes";
+    for (const Op* op : ops) {
         bool ok = false;
-        switch (op.kind) {
-            case OP_REG: ok = batch_class_for(op.rv) >= 0; break;
-            case OP_LDS: ok = batch_lds_class(op.prec, op.ldm, op.stm, op.post) >= 0; break;
+        switch (op->kind) {
+            case OP_REG: ok = batch_class_for(op->rv) >= 0; break;
+            case OP_LDS: ok = batch_lds_class(op->prec, op->ldm, op->stm, op->post) >= 0; break;
             case OP_RADER:
-            case OP_BLUESTEIN: ok = op.prec == P_F32 || op.prec == P_F64; break;
+            case OP_BLUESTEIN:
+            case OP_CHIRP_PRE:
+            case OP_POINTWISE:
+            case OP_CHIRP_POST: ok = op->prec == P_F32 || op->prec == P_F64; break;
             default: ok = false;
         }
-        if (!ok) return "no batched class for " + op.desc;
+        if (!ok) return "no batched class for " + op->desc;
     }
     return "";
 }
@@ -122,33 +136,58 @@ const Plan* BatchPlanCache::get(const mifft_config& c, mifft_kind kind, std::str
     return out;
 }
 
+// Device addresses of a plan's buffer references.  A plan built for one line
+// and run over `mult` lines keeps its input and output bases and scales its
+// workspace offsets (and everything inside the workspace) by `mult`.
+struct Binding {
+    char* base[3];
+    uint64_t scale[3];
+    char* at(int ref, uint64_t off) const { return base[ref] + off * scale[ref]; }
+};
+
+static void bind_ops(const Plan& p, const Binding& b, uint64_t mult,
+                     std::vector<std::pair<const Op*, Binding>>& out) {
+    for (const Op& op : p.ops) {
+        if (op.kind == OP_SUBPLAN) {
+            char* buf = b.at(op.dst, op.dst_off);
+            const Binding cb{{buf, buf, b.at(B_WS, op.sub_ws_off)}, {mult, mult, mult}};
+            bind_ops(*op.sub, cb, mult, out);
+        } else {
+            out.emplace_back(&op, b);
+        }
+    }
+}
+
 void batch_build(const BatchEntry* entries, int32_t count, char* ws_base, BatchBuild& out) {
     size_t first[kMaxPhases];
     for (int k = 0; k < kMaxPhases; k++) first[k] = out.tiles[k].size();
+    std::vector<std::pair<const Op*, Binding>> flat;
     for (int32_t i = 0; i < count; i++) {
         const BatchEntry& e = entries[i];
         const Plan& P = *e.plan;
         char* ws = ws_base + out.ws_bytes;
         out.ws_bytes += P.ws_bytes * e.mult;
-        for (size_t j = 0; j < P.ops.size(); j++) {
-            const Op& op = P.ops[j];
-            // Workspace offsets of a plan built for one line scale with the line count.
-            auto bind = [&](int ref, uint64_t off) -> char* {
-                if (ref == B_IN) return (char*)e.in + off;
-                if (ref == B_OUT) return (char*)e.out + off;
-                return ws + off * e.mult;
-            };
+        flat.clear();
+        bind_ops(P, Binding{{(char*)e.in, (char*)e.out, ws}, {1, 1, e.mult}}, e.mult, flat);
+        if (flat.size() > (size_t)kMaxPhases) fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "too many batched passes");
+        for (size_t j = 0; j < flat.size(); j++) {
+            const Op& op = *flat[j].first;
+            const Binding& bd = flat[j].second;
             OpDesc d;
             std::memset(&d, 0, sizeof(d));
             uint32_t cls = 0, lpw = 1;
             size_t op_lds = 0;
             const size_t es = P.arith == P_F32 ? 8 : 16;
+            const bool f64 = op.prec == P_F64;
+            bool elt = false;
+            uint64_t unit = 0;  // points one line of the op touches (scheduling weight)
             switch (op.kind) {
                 case OP_REG:
                     cls = (uint32_t)batch_class_for(op.rv);
                     d.a = op.a;
                     lpw = (uint32_t)op.rv->lines;
                     op_lds = op.rv->lds_bytes;
+                    unit = op.rv->n;
                     break;
                 case OP_LDS:
                     cls = (uint32_t)batch_lds_class(op.prec, op.ldm, op.stm, op.post);
@@ -156,37 +195,81 @@ void batch_build(const BatchEntry* entries, int32_t count, char* ws_base, BatchB
                     d.sc = op.sc;
                     lpw = op.sc.lpw;
                     op_lds = lds_kernel_bytes(op.prec, op.sc.n, op.sc.lpw);
+                    unit = op.sc.n;
                     break;
-                default:  // OP_RADER, OP_BLUESTEIN (the caches admit nothing else)
-                    if (op.prec == P_F64) cls = op.kind == OP_RADER ? BC_RADER_F64 : BC_BLUESTEIN_F64;
+                case OP_RADER:
+                case OP_BLUESTEIN:
+                    if (f64) cls = op.kind == OP_RADER ? BC_RADER_F64 : BC_BLUESTEIN_F64;
                     else cls = op.kind == OP_RADER ? BC_RADER : BC_BLUESTEIN;
                     d.conv = op.conv;
                     lpw = op.conv.sc.lpw;
                     op_lds = conv_kernel_bytes(op.prec, op.conv.m, op.conv.sc.lpw);
+                    unit = 3ull * op.conv.m;  // convolutions count 3x
                     break;
+                case OP_CHIRP_PRE:
+                case OP_CHIRP_POST:
+                    cls = op.kind == OP_CHIRP_PRE ? (f64 ? BC_CHIRP_PRE_F64 : BC_CHIRP_PRE)
+                                                  : (f64 ? BC_CHIRP_POST_F64 : BC_CHIRP_POST);
+                    d.conv.io = op.a;
+                    d.conv.chirp = op.chirp;
+                    d.conv.n = op.n;
+                    d.conv.m = op.m;
+                    elt = true;
+                    break;
+                case OP_POINTWISE:
+                    cls = f64 ? BC_POINTWISE_F64 : BC_POINTWISE;
+                    d.conv.io.lines = (uint32_t)op.count;
+                    d.conv.filt = op.filt;
+                    d.conv.m = op.m;
+                    elt = true;
+                    break;
+                default: fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "no batched class for " + op.desc);
             }
-            LineArgs& la = (op.kind == OP_RADER || op.kind == OP_BLUESTEIN) ? d.conv.io : d.a;
+            const bool conv = op.kind != OP_REG && op.kind != OP_LDS;
+            LineArgs& la = conv ? d.conv.io : d.a;
             const unsigned __int128 L = (unsigned __int128)la.lines * e.mult;
             if (L > 0x7FFFFFFFu) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batched transform lines exceed 2^31");
             la.lines = (uint32_t)L;
-            la.in = bind(op.src, op.src_off);
-            la.out = bind(op.dst, op.dst_off);
+            if (op.kind == OP_POINTWISE) {
+                la.in = la.out = bd.at(op.dst, op.dst_off);
+            } else {
+                la.in = bd.at(op.src, op.src_off);
+                la.out = bd.at(op.dst, op.dst_off);
+            }
             out.lds_bytes = std::max(out.lds_bytes, op_lds);
             const uint32_t arg = (uint32_t)out.ops.size();
             out.ops.push_back(d);
-            const uint32_t ntiles = (la.lines + lpw - 1) / lpw;
-            // scheduling weight: bytes the tile touches (convolutions count 3x)
-            uint64_t c64 = (uint64_t)lpw * es;
-            c64 *= (op.kind == OP_RADER || op.kind == OP_BLUESTEIN) ? 3ull * op.conv.m
-                   : op.kind == OP_LDS                             ? op.sc.n
-                                                                   : (uint64_t)op.rv->n;
+            uint32_t ntiles, step;
+            uint64_t c64;
+            if (elt) {
+                // element-wise tiles index lines x m (lines x n for the chirp post) in 32 bits
+                const uint64_t total = (uint64_t)la.lines * (op.kind == OP_CHIRP_POST ? op.n : op.m);
+                if (total > 0xFFFFFFFFull - kEltTile)
+                    fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batched Bluestein workspace exceeds 2^32 points");
+                step = kEltTile;
+                ntiles = (uint32_t)((total + kEltTile - 1) / kEltTile);
+                c64 = (uint64_t)kEltTile * es * 2;
+            } else {
+                step = lpw;
+                ntiles = (la.lines + lpw - 1) / lpw;
+                c64 = (uint64_t)lpw * es * unit;
+            }
             const uint32_t cost = (uint32_t)std::min<uint64_t>(c64, 0xFFFFFFFFu);
-            for (uint32_t t = 0; t < ntiles; t++) out.tiles[j].push_back(BatchTile{cls, arg, t * lpw, cost});
+            for (uint32_t t = 0; t < ntiles; t++) out.tiles[j].push_back(BatchTile{cls, arg, t * step, cost});
         }
     }
     for (int k = 0; k < kMaxPhases; k++)
         std::stable_sort(out.tiles[k].begin() + first[k], out.tiles[k].end(),
                          [](const BatchTile& a, const BatchTile& b) { return a.cost > b.cost; });
+}
+
+uint64_t batch_desc_bytes(const Plan& p, uint64_t mult) {
+    // Bind to placeholder addresses: only the descriptor and tile counts are used.
+    static char placeholder[16];
+    const BatchEntry e{&p, mult, placeholder, placeholder};
+    BatchBuild b;
+    batch_build(&e, 1, placeholder, b);
+    return b.ops.size() * sizeof(OpDesc) + b.tile_count() * sizeof(BatchTile);
 }
 
 uint32_t batch_grid(size_t lds_bytes) {

@@ -35,12 +35,23 @@ enum BatchClass : uint32_t {
     BC_BLUESTEIN_F64,
     BC_REG_C256_F64,  // f64 256-point lines, contiguous
     BC_REG_S256_F64,  // f64 256-point lines, strided
+    // element-wise steps of a multi-pass Bluestein convolution (tile = a run
+    // of kEltTile elements)
+    BC_CHIRP_PRE,
+    BC_CHIRP_PRE_F64,
+    BC_POINTWISE,
+    BC_POINTWISE_F64,
+    BC_CHIRP_POST,
+    BC_CHIRP_POST_F64,
     BC_COUNT
 };
 
 // Dependent passes of one transform run in successive phases (one persistent
-// launch each); ND transforms and four-step splits need more than one.
-constexpr int kMaxPhases = 4;
+// launch each); ND transforms, four-step splits and multi-pass Bluestein
+// (chirp, padded FFT, filter, padded FFT, chirp) need more than one.
+constexpr int kMaxPhases = 8;
+// Elements per tile of the element-wise classes (16 per thread).
+constexpr uint32_t kEltTile = 4096;
 
 struct BatchTile {
     uint32_t cls;  // BatchClass
@@ -53,7 +64,7 @@ struct BatchTile {
 struct OpDesc {
     LineArgs a;   // line ops (register and LDS classes)
     Sched sc;     // LDS classes
-    ConvArgs conv;  // Rader / Bluestein classes
+    ConvArgs conv;  // Rader / Bluestein classes; element-wise classes (io, chirp, filt, n, m)
 };
 
 // Class of a register variant usable inside the batched kernel (256 threads,

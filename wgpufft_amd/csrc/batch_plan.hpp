@@ -60,6 +60,8 @@ struct BatchBuild {
 // Appends the entries' ops and tiles to `out`, binding workspace inside
 // [ws_base, ws_base + growth of out.ws_bytes).  Tiles are sorted by cost.
 void batch_build(const BatchEntry* e, int32_t count, char* ws_base, BatchBuild& out);
+// Device descriptor bytes (ops and tiles) one entry adds to a batch.
+uint64_t batch_desc_bytes(const Plan& p, uint64_t mult);
 // Persistent workgroups for a given LDS footprint: CUs x workgroups per CU.
 uint32_t batch_grid(size_t lds_bytes);
 // Enqueues the counter reset and one persistent launch per non-empty phase;

@@ -153,4 +153,41 @@ __device__ void bluestein_tile(const ConvArgs& c, unsigned char* smem_raw, uint3
     }
 }
 
+// Element-wise steps of a multi-pass Bluestein convolution (used by the
+// standalone kernels in kern_util.hip and by the batched kernel).  The padded
+// buffer holds `lines` rows of m points, row-major.
+//   pre:  u[l][p] = x_l[p] c[p] for p < n, 0 for n <= p < m
+//   filt: u[l][k] = swap(u[l][k] F[k])   (F = FFT_M(conj chirp) / M; the swap
+//         turns the next forward FFT into an inverse)
+//   post: X_l[k] = swap(u[l][k]) c[k]    for k < n
+template <class Fmt>
+__device__ __forceinline__ void chirp_pre_elem(const LineArgs& a, const void* chirp, uint32_t n, uint32_t l,
+                                               uint32_t p, uint64_t e) {
+    using T = typename Fmt::T;
+    cpx<T> v = mk<T>(T(0), T(0));
+    if (p < n) {
+        LineBase b = line_base(a, l);
+        v = load_point<Fmt, LD_COMPLEX>(a, b.in, p, n);
+        v = cmul(v, ld_tw<T>(chirp, p));
+    }
+    Fmt::st(a.out, e, v);
+}
+
+template <class Fmt>
+__device__ __forceinline__ void pointwise_elem(void* buf, const void* filt, uint32_t k, uint64_t e) {
+    using T = typename Fmt::T;
+    cpx<T> v = Fmt::ld(buf, e);
+    Fmt::st(buf, e, swap_ri(cmul(v, ld_tw<T>(filt, k))));
+}
+
+template <class Fmt>
+__device__ __forceinline__ void chirp_post_elem(const LineArgs& a, const void* chirp, uint32_t n, uint32_t m,
+                                                uint32_t l, uint32_t k) {
+    using T = typename Fmt::T;
+    cpx<T> v = swap_ri(Fmt::ld(a.in, (uint64_t)l * m + k));
+    v = cmul(v, ld_tw<T>(chirp, k));
+    LineBase b = line_base(a, l);
+    store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, n, v);
+}
+
 }  // namespace mifft

@@ -8,8 +8,9 @@
 // runtime-schedule LDS kernel, fused Rader or Bluestein).  A tile computes
 // exactly what the same op computes in a standalone plan, so every transform
 // is bit-identical alone and in the mix.  Passes that depend on a previous
-// pass of the same transform (the second half of a four-step split) go into a
-// second phase, launched after the first on the same stream.
+// pass of the same transform (the next axis, the second half of a four-step
+// split, the steps of a multi-pass Bluestein convolution) go into later
+// phases, each launched after the previous one on the same stream.
 //
 // This is the gfx950 counterpart of recording many plans into one shared
 // compute pass with FftRecorder (src/plan.rs:38-95, src/runtime/recorder.rs).
@@ -35,6 +36,23 @@ __device__ __noinline__ void rader_tile_nc(const ConvArgs& c, unsigned char* sme
 template <class Fmt>
 __device__ __noinline__ void bluestein_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
     bluestein_tile<Fmt>(c, smem, g0);
+}
+
+// Element-wise step KIND (0 chirp pre, 1 filter, 2 chirp post) over elements
+// [g0, g0 + kEltTile) of the op's index space (lines x m, or lines x n for the
+// chirp post); the host keeps that space below 2^32.
+template <class Fmt, int KIND>
+__device__ __noinline__ void elt_tile(const ConvArgs& c, uint32_t g0) {
+    const uint32_t per = KIND == 2 ? c.n : c.m;
+    const uint32_t total = c.io.lines * per;
+    const uint32_t end = total - g0 < kEltTile ? total : g0 + kEltTile;
+    for (uint32_t e = g0 + threadIdx.x; e < end; e += 256) {
+        const uint32_t l = e / per;
+        const uint32_t p = e - l * per;
+        if (KIND == 0) chirp_pre_elem<Fmt>(c.io, c.chirp, c.n, l, p, e);
+        else if (KIND == 1) pointwise_elem<Fmt>(c.io.out, c.filt, p, e);
+        else chirp_post_elem<Fmt>(c.io, c.chirp, c.n, c.m, l, p);
+    }
 }
 
 #define MIFFT_BK(N, TPL, LINES, LDLF, STLF, POST, VEC, ...) \
@@ -73,6 +91,12 @@ __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ 
                 reg_tile<RegFft<FmtF64, 256, 16, 16, true, true, LD_COMPLEX, ST_COMPLEX, false, RadixList<16, 16>, 1>>(
                     d.a, smem, tl.g0);
                 break;
+            case BC_CHIRP_PRE: elt_tile<FmtF32, 0>(d.conv, tl.g0); break;
+            case BC_CHIRP_PRE_F64: elt_tile<FmtF64, 0>(d.conv, tl.g0); break;
+            case BC_POINTWISE: elt_tile<FmtF32, 1>(d.conv, tl.g0); break;
+            case BC_POINTWISE_F64: elt_tile<FmtF64, 1>(d.conv, tl.g0); break;
+            case BC_CHIRP_POST: elt_tile<FmtF32, 2>(d.conv, tl.g0); break;
+            case BC_CHIRP_POST_F64: elt_tile<FmtF64, 2>(d.conv, tl.g0); break;
             // register-resident variants with 256 threads (the registry entries of the same shape)
             case BC_REG_C4096: reg_tile<MIFFT_BK(4096, 256, 1, 0, 0, 0, 1, 16, 16, 16)>(d.a, smem, tl.g0); break;
             case BC_REG_C2048: reg_tile<MIFFT_BK(2048, 128, 2, 0, 0, 0, 1, 16, 16, 8)>(d.a, smem, tl.g0); break;

@@ -9,7 +9,7 @@
 // src/runtime/rader_axis.rs:1318-1336).
 #include <cstring>
 
-#include "launch.h"
+#include "conv_kernels.hpp"
 
 namespace mifft {
 
@@ -138,42 +138,24 @@ __global__ void k_round_table(cpx<T>* out, const double2* in, uint64_t count, do
 // Workspace layout: line l of the convolution at ws[l*m + k], plan format.
 template <class Fmt>
 __global__ void k_chirp_pre(LineArgs a, const void* chirp, uint32_t n, uint32_t m) {
-    using T = typename Fmt::T;
     const uint64_t total = (uint64_t)a.lines * m;
     GRID_STRIDE(e, total) {
         const uint32_t l = (uint32_t)(e / m);
-        const uint32_t p = (uint32_t)(e - (uint64_t)l * m);
-        cpx<T> v = mk<T>(T(0), T(0));
-        if (p < n) {
-            LineBase b = line_base(a, l);
-            v = load_point<Fmt, LD_COMPLEX>(a, b.in, p, n);
-            v = cmul(v, ld_tw<T>(chirp, p));
-        }
-        Fmt::st(a.out, e, v);
+        chirp_pre_elem<Fmt>(a, chirp, n, l, (uint32_t)(e - (uint64_t)l * m), e);
     }
 }
 
 template <class Fmt>
 __global__ void k_pointwise(void* buf, const void* filt, uint64_t m, uint64_t total) {
-    using T = typename Fmt::T;
-    GRID_STRIDE(e, total) {
-        const uint64_t k = e % m;
-        cpx<T> v = Fmt::ld(buf, e);
-        Fmt::st(buf, e, swap_ri(cmul(v, ld_tw<T>(filt, (uint32_t)k))));
-    }
+    GRID_STRIDE(e, total) { pointwise_elem<Fmt>(buf, filt, (uint32_t)(e % m), e); }
 }
 
 template <class Fmt>
 __global__ void k_chirp_post(LineArgs a, const void* chirp, uint32_t n, uint32_t m) {
-    using T = typename Fmt::T;
     const uint64_t total = (uint64_t)a.lines * n;
     GRID_STRIDE(e, total) {
         const uint32_t l = (uint32_t)(e / n);
-        const uint32_t k = (uint32_t)(e - (uint64_t)l * n);
-        cpx<T> v = swap_ri(Fmt::ld(a.in, (uint64_t)l * m + k));
-        v = cmul(v, ld_tw<T>(chirp, k));
-        LineBase b = line_base(a, l);
-        store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, n, v);
+        chirp_post_elem<Fmt>(a, chirp, n, m, l, (uint32_t)(e - (uint64_t)l * n));
     }
 }
 

@@ -555,9 +555,6 @@ struct Builder {
                              double scale) {
         if (P.prec == P_DF64)
             fail(MIFFT_ERR_PRECISION_UNSUPPORTED, "df64 Bluestein beyond the on-chip convolution length");
-        if (P.batch_mode)
-            fail(MIFFT_ERR_UNSUPPORTED_LENGTH,
-                 "Bluestein length " + std::to_string(n) + " needs a multi-pass convolution (not batched)");
         const uint64_t region = (uint64_t)ls.lines * m * elem_bytes(P.arith);
         mifft_config cc;
         std::memset(&cc, 0, sizeof(cc));
@@ -571,7 +568,12 @@ struct Builder {
         cc.tuning = P.cfg.tuning;
         cc.tuning.n_force_rader_axes = 0;
         cc.tuning.n_force_bluestein_axes = 0;
-        P.children.push_back(build_plan(cc, MIFFT_C2C));
+        // In batched mode the padded transforms are batched-mode plans too, so
+        // that the batched planner can flatten them into its phases.
+        PlanOptions copt;
+        copt.lds_budget = P.lds_budget;
+        copt.batch_mode = P.batch_mode;
+        P.children.push_back(build_plan(cc, MIFFT_C2C, copt));
         Plan* child = P.children.back().get();
         const void* chirp = chirp_table(n);
         const void* filt = bluestein_filter(n, m);

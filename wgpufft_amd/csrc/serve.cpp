@@ -200,9 +200,17 @@ static void replay(mifft_server& S, const mifft_server_request* reqs, const doub
         res[i] = mifft_server_result{arrival[i], -1.0, -1.0, 0, 0};
     }
     std::vector<char> is_batched(n, 0);
-    std::vector<uint64_t> mem(n, 0);
+    std::vector<uint64_t> mem(n, 0), desc(n, 0);
+    std::map<const Plan*, uint64_t> desc_of;
+    const size_t desc_cap = S.slots.empty() ? 0 : S.slots[0].desc_cap;
     for (int32_t i = 0; i < n; i++) {
-        is_batched[i] = mode == 0 && batched_plan(S, reqs[i]) != nullptr;
+        const Plan* bp = mode == 0 ? batched_plan(S, reqs[i]) : nullptr;
+        if (bp) {
+            auto f = desc_of.find(bp);
+            desc[i] = f != desc_of.end() ? f->second : (desc_of[bp] = batch_desc_bytes(*bp, 1));
+        }
+        // a request whose descriptors alone overflow the tick arena runs through its plan
+        is_batched[i] = bp != nullptr && desc[i] <= desc_cap;
         mem[i] = reqs[i].in_bytes + reqs[i].out_bytes + request_ws(S, reqs[i], is_batched[i]);
         res[i].path = is_batched[i] ? 0 : 1;
     }
@@ -261,7 +269,7 @@ static void replay(mifft_server& S, const mifft_server_request* reqs, const doub
             }
         if (!free_slot) continue;
         std::vector<int> take;
-        uint64_t tick_mem = 0, batch_ws = 0, plan_ws = 0;
+        uint64_t tick_mem = 0, batch_ws = 0, plan_ws = 0, tick_desc = 0;
         while (!pending.empty() && (int)take.size() < S.max_items) {
             const int r = pending.front();
             const uint64_t ws = mem[r] - reqs[r].in_bytes - reqs[r].out_bytes;
@@ -275,7 +283,9 @@ static void replay(mifft_server& S, const mifft_server_request* reqs, const doub
             }
             const uint64_t nb = is_batched[r] ? batch_ws + ws : batch_ws;
             const uint64_t np = is_batched[r] ? plan_ws : std::max(plan_ws, ws);
-            if (inflight + tick_mem + mem[r] > S.budget || nb + np > free_slot->ws_cap) break;
+            const uint64_t nd = is_batched[r] ? tick_desc + desc[r] : tick_desc;
+            if (inflight + tick_mem + mem[r] > S.budget || nb + np > free_slot->ws_cap || nd > desc_cap) break;
+            tick_desc = nd;
             batch_ws = nb;
             plan_ws = np;
             tick_mem += mem[r];
