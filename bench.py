@@ -1,6 +1,7 @@
 """Benchmark of the FFT hot path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2c2d|c1|r2c] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--workload c2c2d|c1|r2c|c4a_f64|c4a_df64|c4b_f64|batched|serve]
+                    [--gpus 1]
 
 Default workload: 2D [4096, 2048] f32 C2C forward then inverse, in place (one
 "step" = one forward+inverse pair).  Algorithmic traffic per pair is
@@ -37,7 +38,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "batched", "serve"])
+    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "c4a_f64", "c4a_df64", "c4b_f64", "batched", "serve"])
     ap.add_argument("--requests", type=int, default=6000, help="serve workload: trace length")
     ap.add_argument("--rate", type=float, default=20000.0, help="serve workload: Poisson arrival rate (req/s)")
     ap.add_argument("--no-serve", action="store_true", help="default workload: skip the serving summary")
@@ -63,6 +64,18 @@ def workload_spec(name):
         return dict(shape=[n], batch=b, kind="c2c_fwd", elems=n * b, bytes_per_step=2 * 8 * n * b,
                     flops_per_step=5 * n * 10 * b,
                     label="1D C2C f32 N=1024 batch=4096 forward")
+    if name in ("c4a_f64", "c4a_df64", "c4b_f64"):
+        # 3D 256^3 (three axis passes) or 1D 2^24 (four-step, two passes); one
+        # complex element is 16 bytes in both f64 and df64 [re_hi,re_lo,im_hi,im_lo]
+        shape = [256, 256, 256] if name.startswith("c4a") else [1 << 24]
+        n_elems = 1 << 24
+        passes = 3 if name.startswith("c4a") else 2
+        prec = "df64" if name.endswith("df64") else "f64"
+        return dict(shape=shape, batch=1, kind="c2c_pair", elems=n_elems, precision=prec, elem_bytes=16,
+                    bytes_per_step=2 * passes * 16 * n_elems * 2,  # 2*P*S per FFT, two FFTs
+                    flops_per_step=2 * 5 * n_elems * 24,
+                    label=f"{'3D C2C [256,256,256]' if passes == 3 else '1D C2C [2^24] four-step'} {prec} "
+                          "forward+inverse in place")
     if name == "r2c":
         n = 2048
         s_r, s_p = 4 * n * n, 8 * (n // 2 + 1) * n
@@ -82,7 +95,7 @@ def main():
     import numpy as np
     import torch
 
-    from wgpufft_amd import FftConfig, FftPlan, fill_normal_f32, fill_uniform_f32
+    from wgpufft_amd import FftConfig, FftPlan, FftPrecision, fill_normal_f32, fill_uniform_f32
 
     if args.workload == "batched":
         print(json.dumps(run_batched(args)), flush=True)
@@ -97,9 +110,11 @@ def main():
 
     shape, batch = spec["shape"], spec["batch"]
     plans = []
+    prec = {"f32": FftPrecision.F32, "f64": FftPrecision.F64,
+            "df64": FftPrecision.Df64}[spec.get("precision", "f32")]
     if spec["kind"] == "c2c_pair":
-        fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch))
-        inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch))
+        fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch).with_precision(prec))
+        inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch).with_precision(prec))
         plans = [fwd, inv]
         nbytes = fwd.required_buffer_size_bytes()
     elif spec["kind"] == "c2c_fwd":
@@ -117,8 +132,20 @@ def main():
     for i, b in enumerate(bufs):
         if spec["kind"] == "r2c_c2r":
             fill_normal_f32(b, 3 + i)
-        else:
+        elif spec.get("precision", "f32") == "f32":
             fill_uniform_f32(b, 2 + i)
+        else:
+            # uniform[-1,1) f32 draws, widened to f64 or placed in the df64 hi words (lo = 0)
+            u = torch.empty(2 * spec["elems"] * 4, dtype=torch.uint8, device=dev)
+            fill_uniform_f32(u, 5 + i)
+            uf = u.view(torch.float32)
+            if spec["precision"] == "f64":
+                b.view(torch.float64).copy_(uf)
+            else:
+                hl = b.view(torch.float32).view(-1, 2)
+                hl[:, 0].copy_(uf)
+                hl[:, 1].zero_()
+            del u, uf
     aux = None
     if spec["kind"] == "r2c_c2r":
         aux = [torch.empty(plans[0].required_output_buffer_size_bytes(), dtype=torch.uint8, device=dev)
@@ -173,8 +200,15 @@ def main():
         ref0 = bufs[0].clone()
         step(0)
         torch.cuda.synchronize()
-        a = ref0.view(torch.float32).double()
-        bcur = bufs[0].view(torch.float32).double()
+        def as_f64(t):
+            if spec.get("precision", "f32") == "f64":
+                return t.view(torch.float64)
+            if spec.get("precision", "f32") == "df64":
+                hl = t.view(torch.float32).view(-1, 2).double()
+                return hl[:, 0] + hl[:, 1]
+            return t.view(torch.float32).double()
+        a = as_f64(ref0)
+        bcur = as_f64(bufs[0])
         rt_err = float(torch.linalg.norm(bcur - a) / torch.linalg.norm(a))
         bufs[0].copy_(ref0)
         del ref0
@@ -210,7 +244,7 @@ def main():
         n = shape[0]
         pass_bytes = [4 * n * n + 8 * (n // 2 + 1) * n, 2 * 8 * (n // 2 + 1) * n]
     else:
-        pass_bytes = [2 * 8 * spec["elems"]] * len(kernels)
+        pass_bytes = [2 * spec.get("elem_bytes", 8) * spec["elems"]] * len(kernels)
     for kinfo, pb in zip(kernels, pass_bytes):
         kinfo["alg_bytes"] = pb
         kinfo["gbs"] = pb / (kinfo["us"] * 1e-6) / 1e9
@@ -225,8 +259,25 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(spec)
 
+    # second half of the metric: the serving workload's latency and throughput
+    serve = None
+    if args.workload == "c2c2d" and not args.no_serve:
+        del bufs, aux
+        torch.cuda.empty_cache()
+        sv = run_serve(args)
+        cbm, sqm = sv["modes"]["continuous_batching"], sv["modes"]["sequential_per_plan"]
+        serve = {k: sv[k] for k in ("p50_ms", "p99_ms", "throughput_rps", "saturation_throughput_rps",
+                                    "saturation_speedup_vs_sequential", "arrival_rate_rps", "requests")}
+        serve["batched_fraction"] = cbm["poisson"]["batched_fraction"]
+        serve["sequential_p50_ms"] = round(sqm["poisson"]["p50_ms"], 4)
+        serve["sequential_p99_ms"] = round(sqm["poisson"]["p99_ms"], 4)
+        serve["sequential_saturation_throughput_rps"] = round(sqm["saturation_throughput_rps"], 1)
+
+    metric = "HBM GB/s (% of 8 TB/s) 4096x2048 f32 C2C; mixed-size serve p50/p99 latency"
+    if args.workload != "c2c2d":
+        metric = "HBM GB/s (% of 8 TB/s) " + spec["label"]
     line = {
-        "metric": "HBM GB/s (% of 8 TB/s) 4096x2048 f32 C2C; mixed-size serve p50/p99 latency",
+        "metric": metric,
         "value": round(value, 1),
         "unit": "GB/s",
         "pct_of_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
@@ -237,7 +288,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": spec.get("precision", "f32"),
         "data": "synthetic (SplitMix64 uniform[-1,1) generated on device)",
         "config": {"workload": spec["label"], "shape": shape, "batch": batch,
                    "buffers": nbuf, "buffer_mib": nbytes / 2**20,
@@ -249,6 +300,7 @@ def main():
         "kernels": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 
@@ -429,7 +481,7 @@ def cpu_baseline(spec):
         t0 = time.perf_counter()
         y = oracle.c2c_nd(x, shape, batch=batch, nthreads=ncores)
         oracle.c2c_nd(y, shape, batch=batch, inverse=True, nthreads=ncores)
-        sample = "one forward+inverse pair of the same [4096,2048] workload in FP64"
+        sample = f"one forward+inverse pair of the same {shape} workload in FP64"
     elif spec["kind"] == "c2c_fwd":
         x = oracle.uniform_f32(1, 2 * spec["elems"]).view(np.complex64).astype(np.complex128)
         t0 = time.perf_counter()

@@ -230,3 +230,30 @@ def test_df64(torch_cuda, shape, batch):
     e = oracle.rel_l2(y, ref)
     print(f"df64 {shape} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
     assert e <= 1e-11
+
+
+# Full-size high-precision configurations: 3D 256^3 (three axis passes) and the
+# 1D 2^24 four-step, against the multithreaded FP64 oracle.
+@pytest.mark.parametrize("shape,prec", [([256, 256, 256], "f64"), ([256, 256, 256], "df64"), ([1 << 24], "f64")])
+def test_high_precision_full_size(torch_cuda, shape, prec):
+    import torch
+    cnt = int(np.prod(shape))
+    x = _f64_input(41, cnt)
+    if prec == "f64":
+        plan = FftPlan.c2c(FftConfig.new_nd(shape).with_precision(FftPrecision.F64))
+        d = to_dev(x)
+    else:
+        plan = FftPlan.c2c(FftConfig.new_nd(shape).with_precision(FftPrecision.Df64))
+        hilo = oracle.df64_split(x.view(np.float64))
+        x = oracle.df64_join(hilo).view(np.complex128)
+        d = to_dev(hilo)
+    plan.execute_in_place(d)
+    torch.cuda.synchronize()
+    if prec == "f64":
+        y = from_dev(d, np.complex128)
+    else:
+        y = oracle.df64_join(from_dev(d, np.float32)).view(np.complex128)
+    ref = oracle.c2c_nd(x, shape, nthreads=16)
+    e = oracle.rel_l2(y, ref)
+    print(f"{prec} {shape} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= (1e-13 if prec == "f64" else 1e-11)

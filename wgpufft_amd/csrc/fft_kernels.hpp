@@ -290,7 +290,36 @@ struct RegFft {
         }
     }
 
-    template <int s>
+    // Issues the global loads of lines [g0, g0 + LINES) into w, in the layout
+    // of stage 0.  Branch-free: a line group past the end reads the last valid
+    // group and a butterfly past N/R reads point N/R-1; those values are never
+    // stored.  A conditional load per point would make the compiler wait for
+    // each load before issuing the next.
+    static __device__ __forceinline__ void load_lines(const LineArgs& a, uint32_t g0, cpx<T> (&w)[VEC][EMAX]) {
+        constexpr int R = RL::at(0);
+        constexpr int NR = N / R;
+        constexpr int NB = ceil_div(NR, TPL);
+        const Map mp = map_of<0>();
+        const uint32_t g = g0 + mp.lg * VEC;
+        const uint32_t gl = g < a.lines ? g : (a.lines - VEC);
+        LineBase b = line_base(a, gl);
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            uint32_t j = mp.t + q * TPL;
+            if constexpr (NR % TPL != 0) j = j < (uint32_t)NR ? j : (uint32_t)(NR - 1);
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                if constexpr (VEC == 1)
+                    w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
+                else
+                    load_pair<Fmt>(a, b.in, j + m * NR, w[0][q * R + m], w[1][q * R + m]);
+            }
+        }
+    }
+
+    // Stage s of lines [g0, g0 + LINES) and every later stage; the last one
+    // stores.  With LOAD false, stage 0 takes w as already loaded.
+    template <int s, bool LOAD = true>
     static __device__ __forceinline__ void stage(const LineArgs& a, cpx<T>* lds, uint32_t g0,
                                                  cpx<T> (&w)[VEC][EMAX], cpx<T> (&twr)[TWREG]) {
         if constexpr (s < NS) {
@@ -302,25 +331,8 @@ struct RegFft {
             const Map mp = map_of<s>();
             const uint32_t g = g0 + mp.lg * VEC;
             const bool valid = g < a.lines;
-            if constexpr (FIRST) {
-                // Branch-free loads: a line group past the end reads the last valid
-                // group and a butterfly past N/R reads point N/R-1; those values are
-                // never stored.  A conditional load per point would make the
-                // compiler wait for each load before issuing the next.
-                const uint32_t gl = valid ? g : (a.lines - VEC);
-                LineBase b = line_base(a, gl);
-#pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    uint32_t j = mp.t + q * TPL;
-                    if constexpr (NR % TPL != 0) j = j < (uint32_t)NR ? j : (uint32_t)(NR - 1);
-#pragma unroll
-                    for (int m = 0; m < R; m++) {
-                        if constexpr (VEC == 1)
-                            w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
-                        else
-                            load_pair<Fmt>(a, b.in, j + m * NR, w[0][q * R + m], w[1][q * R + m]);
-                    }
-                }
+            if constexpr (FIRST && LOAD) {
+                load_lines(a, g0, w);
                 if constexpr (PF) prefetch<1>(a, twr);
             }
             // twiddle + butterfly (one twiddle load serves the VEC lines)
@@ -379,12 +391,58 @@ struct RegFft {
         const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
         tile(a, lds, bid * LINES);
     }
+
+    // Persistent form: block b transforms tiles b, b + G, b + 2G, ... (G the
+    // grid size) and issues the loads of its next tile before transforming the
+    // current one, so that every CU keeps reads in flight while it computes
+    // and stores (with one tile per block, the whole chip would load, then
+    // compute, then store in lockstep).  The exchange barriers wait for LDS
+    // only, so the loads stay in flight across them.
+    static __device__ __forceinline__ void run_pipelined(const LineArgs& a, cpx<T>* lds) {
+        const uint32_t G = gridDim.x;
+        const uint32_t ntiles = (a.lines + LINES - 1) / LINES;
+        const uint32_t t0 = xcd_remap(blockIdx.x, G);
+        if (t0 >= ntiles) return;
+        const uint32_t nt = (ntiles - t0 + G - 1) / G;  // tiles of this block
+        cpx<T> wa[VEC][EMAX], wb[VEC][EMAX];
+        // Stage twiddles depend on the thread's position only: with PF they
+        // are loaded once, so that no global load is issued (and waited for)
+        // between the next tile's loads and the current tile's stages; the
+        // in-order load counter would otherwise drain the next tile's loads.
+        cpx<T> twr[TWREG];
+        if constexpr (PF) prefetch<1>(a, twr);
+        // The loads inside the loop are unconditional: a load under a branch
+        // makes the compiler wait for it at the join.
+        load_lines(a, t0 * LINES, wa);
+        uint32_t i = 0;
+        for (; i + 2 < nt; i += 2) {
+            load_lines(a, (t0 + (i + 1) * G) * LINES, wb);
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
+            load_lines(a, (t0 + (i + 2) * G) * LINES, wa);
+            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr);
+        }
+        if (i + 1 < nt) {
+            load_lines(a, (t0 + (i + 1) * G) * LINES, wb);
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
+            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr);
+        } else {
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
+        }
+    }
 };
 
 template <class K>
 __global__ void __launch_bounds__(K::NT) reg_fft_kernel(LineArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     K::run(a, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
+}
+
+// Persistent pipelined launch of K (grid below the tile count; MINB blocks
+// per CU).
+template <class K, int MINB>
+__global__ void __launch_bounds__(K::NT, MINB) reg_fft_pipe_kernel(LineArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    K::run_pipelined(a, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
 }
 
 // ------------------------------------------------ runtime-schedule LDS kernel
