@@ -92,3 +92,44 @@ def test_real_precision_unsupported(torch_cuda):
     with pytest.raises(FftError) as e:
         FftPlan.r2c(FftConfig.new(16).with_precision(FftPrecision.F64))
     assert e.value.kind == "PrecisionUnsupported"
+
+
+# Axis 0 that the fused real kernels cannot take: primes (Rader, Bluestein),
+# composites with a large prime factor, and smooth lengths beyond one workgroup
+# (four-step) run as widened complex lines (R2C) or Hermitian-extended lines
+# (C2R) in the workspace.
+GENERAL_SHAPES = [([17], 3), ([1009], 2), ([10007], 1), ([2 * 1709], 1), ([16384], 2), ([65536], 1),
+                  ([17, 12], 1), ([1009, 6], 2), ([16384, 4], 1), ([23, 5, 3], 1)]
+
+
+@pytest.mark.parametrize("shape,batch", GENERAL_SHAPES)
+def test_r2c_general_axis0(torch_cuda, shape, batch):
+    cnt = int(np.prod(shape)) * batch
+    x = oracle.normal_f32(13, cnt)
+    plan = FftPlan.r2c(FftConfig.new_nd(shape).with_batch(batch))
+    d = to_dev(x)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    _sync()
+    y = from_dev(out, np.complex64)
+    ref = oracle.r2c_nd(x.astype(np.float64), shape, batch, nthreads=16)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"r2c {shape} b={batch} rel_l2={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 <= 1e-6 and em <= 5e-6
+
+
+@pytest.mark.parametrize("shape,batch", GENERAL_SHAPES)
+def test_c2r_general_axis0(torch_cuda, shape, batch):
+    pk = int(np.prod(oracle.packed_shape(shape))) * batch
+    p = oracle.uniform_f32(14, 2 * pk).view(np.complex64)
+    plan = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
+    d = to_dev(p)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    _sync()
+    y = from_dev(out, np.float32)
+    ref = oracle.c2r_nd(p.astype(np.complex128), shape, batch, nthreads=16)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"c2r {shape} b={batch} rel_l2={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 <= 1e-6 and em <= 5e-6
+    assert np.array_equal(from_dev(d, np.complex64).view(np.uint32), p.view(np.uint32))

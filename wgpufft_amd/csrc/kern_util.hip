@@ -165,6 +165,46 @@ __global__ void k_scale_copy(const void* in, void* out, uint64_t count, double s
     GRID_STRIDE(e, count) { Fmt::st(out, e, scale(Fmt::ld(in, e), (T)s)); }
 }
 
+// ------------------------------------------------ general real transforms
+// Real transforms whose axis 0 needs Rader, Bluestein or a four-step split run
+// that axis as a complex transform of full lines in the workspace: widen the
+// real input (im = 0) before it and keep bins k <= N0/2 after it (R2C), or
+// Hermitian-extend the packed lines before it and keep the real part after it
+// (C2R).  The extension follows the packed convention of the fused real load:
+// the imaginary part of bins 0 and N0/2 is dropped and bin N0-k is conj(bin k)
+// (src/math/mod.rs:470-502).
+__global__ void k_real_widen(const float* __restrict__ in, float2* __restrict__ out, uint64_t count) {
+    GRID_STRIDE(i, count) { out[i] = make_float2(in[i], 0.0f); }
+}
+
+__global__ void k_pack(const float2* __restrict__ full, float2* __restrict__ packed, uint32_t n0, uint32_t p0,
+                       uint64_t lines, float sc) {
+    const uint64_t total = lines * p0;
+    GRID_STRIDE(e, total) {
+        const uint64_t l = e / p0;
+        const uint32_t k = (uint32_t)(e - l * p0);
+        const float2 v = full[l * n0 + k];
+        packed[e] = make_float2(v.x * sc, v.y * sc);
+    }
+}
+
+__global__ void k_herm_extend(const float2* __restrict__ packed, float2* __restrict__ full, uint32_t n0,
+                              uint32_t p0, uint64_t lines) {
+    const uint64_t total = lines * n0;
+    GRID_STRIDE(e, total) {
+        const uint64_t l = e / n0;
+        const uint32_t p = (uint32_t)(e - l * n0);
+        const bool up = p > (n0 >> 1);
+        float2 v = packed[l * p0 + (up ? n0 - p : p)];
+        v.y = (p == 0 || 2 * p == n0) ? 0.0f : (up ? -v.y : v.y);
+        full[e] = v;
+    }
+}
+
+__global__ void k_real_part(const float2* __restrict__ full, float* __restrict__ out, uint64_t count, float sc) {
+    GRID_STRIDE(i, count) { out[i] = full[i].x * sc; }
+}
+
 // ------------------------------------------------------------ synthetic data
 __device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
     // State after k+1 steps of SplitMix64 (Vigna), mixed.
@@ -299,6 +339,28 @@ hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uin
     if (prec == P_F32) k_chirp_post<FmtF32><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
     else if (prec == P_F64) k_chirp_post<FmtF64><<<grid_for(total), 256, 0, s>>>(a, chirp, n, m);
     else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_real_widen(const void* in, void* out, uint64_t count, hipStream_t s) {
+    k_real_widen<<<grid_for(count), 256, 0, s>>>((const float*)in, (float2*)out, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const void* full, void* packed, uint32_t n0, uint64_t lines, double sc, hipStream_t s) {
+    const uint32_t p0 = n0 / 2 + 1;
+    k_pack<<<grid_for(lines * p0), 256, 0, s>>>((const float2*)full, (float2*)packed, n0, p0, lines, (float)sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_herm_extend(const void* packed, void* full, uint32_t n0, uint64_t lines, hipStream_t s) {
+    k_herm_extend<<<grid_for(lines * n0), 256, 0, s>>>((const float2*)packed, (float2*)full, n0, n0 / 2 + 1,
+                                                       lines);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_part(const void* full, void* out, uint64_t count, double sc, hipStream_t s) {
+    k_real_part<<<grid_for(count), 256, 0, s>>>((const float2*)full, (float*)out, count, (float)sc);
     return hipGetLastError();
 }
 

@@ -77,6 +77,13 @@ hipError_t launch_pointwise(int prec, void* buf, const void* filt, uint64_t m, u
                             hipStream_t s);
 hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uint32_t n,
                              uint32_t m, hipStream_t s);
+// General real transforms (f32): real -> complex widening, packing of bins
+// k <= n0/2 with a scale, Hermitian extension of packed lines, real part with a
+// scale.  Lines are contiguous.
+hipError_t launch_real_widen(const void* in, void* out, uint64_t count, hipStream_t s);
+hipError_t launch_pack(const void* full, void* packed, uint32_t n0, uint64_t lines, double scale, hipStream_t s);
+hipError_t launch_herm_extend(const void* packed, void* full, uint32_t n0, uint64_t lines, hipStream_t s);
+hipError_t launch_real_part(const void* full, void* out, uint64_t count, double scale, hipStream_t s);
 hipError_t launch_scale_copy(int prec, const void* in, void* out, uint64_t count, double scale,
                              hipStream_t s);
 

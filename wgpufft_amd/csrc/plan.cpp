@@ -837,17 +837,41 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 P->in_bytes = packed * 8;
                 P->out_bytes = total * 4;
             }
-            if (P->axes[0].kind != MIFFT_AXIS_MIXED)
-                fail(MIFFT_ERR_UNSUPPORTED_LENGTH,
-                     "real transforms need a {2,3,5,7,11,13}-smooth axis 0 (length " + std::to_string(N0) + ")");
+            // Axis 0 runs fused with the real load / packed store when it is smooth
+            // and fits one workgroup; otherwise (Rader, Bluestein, four-step
+            // lengths) as a complex transform of full lines in the workspace,
+            // between an element-wise widening / packing (R2C) or Hermitian
+            // extension / real part (C2R).
+            const bool smooth0 = P->axes[0].kind == MIFFT_AXIS_MIXED;
             const LineSet l0{(uint32_t)outer, 1, 1};
             const Layout real_l{0, 0, (int64_t)N0, 1};
             const Layout pack_l{0, 0, (int64_t)P0, 1};
+            const uint64_t full_bytes = total * 8;
+            auto elem_op = [&](OpKind k, int src, uint64_t src_off, int dst, uint64_t dst_off, uint64_t count,
+                               double sc, const char* what) {
+                Op op;
+                op.kind = k;
+                op.src = src;
+                op.src_off = src_off;
+                op.dst = dst;
+                op.dst_off = dst_off;
+                op.prec = P_F32;
+                op.n = (uint32_t)N0;
+                op.count = count;
+                op.scale = sc;
+                op.desc = std::string(what) + " N0=" + std::to_string(N0) + " lines=" + std::to_string(outer);
+                P->ops.push_back(op);
+            };
             uint64_t ws = 0;
             if (kind == MIFFT_R2C) {
-                if (!B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false,
-                               cfg.rank == 1 ? scale : 1.0, LD_REAL, ST_PACKED))
-                    fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "axis 0 of a real transform must fit on chip");
+                const double s0 = cfg.rank == 1 ? scale : 1.0;
+                if (!(smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false, s0, LD_REAL,
+                                           ST_PACKED))) {
+                    elem_op(OP_REAL_WIDEN, B_IN, 0, B_WS, 0, total, 1.0, "real widen");
+                    ws = full_bytes + B.axis_ops(N0, P->axes[0].kind, 1, outer, B_WS, 0, B_WS, 0, full_bytes,
+                                                 false, 1.0);
+                    elem_op(OP_PACK, B_WS, 0, B_OUT, 0, outer, s0, "pack");
+                }
                 uint64_t S = P0;
                 for (int axis = 1; axis < cfg.rank; axis++) {
                     const uint64_t n = cfg.shape[axis];
@@ -873,9 +897,16 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                     }
                     ws += pk_bytes;
                 }
-                if (!B.line_op(N0, l0, pack_l, real_l, src, src_off, B_OUT, 0, true, true, scale,
-                               LD_HERMITIAN, ST_REAL))
-                    fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "axis 0 of a real transform must fit on chip");
+                if (!(smooth0 && B.line_op(N0, l0, pack_l, real_l, src, src_off, B_OUT, 0, true, true, scale,
+                                           LD_HERMITIAN, ST_REAL))) {
+                    // full complex lines after the packed copy of the outer passes
+                    const uint64_t X = cfg.rank > 1 ? pk_bytes : 0;
+                    elem_op(OP_HERM_EXTEND, src, src_off, B_WS, X, outer, 1.0, "hermitian extend");
+                    ws = std::max(ws, X + full_bytes +
+                                          B.axis_ops(N0, P->axes[0].kind, 1, outer, B_WS, X, B_WS, X,
+                                                     X + full_bytes, true, 1.0));
+                    elem_op(OP_REAL_PART, B_WS, X, B_OUT, 0, total, scale, "real part");
+                }
             }
             P->ws_bytes = ws;
         }
@@ -949,6 +980,21 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             hip_check(launch_scale_copy(op.prec, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off),
                                         op.count, op.scale, s),
                       "scale copy");
+            break;
+        case OP_REAL_WIDEN:
+            hip_check(launch_real_widen(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.count, s), "real widen");
+            break;
+        case OP_PACK:
+            hip_check(launch_pack(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.n, op.count, op.scale, s),
+                      "pack");
+            break;
+        case OP_HERM_EXTEND:
+            hip_check(launch_herm_extend(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.n, op.count, s),
+                      "hermitian extend");
+            break;
+        case OP_REAL_PART:
+            hip_check(launch_real_part(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.count, op.scale, s),
+                      "real part");
             break;
     }
 }

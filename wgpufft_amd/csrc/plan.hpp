@@ -32,6 +32,10 @@ enum OpKind {
     OP_CHIRP_POST,  // multi-pass Bluestein: chirp multiply + crop
     OP_SUBPLAN,     // a child plan run in place on a workspace region
     OP_SCALE_COPY,  // direct route of a one-point transform
+    OP_REAL_WIDEN,  // general R2C: real input -> complex lines (im = 0)
+    OP_PACK,        // general R2C: complex lines -> packed bins k <= N0/2
+    OP_HERM_EXTEND, // general C2R: packed lines -> Hermitian-extended complex lines
+    OP_REAL_PART,   // general C2R: complex lines -> real part
 };
 
 struct Plan;
