@@ -249,7 +249,7 @@ def main():
         kinfo["alg_bytes"] = pb
         kinfo["gbs"] = pb / (kinfo["us"] * 1e-6) / 1e9
     dom = max(kernels, key=lambda k: k["us"])
-    traffic, traffic_src = profiled_traffic(dom["desc"])
+    traffic, traffic_src = profiled_traffic(dom["desc"], spec.get("precision", "f32"))
     roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom["desc"], "kernel_us": round(dom["us"], 2),
@@ -449,10 +449,11 @@ def run_serve(args):
     }
 
 
-def profiled_traffic(desc):
+def profiled_traffic(desc, prec="f32"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on length,
-    radices and lines per thread; (None, None) when no summary covers it."""
+    summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on precision,
+    length, radices and lines per thread; (None, None) when no summary covers
+    it."""
     import glob
     import re
     m = re.search(r"N=(\d+) (\S+) .*vec=(\d)", desc)
@@ -463,7 +464,8 @@ def profiled_traffic(desc):
         for line in open(path):
             if not line.startswith("traffic_bytes_per_launch:"):
                 continue
-            if f"N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line:
+            if (f" {prec} N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line
+                    and " post" not in line):
                 return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
     return None, None
 

@@ -269,6 +269,33 @@ mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* re
                                  int32_t n, int32_t mode, mifft_server_result* results);
 void mifft_server_destroy(mifft_server* s);
 
+/* ------------------------------------------------------------ autotuner */
+/* No single counterpart in the reference: wgpuFFT compiles in per-shape
+ * constants measured offline (src/runtime/axis_plan.rs:2092-2110,
+ * register_fft.rs:64-118).  Here the planner consults a process-wide tuning
+ * table of measured choices (a registered kernel variant: radix split,
+ * threads per line, lines per workgroup; or a runtime-schedule LDS shape),
+ * keyed by the shape of each line pass; without an entry it uses its
+ * defaults.
+ *
+ * mifft_autotune builds the plan of (config, kind), times every candidate of
+ * each of its line passes on scratch buffers of the plan's sizes (`reps`
+ * launches each, HIP events on `stream`) and records the fastest.  Plans
+ * created afterwards use the table.  *passes_tuned (optional) receives the
+ * number of passes that had more than one candidate; report (optional,
+ * report_cap bytes) receives one text line per pass with every candidate's
+ * time. */
+mifft_status mifft_autotune(const mifft_config* config, mifft_kind kind, int32_t reps, hipStream_t stream,
+                            int32_t* passes_tuned, char* report, uint64_t report_cap);
+/* Text form "key = choice", one entry per line ('#' starts a comment). */
+mifft_status mifft_tuning_table_save(const char* path);
+/* Adds the entries of a saved table; a malformed line or an unknown variant
+ * fails with MIFFT_ERR_INVALID_TUNING and adds nothing. */
+mifft_status mifft_tuning_table_load(const char* path);
+mifft_status mifft_tuning_table_parse(const char* text, int32_t* entries);
+void mifft_tuning_table_clear(void);
+int32_t mifft_tuning_table_size(void);
+
 /* --------------------------------------------------------------- errors */
 const char* mifft_status_string(mifft_status s);
 /* Thread-local detail text of the last failing call on this thread. */

@@ -1,0 +1,104 @@
+"""GPU tests of the autotuner: every registered alternative of a line shape
+(radix split, threads per line, lines per workgroup) and the runtime-schedule
+LDS shapes, forced through the tuning table, must transform correctly
+(FP64 oracle, relative L2 <= 1e-6); mifft_autotune records a choice per pass
+and plans built afterwards use it."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import c2c_expected, empty_dev, from_dev, seeded_complex, to_dev
+import wgpufft_amd as W
+from wgpufft_amd import FftConfig, FftPlan
+
+pytestmark = pytest.mark.gpu
+
+
+def key(n, lf, pair, lines):
+    lb = max(0, (lines - 1).bit_length())
+    return f"f32|n{n}|lf{lf}|m0/0|p0|pair{int(pair)}|lines2^{lb}"
+
+
+def reg(n, tpl, lines, lf, vec, rad):
+    return f"reg:f32:n{n}:t{tpl}:l{lines}:lf{lf}:m0/0:p0:v{vec}:{rad}"
+
+
+# (shape, batch, key of the pass forced, choices, text expected in describe())
+CASES = [
+    ([4096], 8, key(4096, "00", False, 8), [
+        (reg(4096, 256, 1, "00", 1, "16x16x16"), "tpl=256"),
+        (reg(4096, 512, 1, "00", 1, "8x8x8x8"), "8x8x8x8"),
+        (reg(4096, 256, 2, "00", 1, "16x16x16"), "lpw=2"),
+        (reg(4096, 128, 1, "00", 1, "16x16x16"), "tpl=128"),
+        ("lds:lpw=1", "lds N=4096"), ("lds:lpw=2", "lpw=2")]),
+    ([2048], 12, key(2048, "00", False, 12), [
+        (reg(2048, 256, 1, "00", 1, "8x8x8x4"), "8x8x8x4"),
+        (reg(2048, 128, 4, "00", 1, "16x16x8"), "lpw=4")]),
+    ([1024], 20, key(1024, "00", False, 20), [
+        (reg(1024, 128, 2, "00", 1, "8x8x4x4"), "8x8x4x4"),
+        (reg(1024, 64, 8, "00", 1, "16x16x4"), "lpw=8"),
+        ("lds:lpw=4", "lds N=1024")]),
+    ([8, 4096], 1, key(4096, "11", True, 8), [
+        (reg(4096, 256, 4, "11", 2, "16x16x16"), "vec=2 lpw=4")]),
+    ([16, 2048], 1, key(2048, "11", True, 16), [
+        (reg(2048, 128, 8, "11", 2, "16x16x8"), "vec=2 lpw=8"),
+        (reg(2048, 256, 8, "11", 2, "8x8x8x4"), "8x8x8x4"),
+        ("lds:lpw=16", "lds N=2048")]),
+    ([32, 1024], 1, key(1024, "11", True, 32), [
+        (reg(1024, 64, 16, "11", 2, "16x16x4"), "vec=2 lpw=16")]),
+]
+
+
+@pytest.fixture(autouse=True)
+def _clean():
+    W.clear_tuning()
+    yield
+    W.clear_tuning()
+
+
+def _run(cfg, x):
+    import torch
+    plan = FftPlan.c2c(cfg)
+    d = to_dev(x)
+    out = empty_dev(x.nbytes)
+    plan.execute(d, out)
+    torch.cuda.synchronize()
+    return plan, from_dev(out, np.complex64)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_forced_alternatives_match_oracle(torch_cuda, case):
+    shape, batch, k, choices = CASES[case]
+    x = seeded_complex(70 + case, int(np.prod(shape)) * batch)
+    ref = c2c_expected(x, shape, batch=batch)
+    for choice, marker in choices:
+        W.clear_tuning()
+        W.parse_tuning(f"{k} = {choice}\n")
+        for inverse in (False, True):
+            cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+            plan, y = _run(cfg, x)
+            assert marker in plan.describe(), (choice, plan.describe())
+            want = c2c_expected(x, shape, batch=batch, inverse=True) if inverse else ref
+            e = oracle.rel_l2(y, want)
+            print(f"{shape} b={batch} {choice} inverse={inverse} rel_l2={e:.3e}", flush=True)
+            assert e <= 1e-6, (choice, inverse, e)
+
+
+def test_autotune_records_and_plans_follow(torch_cuda, tmp_path):
+    shape = [2048, 256]
+    n, report = W.autotune(FftConfig.new_nd(shape), reps=5)
+    print(report, flush=True)
+    assert n >= 1 and W.tuning_size() >= 1
+    assert report.count("->") == n
+    x = seeded_complex(91, 2048 * 256)
+    plan, y = _run(FftConfig.new_nd(shape), x)
+    e = oracle.rel_l2(y, c2c_expected(x, shape))
+    print(f"tuned {shape}: {plan.describe()} rel_l2={e:.3e}", flush=True)
+    assert e <= 1e-6
+    # the table survives a save / clear / load round trip and gives the same plan
+    p = tmp_path / "t.txt"
+    W.save_tuning(str(p))
+    desc = plan.describe()
+    W.clear_tuning()
+    W.load_tuning(str(p))
+    assert FftPlan.c2c(FftConfig.new_nd(shape)).describe() == desc

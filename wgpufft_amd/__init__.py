@@ -13,6 +13,7 @@ creating a plan raises ``LibraryMissing``.
 from __future__ import annotations
 
 import ctypes
+import os
 import enum
 from typing import Iterable, Optional, Sequence
 
@@ -508,6 +509,43 @@ def create_r2c_plan(config: FftConfig) -> FftPlan:
 
 def create_c2r_plan(config: FftConfig) -> FftPlan:
     return FftPlan.c2r(config)
+
+
+# ------------------------------------------------------------------ autotuner
+def autotune(config: FftConfig, kind: FftTransformKind = FftTransformKind.C2c, reps: int = 20,
+             stream=None) -> tuple:
+    """Times every candidate kernel of each line pass of the plan of (config,
+    kind) and records the fastest in the process-wide tuning table, which
+    plans created afterwards use.  Returns (passes tuned, report text)."""
+    lib = _lib.load()
+    c = config._to_c()
+    n = ctypes.c_int32()
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(lib.mifft_autotune(ctypes.byref(c), int(kind), int(reps), _stream(stream), ctypes.byref(n), buf,
+                              len(buf)))
+    return n.value, buf.value.decode()
+
+
+def save_tuning(path: str) -> None:
+    _check(_lib.load().mifft_tuning_table_save(os.fsencode(path)))
+
+
+def load_tuning(path: str) -> None:
+    _check(_lib.load().mifft_tuning_table_load(os.fsencode(path)))
+
+
+def parse_tuning(text: str) -> int:
+    n = ctypes.c_int32()
+    _check(_lib.load().mifft_tuning_table_parse(text.encode(), ctypes.byref(n)))
+    return n.value
+
+
+def clear_tuning() -> None:
+    _lib.load().mifft_tuning_table_clear()
+
+
+def tuning_size() -> int:
+    return int(_lib.load().mifft_tuning_table_size())
 
 
 def fill_uniform_f32(tensor, seed: int, stream=None) -> None:

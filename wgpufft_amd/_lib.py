@@ -134,6 +134,12 @@ SIGNATURES = [
     ("mifft_status_string", cp, [i32]),
     ("mifft_last_error_detail", cp, []),
     ("mifft_version", cp, []),
+    ("mifft_autotune", i32, [ctypes.POINTER(Config), i32, i32, vp, ctypes.POINTER(i32), ctypes.c_char_p, u64]),
+    ("mifft_tuning_table_save", i32, [ctypes.c_char_p]),
+    ("mifft_tuning_table_load", i32, [ctypes.c_char_p]),
+    ("mifft_tuning_table_parse", i32, [ctypes.c_char_p, ctypes.POINTER(i32)]),
+    ("mifft_tuning_table_clear", None, []),
+    ("mifft_tuning_table_size", i32, []),
     ("mifft_fill_uniform_f32", i32, [vp, u64, u64, vp]),
     ("mifft_fill_normal_f32", i32, [vp, u64, u64, vp]),
 ]

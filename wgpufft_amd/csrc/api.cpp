@@ -4,6 +4,8 @@
 #include <cstring>
 #include <string>
 
+#include <cstdio>
+#include <algorithm>
 #include "plan.hpp"
 
 using namespace mifft;
@@ -335,6 +337,60 @@ const char* mifft_status_string(mifft_status s) {
 const char* mifft_last_error_detail(void) { return g_detail.c_str(); }
 
 const char* mifft_version(void) { return "mifft 0.1.0 (gfx950)"; }
+
+mifft_status mifft_autotune(const mifft_config* config, mifft_kind kind, int32_t reps, hipStream_t stream,
+                            int32_t* passes_tuned, char* report, uint64_t report_cap) {
+    if (!config) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null config");
+    return guard([&] {
+        std::string text;
+        const int n = autotune(*config, kind, reps, stream, &text);
+        if (passes_tuned) *passes_tuned = n;
+        if (report && report_cap) {
+            const size_t k = std::min<size_t>(text.size(), (size_t)report_cap - 1);
+            std::memcpy(report, text.data(), k);
+            report[k] = 0;
+        }
+    });
+}
+
+mifft_status mifft_tuning_table_save(const char* path) {
+    if (!path) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null path");
+    return guard([&] {
+        std::string text = "# mifft tuning table: key = choice\n";
+        for (auto& kv : tuning_entries()) text += kv.first + " = " + kv.second + "\n";
+        FILE* f = std::fopen(path, "w");
+        if (!f) fail(MIFFT_ERR_INVALID_TUNING, std::string("cannot write ") + path);
+        const size_t w = std::fwrite(text.data(), 1, text.size(), f);
+        const int c = std::fclose(f);
+        if (w != text.size() || c != 0) fail(MIFFT_ERR_INVALID_TUNING, std::string("cannot write ") + path);
+    });
+}
+
+mifft_status mifft_tuning_table_load(const char* path) {
+    if (!path) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null path");
+    return guard([&] {
+        FILE* f = std::fopen(path, "r");
+        if (!f) fail(MIFFT_ERR_INVALID_TUNING, std::string("cannot read ") + path);
+        std::string text;
+        char buf[4096];
+        size_t r;
+        while ((r = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, r);
+        std::fclose(f);
+        tuning_parse(text);
+    });
+}
+
+mifft_status mifft_tuning_table_parse(const char* text, int32_t* entries) {
+    if (!text) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null text");
+    return guard([&] {
+        const int n = tuning_parse(text);
+        if (entries) *entries = n;
+    });
+}
+
+void mifft_tuning_table_clear(void) { tuning_clear(); }
+
+int32_t mifft_tuning_table_size(void) { return (int32_t)tuning_entries().size(); }
 
 mifft_status mifft_fill_uniform_f32(float* dst, uint64_t count, uint64_t seed, hipStream_t stream) {
     if (!dst) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null buffer");

@@ -8,6 +8,12 @@
 // S1 = stride (adjacent lines adjacent in memory); the two passes of a
 // four-step split use S2 for the split index.  This replaces the reference's
 // line_base_for_axis (src/runtime/nd_wgsl.rs:1-55, src/math/mod.rs:506-516).
+// Element offsets are computed in 32 bits: every buffer a kernel indexes holds
+// fewer than 2^32 elements (the per-plan u32 element limit of the reference,
+// enforced by the planners).
+//
+// Inverse transforms are conj(DFT(conj(x))): conj_in / conj_out flip the sign of
+// the imaginary part on load / store.
 //
 // Algorithm: Stockham autosort, radix R_s at stage s with L_s = R_0...R_{s-1}:
 // butterfly j (< N/R) reads points j + m*N/R, multiplies point m by
@@ -35,7 +41,7 @@ struct LineArgs {
     const double2* post_lo;
     uint32_t post_shift, post_mask;
     double scale;
-    uint32_t swap_in, swap_out;
+    uint32_t conj_in, conj_out;
 };
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
@@ -46,7 +52,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 }
 
 struct LineBase {
-    int64_t in, out;
+    uint32_t in, out;  // element offsets of the line's first point
     uint32_t j;
 };
 __device__ __forceinline__ LineBase line_base(const LineArgs& a, uint32_t g) {
@@ -55,8 +61,8 @@ __device__ __forceinline__ LineBase line_base(const LineArgs& a, uint32_t g) {
     uint32_t o = fdiv(r, a.d2);
     uint32_t j = r - o * a.S2;
     LineBase b;
-    b.in = (int64_t)i * a.in_si + (int64_t)j * a.in_sj + (int64_t)o * a.in_so;
-    b.out = (int64_t)i * a.out_si + (int64_t)j * a.out_sj + (int64_t)o * a.out_so;
+    b.in = i * (uint32_t)a.in_si + j * (uint32_t)a.in_sj + o * (uint32_t)a.in_so;
+    b.out = i * (uint32_t)a.out_si + j * (uint32_t)a.out_sj + o * (uint32_t)a.out_so;
     b.j = j;
     return b;
 }
@@ -71,15 +77,25 @@ __device__ __forceinline__ cpx<T> post_twiddle(const LineArgs& a, uint64_t t) {
     return mk<T>((T)re, (T)im);
 }
 
+// Complex or real point at element index idx (not the Hermitian mode).
 template <class Fmt, int LDM>
-__device__ __forceinline__ cpx<typename Fmt::T> load_point(const LineArgs& a, int64_t base,
+__device__ __forceinline__ cpx<typename Fmt::T> load_point_at(const LineArgs& a, uint32_t idx) {
+    using T = typename Fmt::T;
+    static_assert(LDM != LD_HERMITIAN, "the Hermitian load needs the point index");
+    if constexpr (LDM == LD_REAL) return mk<T>(Fmt::ld_real(a.in, idx), T(0));
+    else return conj_by(Fmt::ld(a.in, idx), sign_mask(a.conj_in));
+}
+
+// Point p of a line starting at element offset `base`.
+template <class Fmt, int LDM>
+__device__ __forceinline__ cpx<typename Fmt::T> load_point(const LineArgs& a, uint32_t base,
                                                            uint32_t p, uint32_t n) {
     using T = typename Fmt::T;
     cpx<T> v;
     if constexpr (LDM == LD_COMPLEX) {
-        v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)p * a.in_sn));
+        v = Fmt::ld(a.in, base + p * (uint32_t)a.in_sn);
     } else if constexpr (LDM == LD_REAL) {
-        v = mk<T>(Fmt::ld_real(a.in, (uint64_t)(base + (int64_t)p * a.in_sn)), T(0));
+        v = mk<T>(Fmt::ld_real(a.in, base + p * (uint32_t)a.in_sn), T(0));
     } else {
         // Hermitian extension of a packed half line (n/2 + 1 points): the
         // imaginary part of the self-conjugate bins 0 and n/2 is dropped and the
@@ -88,27 +104,46 @@ __device__ __forceinline__ cpx<typename Fmt::T> load_point(const LineArgs& a, in
         const uint32_t h = n >> 1;
         const bool up = p > h;
         const uint32_t idx = up ? n - p : p;
-        v = Fmt::ld(a.in, (uint64_t)(base + (int64_t)idx * a.in_sn));
+        v = Fmt::ld(a.in, base + idx * (uint32_t)a.in_sn);
         v.y = (p == 0 || 2 * p == n) ? T(0) : (up ? -v.y : v.y);
     }
-    if (a.swap_in) v = swap_ri(v);
-    return v;
+    return conj_by(v, sign_mask(a.conj_in));
 }
 
+// Scale of the stored real and imaginary parts (the imaginary one carries the
+// output conjugation).
+template <class T>
+struct OutScale {
+    T re, im;
+};
+template <class T>
+__device__ __forceinline__ OutScale<T> out_scale(const LineArgs& a) {
+    const T s = (T)a.scale;
+    return OutScale<T>{s, a.conj_out ? -s : s};
+}
+
+// Output point k (of line index jline along the split axis, for the
+// four-step twiddle) at element index idx.
 template <class Fmt, int STM, bool POST>
-__device__ __forceinline__ void store_point(const LineArgs& a, int64_t base, uint32_t jline,
-                                            uint32_t k, uint32_t n, cpx<typename Fmt::T> v) {
+__device__ __forceinline__ void store_point_at(const LineArgs& a, uint32_t idx, uint32_t jline, uint32_t k,
+                                               uint32_t n, cpx<typename Fmt::T> v) {
     using T = typename Fmt::T;
     if constexpr (POST) v = cmul(v, post_twiddle<T>(a, (uint64_t)jline * k));
-    if (a.swap_out) v = swap_ri(v);
-    const T s = (T)a.scale;
+    const OutScale<T> s = out_scale<T>(a);
     if constexpr (STM == ST_COMPLEX) {
-        Fmt::st(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), scale(v, s));
+        Fmt::st(a.out, idx, mk<T>(v.x * s.re, v.y * s.im));
     } else if constexpr (STM == ST_PACKED) {
-        if (k <= (n >> 1)) Fmt::st(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), scale(v, s));
+        if (k <= (n >> 1)) Fmt::st(a.out, idx, mk<T>(v.x * s.re, v.y * s.im));
     } else {
-        Fmt::st_real(a.out, (uint64_t)(base + (int64_t)k * a.out_sk), v.x * s);
+        Fmt::st_real(a.out, idx, v.x * s.re);
     }
+}
+
+// Output point k of a line whose output starts at element offset `base`.
+template <class Fmt, int STM, bool POST>
+__device__ __forceinline__ void store_point(const LineArgs& a, uint32_t base, uint32_t jline,
+                                            uint32_t k, uint32_t n, cpx<typename Fmt::T> v) {
+    store_point_at<Fmt, STM, POST>(a, base + k * (uint32_t)a.out_sk, jline, k, n, v);
 }
 
 // ------------------------------------------------- register-resident kernel
@@ -141,10 +176,10 @@ constexpr int lds_pad(int p) { return p + (p >> 4); }
 
 // Two adjacent lines (line g even and g+1, adjacent in memory) in one 16-byte
 // access per point; the host guarantees even strides and 16-byte alignment.
+// Element index i of lines g and g+1 (adjacent in memory).
 template <class Fmt>
-__device__ __forceinline__ void load_pair(const LineArgs& a, int64_t base, uint32_t p,
-                                          cpx<typename Fmt::T>& v0, cpx<typename Fmt::T>& v1) {
-    const uint64_t i = (uint64_t)(base + (int64_t)p * a.in_sn);
+__device__ __forceinline__ void load_pair_at(const LineArgs& a, uint32_t i, cpx<typename Fmt::T>& v0,
+                                             cpx<typename Fmt::T>& v1) {
     if constexpr (Fmt::kId == 0) {
         const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(a.in) + i);
         v0 = mk<float>(q.x, q.y);
@@ -153,30 +188,24 @@ __device__ __forceinline__ void load_pair(const LineArgs& a, int64_t base, uint3
         v0 = Fmt::ld(a.in, i);
         v1 = Fmt::ld(a.in, i + 1);
     }
-    if (a.swap_in) {
-        v0 = swap_ri(v0);
-        v1 = swap_ri(v1);
-    }
+    const uint32_t m = sign_mask(a.conj_in);
+    v0 = conj_by(v0, m);
+    v1 = conj_by(v1, m);
 }
 
+// Output point k of lines g and g+1 at element index i.
 template <class Fmt, bool POST>
-__device__ __forceinline__ void store_pair(const LineArgs& a, int64_t base, uint32_t jline,
-                                           uint32_t k, cpx<typename Fmt::T> v0,
-                                           cpx<typename Fmt::T> v1) {
+__device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uint32_t jline, uint32_t k,
+                                              cpx<typename Fmt::T> v0, cpx<typename Fmt::T> v1) {
     using T = typename Fmt::T;
     if constexpr (POST) {
         const cpx<T> w = post_twiddle<T>(a, (uint64_t)jline * k);
         v0 = cmul(v0, w);
         v1 = cmul(v1, w);
     }
-    if (a.swap_out) {
-        v0 = swap_ri(v0);
-        v1 = swap_ri(v1);
-    }
-    const T s = (T)a.scale;
-    v0 = scale(v0, s);
-    v1 = scale(v1, s);
-    const uint64_t i = (uint64_t)(base + (int64_t)k * a.out_sk);
+    const OutScale<T> s = out_scale<T>(a);
+    v0 = mk<T>(v0.x * s.re, v0.y * s.im);
+    v1 = mk<T>(v1.x * s.re, v1.y * s.im);
     if constexpr (Fmt::kId == 0) {
         *reinterpret_cast<float4*>(reinterpret_cast<float2*>(a.out) + i) = make_float4(v0.x, v0.y, v1.x, v1.y);
     } else {
@@ -303,16 +332,23 @@ struct RegFft {
         const uint32_t g = g0 + mp.lg * VEC;
         const uint32_t gl = g < a.lines ? g : (a.lines - VEC);
         LineBase b = line_base(a, gl);
+        // point j + m*NR sits at o0 + (m*NR)*sn: one add per point, the
+        // products of constants and the uniform stride stay scalar
+        const uint32_t sn = (uint32_t)a.in_sn;
 #pragma unroll
         for (int q = 0; q < NB; q++) {
             uint32_t j = mp.t + q * TPL;
             if constexpr (NR % TPL != 0) j = j < (uint32_t)NR ? j : (uint32_t)(NR - 1);
+            const uint32_t o0 = b.in + j * sn;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                if constexpr (VEC == 1)
+                const uint32_t idx = o0 + (uint32_t)(m * NR) * sn;
+                if constexpr (VEC == 2)
+                    load_pair_at<Fmt>(a, idx, w[0][q * R + m], w[1][q * R + m]);
+                else if constexpr (LDM == LD_HERMITIAN)
                     w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
                 else
-                    load_pair<Fmt>(a, b.in, j + m * NR, w[0][q * R + m], w[1][q * R + m]);
+                    w[0][q * R + m] = load_point_at<Fmt, LDM>(a, idx);
             }
         }
     }
@@ -358,18 +394,22 @@ struct RegFft {
             }
             if constexpr (LAST) {
                 LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
+                const uint32_t sk = (uint32_t)a.out_sk;
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
                     const uint32_t j = mp.t + q * TPL;
                     if (valid && (NR % TPL == 0 || j < (uint32_t)NR)) {
                         const uint32_t k = j % L;
+                        const uint32_t p0 = (j - k) * R + k;
+                        const uint32_t o0 = b.out + p0 * sk;
 #pragma unroll
                         for (int m = 0; m < R; m++) {
-                            const uint32_t p = (j - k) * R + k + m * L;
+                            const uint32_t idx = o0 + (uint32_t)(m * L) * sk;
                             if constexpr (VEC == 1)
-                                store_point<Fmt, STM, POST>(a, b.out, b.j, p, N, w[0][q * R + m]);
+                                store_point_at<Fmt, STM, POST>(a, idx, b.j, p0 + m * L, N, w[0][q * R + m]);
                             else
-                                store_pair<Fmt, POST>(a, b.out, b.j, p, w[0][q * R + m], w[1][q * R + m]);
+                                store_pair_at<Fmt, POST>(a, idx, b.j, p0 + m * L, w[0][q * R + m],
+                                                         w[1][q * R + m]);
                         }
                     }
                 }

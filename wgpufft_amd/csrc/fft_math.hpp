@@ -62,6 +62,19 @@ __device__ __forceinline__ cpx<T> scale(cpx<T> a, T s) {
     return mk<T>(a.x * s, a.y * s);
 }
 
+// Sign flip by a uniform mask (0 or the sign bit): one XOR, no select.  The
+// inverse transform is conj(DFT(conj(x))); the input side conjugates with this
+// and the output side folds the sign into its scale factor.
+__device__ __forceinline__ float flip_sign(float y, uint32_t m) { return __uint_as_float(__float_as_uint(y) ^ m); }
+__device__ __forceinline__ double flip_sign(double y, uint32_t m) {
+    return __hiloint2double(__double2hiint(y) ^ (int)m, __double2loint(y));
+}
+template <class T>
+__device__ __forceinline__ cpx<T> conj_by(cpx<T> a, uint32_t m) {
+    return mk<T>(a.x, flip_sign(a.y, m));
+}
+__host__ __device__ __forceinline__ uint32_t sign_mask(uint32_t flag) { return flag ? 0x80000000u : 0u; }
+
 // ------------------------------------------------------------------ formats
 // Storage formats of the plan precisions (wgpuFFT README.md:111-123):
 //   F32  interleaved [re, im] f32        -> arithmetic in f32

@@ -80,11 +80,48 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    // Alternatives for the autotuner: other workgroup shapes and radix splits
+    // of the hot lengths (the first entry of a shape above is the default).
+    REGV(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x8", 8, 8, 8, 8),
+    REGV(FmtF32, P_F32, 4096, 256, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF32, P_F32, 4096, 128, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF32, P_F32, 2048, 256, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x4", 8, 8, 8, 4),
+    REGV(FmtF32, P_F32, 2048, 128, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x8", 16, 16, 8),
+    REGV(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4x4", 8, 8, 4, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 8, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 4096, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x16", 16, 16, 16),
+    REGV(FmtF32, P_F32, 2048, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x8", 16, 16, 8),
+    REGV(FmtF32, P_F32, 2048, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8x4", 8, 8, 8, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
 };
 
 int reg_variant_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }
 const RegVariant* reg_variant_at(int i) {
     return (i >= 0 && i < reg_variant_count()) ? &kRegistry[i] : nullptr;
+}
+
+std::vector<const RegVariant*> find_reg_variants(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
+                                                 int post) {
+    std::vector<const RegVariant*> out;
+    for (const RegVariant& v : kRegistry)
+        if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm && v.stm == stm &&
+            v.post == post)
+            out.push_back(&v);
+    return out;
+}
+
+std::string reg_variant_name(const RegVariant* v) {
+    static const char* prec[] = {"f32", "f64", "df64"};
+    char b[160];
+    snprintf(b, sizeof(b), "reg:%s:n%d:t%d:l%d:lf%d%d:m%d/%d:p%d:v%d:%s", prec[v->prec], v->n, v->tpl, v->lines,
+             v->ld_lf, v->st_lf, v->ldm, v->stm, v->post, v->vec, v->radices);
+    return b;
+}
+
+const RegVariant* find_reg_variant_named(const std::string& name) {
+    for (const RegVariant& v : kRegistry)
+        if (reg_variant_name(&v) == name) return &v;
+    return nullptr;
 }
 
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,

@@ -2,6 +2,8 @@
 // (the counterpart of the reference's typed pipeline-cache keys,
 // src/runtime/pipeline_cache.rs:2048-2260) and the launch wrappers.
 #pragma once
+#include <string>
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -31,6 +33,12 @@ struct RegVariant {
 // Registry lookup; nullptr when no compiled variant matches.
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
                                    int post, int vec);
+// Every registered variant of a line shape (any lines-per-thread width), the
+// default first; stable names for the tuning table.
+std::vector<const RegVariant*> find_reg_variants(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
+                                                 int post);
+std::string reg_variant_name(const RegVariant* v);
+const RegVariant* find_reg_variant_named(const std::string& name);
 int reg_variant_count();
 const RegVariant* reg_variant_at(int i);
 

@@ -215,6 +215,95 @@ double config_scale(const mifft_config& c) {
 // ------------------------------------------------------------------ builder
 static size_t elem_bytes(int prec) { return prec == P_F32 ? 8 : 16; }
 
+// ------------------------------------------------------------ tuning table
+namespace {
+struct TuningTable {
+    std::mutex mu;
+    std::map<std::string, std::string> m;
+};
+TuningTable& tuning_table() {
+    static TuningTable t;
+    return t;
+}
+}  // namespace
+
+std::string tuning_lookup(const std::string& key) {
+    TuningTable& t = tuning_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    auto f = t.m.find(key);
+    return f == t.m.end() ? std::string() : f->second;
+}
+
+void tuning_set(const std::string& key, const std::string& choice) {
+    TuningTable& t = tuning_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.m[key] = choice;
+}
+
+void tuning_clear() {
+    TuningTable& t = tuning_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.m.clear();
+}
+
+std::vector<std::pair<std::string, std::string>> tuning_entries() {
+    TuningTable& t = tuning_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    return std::vector<std::pair<std::string, std::string>>(t.m.begin(), t.m.end());
+}
+
+static std::string trim(const std::string& x) {
+    size_t a = x.find_first_not_of(" \t\r");
+    size_t b = x.find_last_not_of(" \t\r");
+    return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+}
+
+static bool valid_choice(const std::string& c) {
+    if (c.rfind("reg:", 0) == 0) return find_reg_variant_named(c) != nullptr;
+    if (c.rfind("lds:lpw=", 0) == 0) {
+        const std::string v = c.substr(8);
+        if (v.empty() || v.size() > 4 || v.find_first_not_of("0123456789") != std::string::npos) return false;
+        const int lpw = std::stoi(v);
+        return lpw >= 1 && lpw <= 1024;
+    }
+    return false;
+}
+
+int tuning_parse(const std::string& text) {
+    std::vector<std::pair<std::string, std::string>> got;
+    size_t pos = 0;
+    int line_no = 0;
+    while (pos <= text.size()) {
+        size_t nl = text.find('\n', pos);
+        if (nl == std::string::npos) nl = text.size();
+        const std::string line = trim(text.substr(pos, nl - pos));
+        line_no++;
+        pos = nl + 1;
+        if (line.empty() || line[0] == '#') continue;
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos)
+            fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": expected 'key = choice'");
+        const std::string key = trim(line.substr(0, eq)), choice = trim(line.substr(eq + 1));
+        const bool key_ok = (key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 || key.rfind("df64|n", 0) == 0);
+        if (!key_ok || !valid_choice(choice))
+            fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": unknown key or choice");
+        got.emplace_back(key, choice);
+    }
+    for (auto& kv : got) tuning_set(kv.first, kv.second);
+    return (int)got.size();
+}
+
+static std::string tune_key(int prec, uint64_t n, bool ld_lf, bool st_lf, int ldm, int stm, int post, bool pairable,
+                            uint64_t lines) {
+    static const char* pn[] = {"f32", "f64", "df64"};
+    int lb = 0;
+    while ((1ull << lb) < lines) lb++;
+    char b[160];
+    snprintf(b, sizeof(b), "%s|n%llu|lf%d%d|m%d/%d|p%d|pair%d|lines2^%d", pn[prec], (unsigned long long)n, (int)ld_lf,
+             (int)st_lf, ldm, stm, post, (int)pairable, lb);
+    return b;
+}
+
 struct Layout {
     int64_t si, sj, so, sn;
 };
@@ -341,8 +430,8 @@ struct Builder {
         return r;
     }
 
-    LineArgs base_args(const LineSet& ls, const Layout& in, const Layout& out, bool swap_in,
-                       bool swap_out, double scale) {
+    LineArgs base_args(const LineSet& ls, const Layout& in, const Layout& out, bool conj_in,
+                       bool conj_out, double scale) {
         LineArgs a;
         std::memset(&a, 0, sizeof(a));
         a.lines = ls.lines;
@@ -354,15 +443,15 @@ struct Builder {
         a.out_si = out.si; a.out_sj = out.sj; a.out_so = out.so; a.out_sk = out.sn;
         a.tw = nullptr;
         a.scale = scale;
-        a.swap_in = swap_in;
-        a.swap_out = swap_out;
+        a.conj_in = conj_in;
+        a.conj_out = conj_out;
         return a;
     }
 
     // One line kernel (register variant when compiled, else the LDS kernel).
     // Returns false if the line does not fit on chip.
     bool line_op(uint64_t n, const LineSet& ls, const Layout& in, const Layout& out, int src,
-                 uint64_t src_off, int dst, uint64_t dst_off, bool swap_in, bool swap_out,
+                 uint64_t src_off, int dst, uint64_t dst_off, bool conj_in, bool conj_out,
                  double scale, int ldm, int stm, const double2* post_hi = nullptr,
                  const double2* post_lo = nullptr, uint32_t post_shift = 0, uint32_t post_mask = 0) {
         const int post = post_hi != nullptr;
@@ -383,14 +472,37 @@ struct Builder {
                               ls.lines % 2 == 0 && ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 &&
                               in.sj % 2 == 0 && in.so % 2 == 0 && in.sn % 2 == 0 &&
                               out.sj % 2 == 0 && out.so % 2 == 0 && out.sn % 2 == 0;
-        const RegVariant* rv =
-            pairable ? find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 2) : nullptr;
-        if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
+        op.n = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
+        op.ld_lf = ld_lf;
+        op.st_lf = st_lf;
+        op.pairable = pairable;
+        op.tkey = tune_key(P.prec, n, ld_lf, st_lf, ldm, stm, post, pairable, ls.lines);
+        // A measured choice from the tuning table wins over the default
+        // (batched plans keep the fixed set of batched classes).
+        const std::string choice = P.batch_mode ? std::string() : tuning_lookup(op.tkey);
+        const RegVariant* rv = nullptr;
+        uint32_t forced_lpw = 0;
+        bool tuned = false;
+        if (choice.rfind("reg:", 0) == 0) {
+            const RegVariant* t = find_reg_variant_named(choice);
+            if (t && t->prec == P.prec && t->n == (int)n && t->ld_lf == (int)ld_lf && t->st_lf == (int)st_lf &&
+                t->ldm == ldm && t->stm == stm && t->post == post && (t->vec == 1 || pairable)) {
+                rv = t;
+                tuned = true;
+            }
+        } else if (choice.rfind("lds:lpw=", 0) == 0) {
+            forced_lpw = (uint32_t)std::strtoul(choice.c_str() + 8, nullptr, 10);
+            tuned = forced_lpw > 0;
+        }
+        if (!tuned) {
+            rv = pairable ? find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 2) : nullptr;
+            if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
+        }
         const size_t es = elem_bytes(P.arith);
         const uint64_t lstr = lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30));
         if (rv && P.batch_mode && batch_class_for(rv) < 0) rv = nullptr;
         if (!rv && 2 * lstr * es > P.lds_budget) return false;
-        op.a = base_args(ls, in, out, swap_in, swap_out, scale);
+        op.a = base_args(ls, in, out, conj_in, conj_out, scale);
         op.a.post_hi = post_hi;
         op.a.post_lo = post_lo;
         op.a.post_shift = post_shift;
@@ -409,6 +521,7 @@ struct Builder {
             uint32_t lpw;
             if (ld_lf || st_lf) lpw = P.arith == P_F32 ? 16 : 8;
             else lpw = (uint32_t)std::max<uint64_t>(1, 4096 / n);
+            if (forced_lpw) lpw = forced_lpw;
             lpw = std::max<uint32_t>(1, std::min(lpw, fit));
             op.sc = make_sched(n, lpw, ld_lf, st_lf);
             op.a.tw = stage_twiddles(P.arith, sched_radices(op.sc));
@@ -467,7 +580,7 @@ struct Builder {
 
     void rader_op(uint64_t p, uint64_t m, uint32_t lpw, const LineSet& ls, const Layout& in,
                   const Layout& out, int src, uint64_t src_off, int dst, uint64_t dst_off,
-                  bool swap_in, bool swap_out, double scale) {
+                  bool conj_in, bool conj_out, double scale) {
         const uint64_t g = primitive_root(p);
         const uint64_t ginv = mod_pow_u64(g, p - 2, p);
         std::vector<uint32_t> pin(p - 1), pout(p - 1);
@@ -492,7 +605,7 @@ struct Builder {
         op.dst = dst;
         op.dst_off = dst_off;
         op.prec = P.prec;
-        op.conv.io = base_args(ls, in, out, swap_in, swap_out, scale);
+        op.conv.io = base_args(ls, in, out, conj_in, conj_out, scale);
         op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
         op.conv.io.tw = stage_twiddles(P.arith, sched_radices(op.conv.sc));
         op.conv.filt = filt;
@@ -525,7 +638,7 @@ struct Builder {
 
     void bluestein_fused_op(uint64_t n, uint64_t m, uint32_t lpw, const LineSet& ls,
                             const Layout& in, const Layout& out, int src, uint64_t src_off,
-                            int dst, uint64_t dst_off, bool swap_in, bool swap_out, double scale) {
+                            int dst, uint64_t dst_off, bool conj_in, bool conj_out, double scale) {
         Op op;
         op.kind = OP_BLUESTEIN;
         op.src = src;
@@ -533,7 +646,7 @@ struct Builder {
         op.dst = dst;
         op.dst_off = dst_off;
         op.prec = P.prec;
-        op.conv.io = base_args(ls, in, out, swap_in, swap_out, scale);
+        op.conv.io = base_args(ls, in, out, conj_in, conj_out, scale);
         op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
         op.conv.io.tw = stage_twiddles(P.arith, sched_radices(op.conv.sc));
         op.conv.filt = bluestein_filter(n, m);
@@ -551,7 +664,7 @@ struct Builder {
     // workspace bytes used from ws_off.
     uint64_t bluestein_multi(uint64_t n, uint64_t m, const LineSet& ls, const Layout& in,
                              const Layout& out, int src, uint64_t src_off, int dst,
-                             uint64_t dst_off, uint64_t ws_off, bool swap_in, bool swap_out,
+                             uint64_t dst_off, uint64_t ws_off, bool conj_in, bool conj_out,
                              double scale) {
         if (P.prec == P_DF64)
             fail(MIFFT_ERR_PRECISION_UNSUPPORTED, "df64 Bluestein beyond the on-chip convolution length");
@@ -577,7 +690,7 @@ struct Builder {
         Plan* child = P.children.back().get();
         const void* chirp = chirp_table(n);
         const void* filt = bluestein_filter(n, m);
-        LineArgs a = base_args(ls, in, out, swap_in, swap_out, scale);
+        LineArgs a = base_args(ls, in, out, conj_in, conj_out, scale);
         Op pre;
         pre.kind = OP_CHIRP_PRE;
         pre.src = src;
@@ -586,7 +699,7 @@ struct Builder {
         pre.dst_off = ws_off;
         pre.prec = P.prec;
         pre.a = a;
-        pre.a.swap_out = 0;
+        pre.a.conj_out = 0;
         pre.a.scale = 1.0;
         pre.chirp = chirp;
         pre.n = (uint32_t)n;
@@ -620,7 +733,7 @@ struct Builder {
         post.dst_off = dst_off;
         post.prec = P.prec;
         post.a = a;
-        post.a.swap_in = 0;
+        post.a.conj_in = 0;
         post.chirp = chirp;
         post.n = (uint32_t)n;
         post.m = (uint32_t)m;
@@ -634,7 +747,7 @@ struct Builder {
     // workspace, pass 2 transforms n2-point rows and stores transposed
     // (k = k1 + n1 k2).  Returns false if no split fits.
     bool four_step(uint64_t n, uint64_t S, uint64_t O, int src, uint64_t src_off, int dst,
-                   uint64_t dst_off, uint64_t ws_off, bool swap_in, bool swap_out, double scale,
+                   uint64_t dst_off, uint64_t ws_off, bool conj_in, bool conj_out, double scale,
                    uint64_t* ws_used) {
         // Prefer a balanced split with register variants on both sides.
         uint64_t best1 = 0;
@@ -670,14 +783,14 @@ struct Builder {
         LineSet l1{(uint32_t)(lines_total * n2), (uint32_t)S, (uint32_t)n2};
         Layout in1{1, (int64_t)S, (int64_t)(S * N), (int64_t)(S * n2)};
         Layout ws1 = in1;
-        if (!line_op(n1, l1, in1, ws1, src, src_off, B_WS, ws_off, swap_in, false, 1.0, LD_COMPLEX,
+        if (!line_op(n1, l1, in1, ws1, src, src_off, B_WS, ws_off, conj_in, false, 1.0, LD_COMPLEX,
                      ST_COMPLEX, hi, lo, sh, (uint32_t)(lo_n - 1)))
             return false;
         // pass 2: lines (i < S, k1 < n1, o < O), points q2 at stride S
         LineSet l2{(uint32_t)(lines_total * n1), (uint32_t)S, (uint32_t)n1};
         Layout in2{1, (int64_t)(S * n2), (int64_t)(S * N), (int64_t)S};
         Layout out2{1, (int64_t)S, (int64_t)(S * N), (int64_t)(S * n1)};
-        if (!line_op(n2, l2, in2, out2, B_WS, ws_off, dst, dst_off, false, swap_out, scale,
+        if (!line_op(n2, l2, in2, out2, B_WS, ws_off, dst, dst_off, false, conj_out, scale,
                      LD_COMPLEX, ST_COMPLEX))
             return false;
         *ws_used = lines_total * N * elem_bytes(P.prec);
@@ -1043,6 +1156,95 @@ std::string describe_plan(const Plan& p) {
         d << op.desc;
     }
     return d.str();
+}
+
+// ------------------------------------------------------------- autotuner
+int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report) {
+    if (reps < 1 || reps > 10000) fail(MIFFT_ERR_INVALID_TUNING, "autotune repetitions must be 1..10000");
+    std::unique_ptr<Plan> P = build_plan(cfg, kind);
+    // scratch buffers of the plan's sizes and a holder for candidate tables
+    Plan T;
+    T.cfg = P->cfg;
+    T.prec = P->prec;
+    T.arith = P->arith;
+    T.lds_budget = P->lds_budget;
+    auto alloc = [&](size_t bytes) -> void* {
+        DevBuf b;
+        b.bytes = std::max<size_t>(bytes, 256);
+        hip_check(hipMalloc(&b.p, b.bytes), "autotune buffer");
+        T.owned.push_back(b);
+        hip_check(hipMemsetAsync(b.p, 0, b.bytes, s), "autotune buffer");
+        return b.p;
+    };
+    void* in = alloc(P->in_bytes);
+    void* out = alloc(P->out_bytes);
+    void* ws = alloc(P->ws_bytes);
+    Builder TB{T, s, {}};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "autotune event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        fail(MIFFT_ERR_HIP, "autotune event");
+    }
+    int tuned = 0;
+    try {
+        for (const Op& op : P->ops) {
+            if ((op.kind != OP_REG && op.kind != OP_LDS) || op.tkey.empty()) continue;
+            std::vector<std::pair<std::string, Op>> cands;
+            for (const RegVariant* v : find_reg_variants(op.prec, (int)op.n, op.ld_lf, op.st_lf, op.ldm, op.stm, op.post)) {
+                if ((v->vec == 2 && !op.pairable) || v->lds_bytes > kLdsBudget) continue;
+                Op c = op;
+                c.kind = OP_REG;
+                c.rv = v;
+                c.a.tw = TB.stage_twiddles(P->arith, Builder::parse_radices(v->radices));
+                cands.emplace_back(reg_variant_name(v), c);
+            }
+            const uint64_t lstr = lds_line_stride(op.n);
+            const size_t es = elem_bytes(P->arith);
+            for (uint32_t lpw = 1; lpw <= 32; lpw *= 2) {
+                if (2 * lstr * es * lpw > P->lds_budget || lpw > op.a.lines) break;
+                Op c = op;
+                c.kind = OP_LDS;
+                c.rv = nullptr;
+                c.sc = make_sched(op.n, lpw, op.ld_lf, op.st_lf);
+                c.a.tw = TB.stage_twiddles(P->arith, Builder::sched_radices(c.sc));
+                cands.emplace_back("lds:lpw=" + std::to_string(lpw), c);
+            }
+            if (cands.size() < 2) continue;
+            hip_check(hipStreamSynchronize(s), "autotune tables");
+            std::string best;
+            float best_ms = 0.0f;
+            std::ostringstream line;
+            line << op.tkey << ":";
+            for (auto& c : cands) {
+                for (int w = 0; w < 2; w++) run_op(c.second, s, in, out, ws);
+                hip_check(hipEventRecord(e0, s), "autotune event");
+                for (int r = 0; r < reps; r++) run_op(c.second, s, in, out, ws);
+                hip_check(hipEventRecord(e1, s), "autotune event");
+                hip_check(hipEventSynchronize(e1), "autotune event");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "autotune event");
+                ms /= (float)reps;
+                line << " " << c.first << "=" << ms * 1e3f << "us";
+                if (best.empty() || ms < best_ms) {
+                    best = c.first;
+                    best_ms = ms;
+                }
+            }
+            tuning_set(op.tkey, best);
+            tuned++;
+            if (report) *report += line.str() + " -> " + best + "\n";
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    hip_check(hipStreamSynchronize(s), "autotune");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return tuned;
 }
 
 }  // namespace mifft

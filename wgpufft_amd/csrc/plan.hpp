@@ -58,6 +58,10 @@ struct Op {
     uint64_t count = 0;
     double scale = 1.0;
     std::string desc;
+    // line ops: lane layout, whether line pairs are allowed, and the
+    // tuning-table key of the op's shape
+    bool ld_lf = false, st_lf = false, pairable = false;
+    std::string tkey;
 };
 
 struct DevBuf {
@@ -111,6 +115,23 @@ void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws)
 bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const void* in, void* out, void* ws,
                      std::string* desc);
 int plan_launch_count(const Plan& p);
+
+// Process-wide tuning table: measured line-kernel choices keyed by op shape
+// (precision, length, lane layout, load/store modes, twiddle, pairability,
+// line-count bucket).  A choice is a registered variant name ("reg:...") or
+// a runtime-schedule LDS shape ("lds:lpw=<lines per workgroup>").
+std::string tuning_lookup(const std::string& key);
+void tuning_set(const std::string& key, const std::string& choice);
+void tuning_clear();
+std::vector<std::pair<std::string, std::string>> tuning_entries();
+// Parses "key = choice" lines ('#' comments); throws on a malformed line or
+// an unknown variant.  Returns the number of entries read.
+int tuning_parse(const std::string& text);
+// Times every candidate of each line op of the plan of (cfg, kind) on
+// scratch buffers of the plan's sizes (reps launches each) and records the
+// fastest; returns the number of ops tuned and appends one line per op to
+// `report`.
+int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report);
 std::string describe_plan(const Plan& p);
 
 // Host-side integer helpers (the reference's route-choice arithmetic).
