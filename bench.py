@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--transforms", type=int, default=2000, help="batched workload: transform count")
     ap.add_argument("--nbuf", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tuning", default="table", choices=["table", "auto", "none"],
+                    help="kernel choices: the committed MI355X tuning table (wgpufft_amd/tuning/mi355x.txt, "
+                         "made by python -m wgpufft_amd.tune), a fresh autotune of this workload, or the "
+                         "planner's defaults")
     ap.add_argument("--calibrate", action="store_true",
                     help="profiling driver: 5 plain 64 MiB device copies (torch), then 5 launches "
                          "of every kernel of the workload; prints nothing else")
@@ -95,7 +99,9 @@ def main():
     import numpy as np
     import torch
 
-    from wgpufft_amd import FftConfig, FftPlan, FftPrecision, fill_normal_f32, fill_uniform_f32
+    from wgpufft_amd import (FftConfig, FftPlan, FftPrecision, FftTransformKind, autotune_sequence, clear_tuning,
+                             fill_normal_f32, fill_uniform_f32, load_tuning, tuning_size)
+    from wgpufft_amd.tune import DEFAULT_TABLE
 
     if args.workload == "batched":
         print(json.dumps(run_batched(args)), flush=True)
@@ -112,6 +118,28 @@ def main():
     plans = []
     prec = {"f32": FftPrecision.F32, "f64": FftPrecision.F64,
             "df64": FftPrecision.Df64}[spec.get("precision", "f32")]
+    # kernel choices: committed tuning table, fresh autotune, or defaults
+    clear_tuning()
+    mode = args.tuning
+    if mode == "table" and not os.path.exists(DEFAULT_TABLE):
+        mode = "auto"
+    tuned = {"mode": mode}
+    if mode == "table":
+        load_tuning(DEFAULT_TABLE)
+        tuned.update(path=os.path.relpath(DEFAULT_TABLE, ROOT), entries=tuning_size())
+    if mode == "auto":
+        t0 = time.perf_counter()
+        if spec["kind"] == "r2c_c2r":
+            jobs = [(FftConfig.new_nd(shape), FftTransformKind.R2c), (FftConfig.inverse_nd(shape), FftTransformKind.C2r)]
+        elif spec["kind"] == "c2c_pair":
+            jobs = [(FftConfig.new_nd(shape).with_batch(batch).with_precision(prec), FftTransformKind.C2c),
+                    (FftConfig.inverse_nd(shape).with_batch(batch).with_precision(prec), FftTransformKind.C2c)]
+        else:
+            jobs = [(FftConfig.new_nd(shape).with_batch(batch).with_precision(prec), FftTransformKind.C2c)]
+        report = autotune_sequence(jobs, reps=20, stream=stream)[1]
+        tuned.update(seconds=round(time.perf_counter() - t0, 3),
+                     choices=[ln.split(" -> ")[0].split(":")[0] + " -> " + ln.split(" -> ")[1]
+                              for ln in report.strip().splitlines() if " -> " in ln])
     if spec["kind"] == "c2c_pair":
         fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch).with_precision(prec))
         inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch).with_precision(prec))
@@ -300,6 +328,7 @@ def main():
         "kernels": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": tuned,
         "serve": serve,
     }
     print(json.dumps(line), flush=True)

@@ -278,15 +278,26 @@ void mifft_server_destroy(mifft_server* s);
  * keyed by the shape of each line pass; without an entry it uses its
  * defaults.
  *
- * mifft_autotune builds the plan of (config, kind), times every candidate of
- * each of its line passes on scratch buffers of the plan's sizes (`reps`
- * launches each, HIP events on `stream`) and records the fastest.  Plans
- * created afterwards use the table.  *passes_tuned (optional) receives the
+ * mifft_autotune builds the plan of (config, kind) and, pass by pass, times
+ * the whole plan with each candidate of that pass swapped in (`reps`
+ * executions per trial, three interleaved trials, HIP events on `stream`,
+ * input and output buffers rotated beyond the 256 MB MALL, one workspace, in
+ * place when the plan allows it), keeping each pass's fastest candidate for
+ * the passes after it, and records the winners.  Plans created afterwards
+ * use the table.  *passes_tuned (optional) receives the
  * number of passes that had more than one candidate; report (optional,
  * report_cap bytes) receives one text line per pass with every candidate's
  * time. */
 mifft_status mifft_autotune(const mifft_config* config, mifft_kind kind, int32_t reps, hipStream_t stream,
                             int32_t* passes_tuned, char* report, uint64_t report_cap);
+/* The same over `count` (1..16) plans executed back to back, as a caller runs
+ * them: in place on one array when every plan allows it (a forward+inverse
+ * pair), otherwise each plan reading the previous plan's output (R2C then
+ * C2R).  Shapes are tuned in launch order; a candidate applies to every pass
+ * of its shape in every plan and the whole sequence is timed. */
+mifft_status mifft_autotune_sequence(const mifft_config* configs, const mifft_kind* kinds, int32_t count,
+                                     int32_t reps, hipStream_t stream, int32_t* shapes_tuned, char* report,
+                                     uint64_t report_cap);
 /* Text form "key = choice", one entry per line ('#' starts a comment). */
 mifft_status mifft_tuning_table_save(const char* path);
 /* Adds the entries of a saved table; a malformed line or an unknown variant

@@ -234,7 +234,8 @@ def test_df64(torch_cuda, shape, batch):
 
 # Full-size high-precision configurations: 3D 256^3 (three axis passes) and the
 # 1D 2^24 four-step, against the multithreaded FP64 oracle.
-@pytest.mark.parametrize("shape,prec", [([256, 256, 256], "f64"), ([256, 256, 256], "df64"), ([1 << 24], "f64")])
+@pytest.mark.parametrize("shape,prec", [([256, 256, 256], "f64"), ([256, 256, 256], "df64"), ([1 << 24], "f64"),
+                                        ([1 << 24], "df64")])
 def test_high_precision_full_size(torch_cuda, shape, prec):
     import torch
     cnt = int(np.prod(shape))

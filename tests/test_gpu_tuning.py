@@ -102,3 +102,32 @@ def test_autotune_records_and_plans_follow(torch_cuda, tmp_path):
     W.clear_tuning()
     W.load_tuning(str(p))
     assert FftPlan.c2c(FftConfig.new_nd(shape)).describe() == desc
+
+
+def test_autotune_sequence_pair_and_real_chain(torch_cuda):
+    """A forward+inverse pair tuned as it runs (in place) and an R2C/C2R chain;
+    plans built afterwards still match the oracle and round-trip."""
+    from wgpufft_amd import FftTransformKind as K
+    shape = [1024, 512]
+    n, report = W.autotune_sequence([(FftConfig.new_nd(shape), K.C2c), (FftConfig.inverse_nd(shape), K.C2c)],
+                                    reps=3)
+    print(report, flush=True)
+    assert n >= 1 and report.count("->") == n
+    x = seeded_complex(92, 1024 * 512)
+    plan, y = _run(FftConfig.new_nd(shape), x)
+    assert oracle.rel_l2(y, c2c_expected(x, shape)) <= 1e-6
+    n2, report2 = W.autotune_sequence([(FftConfig.new_nd([512, 256]), K.R2c),
+                                       (FftConfig.inverse_nd([512, 256]), K.C2r)], reps=3)
+    print(report2, flush=True)
+    assert n2 >= 1
+    import torch
+    xr = oracle.normal_f32(93, 512 * 256)
+    r2c = FftPlan.r2c(FftConfig.new_nd([512, 256]))
+    c2r = FftPlan.c2r(FftConfig.inverse_nd([512, 256]))
+    d = to_dev(xr)
+    pk = empty_dev(r2c.required_output_buffer_size_bytes())
+    back = empty_dev(xr.nbytes)
+    r2c.execute(d, pk)
+    c2r.execute(pk, back)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(from_dev(back, np.float32), xr) <= 1e-6

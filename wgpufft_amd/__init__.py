@@ -526,6 +526,21 @@ def autotune(config: FftConfig, kind: FftTransformKind = FftTransformKind.C2c, r
     return n.value, buf.value.decode()
 
 
+def autotune_sequence(jobs, reps: int = 20, stream=None) -> tuple:
+    """Tunes the kernel shapes of several plans executed back to back, as a
+    caller runs them (a forward+inverse pair in place; R2C then C2R chained).
+    `jobs` is a list of (FftConfig, FftTransformKind).  Returns (shapes tuned,
+    report text)."""
+    lib = _lib.load()
+    cs = (_lib.Config * len(jobs))(*[cfg._to_c() for cfg, _ in jobs])
+    ks = (ctypes.c_int32 * len(jobs))(*[int(k) for _, k in jobs])
+    n = ctypes.c_int32()
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(lib.mifft_autotune_sequence(cs, ks, len(jobs), int(reps), _stream(stream), ctypes.byref(n), buf,
+                                       len(buf)))
+    return n.value, buf.value.decode()
+
+
 def save_tuning(path: str) -> None:
     _check(_lib.load().mifft_tuning_table_save(os.fsencode(path)))
 

@@ -135,6 +135,8 @@ SIGNATURES = [
     ("mifft_last_error_detail", cp, []),
     ("mifft_version", cp, []),
     ("mifft_autotune", i32, [ctypes.POINTER(Config), i32, i32, vp, ctypes.POINTER(i32), ctypes.c_char_p, u64]),
+    ("mifft_autotune_sequence", i32, [ctypes.POINTER(Config), ctypes.POINTER(i32), i32, i32, vp,
+                                       ctypes.POINTER(i32), ctypes.c_char_p, u64]),
     ("mifft_tuning_table_save", i32, [ctypes.c_char_p]),
     ("mifft_tuning_table_load", i32, [ctypes.c_char_p]),
     ("mifft_tuning_table_parse", i32, [ctypes.c_char_p, ctypes.POINTER(i32)]),

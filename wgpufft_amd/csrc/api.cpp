@@ -353,6 +353,22 @@ mifft_status mifft_autotune(const mifft_config* config, mifft_kind kind, int32_t
     });
 }
 
+mifft_status mifft_autotune_sequence(const mifft_config* configs, const mifft_kind* kinds, int32_t count,
+                                     int32_t reps, hipStream_t stream, int32_t* shapes_tuned, char* report,
+                                     uint64_t report_cap) {
+    if (!configs || !kinds) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null configs or kinds");
+    return guard([&] {
+        std::string text;
+        const int n = autotune_sequence(configs, kinds, count, reps, stream, &text);
+        if (shapes_tuned) *shapes_tuned = n;
+        if (report && report_cap) {
+            const size_t k = std::min<size_t>(text.size(), (size_t)report_cap - 1);
+            std::memcpy(report, text.data(), k);
+            report[k] = 0;
+        }
+    });
+}
+
 mifft_status mifft_tuning_table_save(const char* path) {
     if (!path) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null path");
     return guard([&] {

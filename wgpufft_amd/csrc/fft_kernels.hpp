@@ -77,13 +77,15 @@ __device__ __forceinline__ cpx<T> post_twiddle(const LineArgs& a, uint64_t t) {
     return mk<T>((T)re, (T)im);
 }
 
-// Complex or real point at element index idx (not the Hermitian mode).
-template <class Fmt, int LDM>
+// Complex or real point at element index idx (not the Hermitian mode); NT
+// selects the non-temporal load.
+template <class Fmt, int LDM, bool NT = true>
 __device__ __forceinline__ cpx<typename Fmt::T> load_point_at(const LineArgs& a, uint32_t idx) {
     using T = typename Fmt::T;
     static_assert(LDM != LD_HERMITIAN, "the Hermitian load needs the point index");
     if constexpr (LDM == LD_REAL) return mk<T>(Fmt::ld_real(a.in, idx), T(0));
-    else return conj_by(Fmt::ld(a.in, idx), sign_mask(a.conj_in));
+    else if constexpr (NT) return conj_by(Fmt::ld(a.in, idx), sign_mask(a.conj_in));
+    else return conj_by(Fmt::ld_c(a.in, idx), sign_mask(a.conj_in));
 }
 
 // Point p of a line starting at element offset `base`.
@@ -177,16 +179,22 @@ constexpr int lds_pad(int p) { return p + (p >> 4); }
 // Two adjacent lines (line g even and g+1, adjacent in memory) in one 16-byte
 // access per point; the host guarantees even strides and 16-byte alignment.
 // Element index i of lines g and g+1 (adjacent in memory).
-template <class Fmt>
+template <class Fmt, bool NT = true>
 __device__ __forceinline__ void load_pair_at(const LineArgs& a, uint32_t i, cpx<typename Fmt::T>& v0,
                                              cpx<typename Fmt::T>& v1) {
     if constexpr (Fmt::kId == 0) {
-        const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(a.in) + i);
+        const vf4_t* q4 = reinterpret_cast<const vf4_t*>(reinterpret_cast<const float2*>(a.in) + i);
+        vf4_t q;
+        if constexpr (NT) q = __builtin_nontemporal_load(q4);
+        else q = *q4;
         v0 = mk<float>(q.x, q.y);
         v1 = mk<float>(q.z, q.w);
-    } else {
+    } else if constexpr (NT) {
         v0 = Fmt::ld(a.in, i);
         v1 = Fmt::ld(a.in, i + 1);
+    } else {
+        v0 = Fmt::ld_c(a.in, i);
+        v1 = Fmt::ld_c(a.in, i + 1);
     }
     const uint32_t m = sign_mask(a.conj_in);
     v0 = conj_by(v0, m);
@@ -243,9 +251,16 @@ struct RegFft {
     static constexpr int NT = TPL * GROUPS;
     static constexpr int LSTR = lds_pad(N) + 1;
     static_assert(VEC == 1 || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
+    // Lines of 16-byte points whose exchange buffer would exceed 80 KiB go
+    // through LDS one component at a time: half the space, twice the rounds
+    // (four 4096-point f64 lines per workgroup instead of two).
+    static constexpr bool SPLIT = sizeof(cpx<T>) == 16 && (size_t)GROUPS * LSTR * sizeof(cpx<T>) > 80 * 1024;
     static constexpr size_t lds_bytes() {
-        return NS > 1 ? (size_t)GROUPS * LSTR * sizeof(cpx<T>) : 0;
+        return NS > 1 ? (size_t)GROUPS * LSTR * (SPLIT ? sizeof(T) : sizeof(cpx<T>)) : 0;
     }
+    // Non-temporal loads only where a load instruction covers whole
+    // 128-byte row segments: contiguous lines, or line groups that wide.
+    static constexpr bool NTL = !LD_LF || (size_t)GROUPS * VEC * sizeof(cpx<T>) >= 128;
     static constexpr int max_regs(int s) {
         return s >= NS ? 0
                        : (ceil_div(N / RL::at(s), TPL) * RL::at(s) > max_regs(s + 1)
@@ -277,27 +292,61 @@ struct RegFft {
         const Map mr = map_of<s + 1>();
 #pragma unroll
         for (int v = 0; v < VEC; v++) {
-            cpx<T>* ln = lds + mw.lg * LSTR;
+            if constexpr (!SPLIT) {
+                cpx<T>* ln = lds + mw.lg * LSTR;
 #pragma unroll
-            for (int q = 0; q < NB; q++) {
-                const uint32_t j = mw.t + q * TPL;
-                if (NR % TPL == 0 || j < (uint32_t)NR) {
-                    const uint32_t k = j % L;
+                for (int q = 0; q < NB; q++) {
+                    const uint32_t j = mw.t + q * TPL;
+                    if (NR % TPL == 0 || j < (uint32_t)NR) {
+                        const uint32_t k = j % L;
 #pragma unroll
-                    for (int m = 0; m < R; m++) ln[lds_pad((j - k) * R + k + m * L)] = w[v][q * R + m];
+                        for (int m = 0; m < R; m++) ln[lds_pad((j - k) * R + k + m * L)] = w[v][q * R + m];
+                    }
+                }
+                __syncthreads();
+                const cpx<T>* lr = lds + mr.lg * LSTR;
+#pragma unroll
+                for (int q = 0; q < NB2; q++) {
+                    const uint32_t j = mr.t + q * TPL;
+                    if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
+#pragma unroll
+                        for (int m = 0; m < R2; m++) w[v][q * R2 + m] = lr[lds_pad(j + m * NR2)];
+                    }
+                }
+                __syncthreads();
+            } else {
+                // Round c moves component c only; a slot's other component still
+                // holds its stage-s value when its own round comes.
+                T* lt = reinterpret_cast<T*>(lds);
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    T* ln = lt + mw.lg * LSTR;
+#pragma unroll
+                    for (int q = 0; q < NB; q++) {
+                        const uint32_t j = mw.t + q * TPL;
+                        if (NR % TPL == 0 || j < (uint32_t)NR) {
+                            const uint32_t k = j % L;
+#pragma unroll
+                            for (int m = 0; m < R; m++)
+                                ln[lds_pad((j - k) * R + k + m * L)] = c ? w[v][q * R + m].y : w[v][q * R + m].x;
+                        }
+                    }
+                    __syncthreads();
+                    const T* lr = lt + mr.lg * LSTR;
+#pragma unroll
+                    for (int q = 0; q < NB2; q++) {
+                        const uint32_t j = mr.t + q * TPL;
+                        if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
+#pragma unroll
+                            for (int m = 0; m < R2; m++) {
+                                if (c) w[v][q * R2 + m].y = lr[lds_pad(j + m * NR2)];
+                                else w[v][q * R2 + m].x = lr[lds_pad(j + m * NR2)];
+                            }
+                        }
+                    }
+                    __syncthreads();
                 }
             }
-            __syncthreads();
-            const cpx<T>* lr = lds + mr.lg * LSTR;
-#pragma unroll
-            for (int q = 0; q < NB2; q++) {
-                const uint32_t j = mr.t + q * TPL;
-                if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
-#pragma unroll
-                    for (int m = 0; m < R2; m++) w[v][q * R2 + m] = lr[lds_pad(j + m * NR2)];
-                }
-            }
-            __syncthreads();
         }
     }
 
@@ -344,11 +393,11 @@ struct RegFft {
             for (int m = 0; m < R; m++) {
                 const uint32_t idx = o0 + (uint32_t)(m * NR) * sn;
                 if constexpr (VEC == 2)
-                    load_pair_at<Fmt>(a, idx, w[0][q * R + m], w[1][q * R + m]);
+                    load_pair_at<Fmt, NTL>(a, idx, w[0][q * R + m], w[1][q * R + m]);
                 else if constexpr (LDM == LD_HERMITIAN)
                     w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
                 else
-                    w[0][q * R + m] = load_point_at<Fmt, LDM>(a, idx);
+                    w[0][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx);
             }
         }
     }

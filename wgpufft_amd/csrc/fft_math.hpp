@@ -82,11 +82,25 @@ __host__ __device__ __forceinline__ uint32_t sign_mask(uint32_t flag) { return f
 //   DF64 [re_hi, re_lo, im_hi, im_lo] f32, value = f64(hi) + f64(lo)
 //        -> arithmetic in native f64; split on store hi = f32(v), lo = f32(v - hi)
 //        (src/math/double_float.rs:20-31).
+// Line data is read once per pass: `ld` carries the non-temporal hint,
+// `ld_c` is a plain load.  Measured on MI355X: the hint helps streams of
+// whole 128-byte rows (contiguous lines) and hurts narrower strided segments
+// (a 64-byte segment per row pass ran 36 % slower); non-temporal stores are
+// slower everywhere (they give up the MALL reuse of a pass's output by the
+// next pass).
+typedef float vf2_t __attribute__((ext_vector_type(2)));
+typedef float vf4_t __attribute__((ext_vector_type(4)));
+typedef double vd2_t __attribute__((ext_vector_type(2)));
+
 struct FmtF32 {
     using T = float;
     static constexpr int kId = 0;
     static __device__ __forceinline__ cpx<float> ld(const void* p, uint64_t i) {
-        float2 v = reinterpret_cast<const float2*>(p)[i];
+        const vf2_t v = __builtin_nontemporal_load(reinterpret_cast<const vf2_t*>(p) + i);
+        return mk<float>(v.x, v.y);
+    }
+    static __device__ __forceinline__ cpx<float> ld_c(const void* p, uint64_t i) {
+        const vf2_t v = reinterpret_cast<const vf2_t*>(p)[i];
         return mk<float>(v.x, v.y);
     }
     static __device__ __forceinline__ void st(void* p, uint64_t i, cpx<float> v) {
@@ -104,7 +118,11 @@ struct FmtF64 {
     using T = double;
     static constexpr int kId = 1;
     static __device__ __forceinline__ cpx<double> ld(const void* p, uint64_t i) {
-        double2 v = reinterpret_cast<const double2*>(p)[i];
+        const vd2_t v = __builtin_nontemporal_load(reinterpret_cast<const vd2_t*>(p) + i);
+        return mk<double>(v.x, v.y);
+    }
+    static __device__ __forceinline__ cpx<double> ld_c(const void* p, uint64_t i) {
+        const vd2_t v = reinterpret_cast<const vd2_t*>(p)[i];
         return mk<double>(v.x, v.y);
     }
     static __device__ __forceinline__ void st(void* p, uint64_t i, cpx<double> v) {
@@ -130,7 +148,11 @@ struct FmtDF64 {
         return make_float2(hi, lo);
     }
     static __device__ __forceinline__ cpx<double> ld(const void* p, uint64_t i) {
-        float4 v = reinterpret_cast<const float4*>(p)[i];
+        const vf4_t v = __builtin_nontemporal_load(reinterpret_cast<const vf4_t*>(p) + i);
+        return mk<double>(join(v.x, v.y), join(v.z, v.w));
+    }
+    static __device__ __forceinline__ cpx<double> ld_c(const void* p, uint64_t i) {
+        const vf4_t v = reinterpret_cast<const vf4_t*>(p)[i];
         return mk<double>(join(v.x, v.y), join(v.z, v.w));
     }
     static __device__ __forceinline__ void st(void* p, uint64_t i, cpx<double> v) {

@@ -80,6 +80,15 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    // f64 / df64 4096-point lines: the two passes of the 2^24 four-step
+    // (columns with twiddle on store; rows with transposed store), strided
+    // and contiguous axes; component-split LDS exchange at four lines
+    REGV(FmtF64, P_F64, 4096, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 256, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 256, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtDF64, P_DF64, 4096, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtDF64, P_DF64, 4096, 256, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     // Alternatives for the autotuner: other workgroup shapes and radix splits
     // of the hot lengths (the first entry of a shape above is the default).
     REGV(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x8", 8, 8, 8, 8),
@@ -93,6 +102,10 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 2048, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x8", 16, 16, 8),
     REGV(FmtF32, P_F32, 2048, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8x4", 8, 8, 8, 4),
     REGV(FmtF32, P_F32, 1024, 64, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
+    REGV(FmtF64, P_F64, 4096, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 256, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
+    REGV(FmtF64, P_F64, 4096, 128, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
 };
 
 int reg_variant_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }

@@ -1159,15 +1159,18 @@ std::string describe_plan(const Plan& p) {
 }
 
 // ------------------------------------------------------------- autotuner
-int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report) {
+int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int count, int reps, hipStream_t s,
+                      std::string* report) {
     if (reps < 1 || reps > 10000) fail(MIFFT_ERR_INVALID_TUNING, "autotune repetitions must be 1..10000");
-    std::unique_ptr<Plan> P = build_plan(cfg, kind);
-    // scratch buffers of the plan's sizes and a holder for candidate tables
+    if (count < 1 || count > 16) fail(MIFFT_ERR_INVALID_TUNING, "autotune takes 1..16 plans");
+    std::vector<std::unique_ptr<Plan>> plans;
+    for (int i = 0; i < count; i++) plans.push_back(build_plan(cfgs[i], kinds[i]));
+    // holder of candidate tables and scratch buffers
     Plan T;
-    T.cfg = P->cfg;
-    T.prec = P->prec;
-    T.arith = P->arith;
-    T.lds_budget = P->lds_budget;
+    T.cfg = plans[0]->cfg;
+    T.prec = plans[0]->prec;
+    T.arith = plans[0]->arith;
+    T.lds_budget = plans[0]->lds_budget;
     auto alloc = [&](size_t bytes) -> void* {
         DevBuf b;
         b.bytes = std::max<size_t>(bytes, 256);
@@ -1176,64 +1179,122 @@ int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, 
         hip_check(hipMemsetAsync(b.p, 0, b.bytes, s), "autotune buffer");
         return b.p;
     };
-    void* in = alloc(P->in_bytes);
-    void* out = alloc(P->out_bytes);
-    void* ws = alloc(P->ws_bytes);
+    // The sequence runs as a caller runs it: in place on one array when every
+    // plan allows it (a forward+inverse pair), otherwise each plan reading the
+    // previous plan's output (R2C then C2R).  Arrays rotate over sets larger
+    // than the 256 MB MALL so each execution streams from HBM; each plan has
+    // one workspace, as it executes with its own.
+    bool in_place = true;
+    size_t arr = 256;
+    for (auto& p : plans) {
+        in_place &= p->in_place_kernels;
+        arr = std::max({arr, (size_t)p->in_bytes, (size_t)p->out_bytes});
+    }
+    const int nsets = (int)std::min<size_t>(8, std::max<size_t>(1, ((size_t)512 << 20) / (2 * arr) + 1));
+    std::vector<void*> bufa, bufb, wsp;
+    for (int k = 0; k < nsets; k++) {
+        bufa.push_back(alloc(arr));
+        bufb.push_back(in_place ? nullptr : alloc(arr));
+    }
+    for (auto& p : plans) wsp.push_back(alloc(p->ws_bytes));
     Builder TB{T, s, {}};
+    std::vector<std::vector<Op>> ops;
+    for (auto& p : plans) ops.push_back(p->ops);
+    auto run_all = [&](int k) {
+        void* x = bufa[k % nsets];
+        void* y = in_place ? x : bufb[k % nsets];
+        for (size_t pi = 0; pi < ops.size(); pi++) {
+            for (const Op& o : ops[pi]) run_op(o, s, x, y, wsp[pi]);
+            if (!in_place) std::swap(x, y);
+        }
+    };
+    // distinct tunable op shapes, in launch order
+    std::vector<std::string> keys;
+    for (auto& p : plans)
+        for (const Op& o : p->ops)
+            if ((o.kind == OP_REG || o.kind == OP_LDS) && !o.tkey.empty() &&
+                std::find(keys.begin(), keys.end(), o.tkey) == keys.end())
+                keys.push_back(o.tkey);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hip_check(hipEventCreate(&e0), "autotune event");
     if (hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
         fail(MIFFT_ERR_HIP, "autotune event");
     }
+    // Coordinate descent over the shapes: every candidate of a shape is
+    // applied to all ops of that shape in every plan and the whole sequence is
+    // timed (so each pass sees the cache state its predecessor leaves: a pass
+    // reading an array its predecessor just wrote partly hits the MALL); the
+    // winner stays for the shapes tuned after it.
     int tuned = 0;
     try {
-        for (const Op& op : P->ops) {
-            if ((op.kind != OP_REG && op.kind != OP_LDS) || op.tkey.empty()) continue;
+        for (const std::string& key : keys) {
+            const Op* ref = nullptr;
+            for (auto& p : plans)
+                for (const Op& o : p->ops)
+                    if (!ref && o.tkey == key) ref = &o;
+            // candidate (name, op template: kind, rv or schedule, tables)
             std::vector<std::pair<std::string, Op>> cands;
-            for (const RegVariant* v : find_reg_variants(op.prec, (int)op.n, op.ld_lf, op.st_lf, op.ldm, op.stm, op.post)) {
-                if ((v->vec == 2 && !op.pairable) || v->lds_bytes > kLdsBudget) continue;
-                Op c = op;
+            for (const RegVariant* v :
+                 find_reg_variants(ref->prec, (int)ref->n, ref->ld_lf, ref->st_lf, ref->ldm, ref->stm, ref->post)) {
+                if ((v->vec == 2 && !ref->pairable) || v->lds_bytes > kLdsBudget) continue;
+                Op c = *ref;
                 c.kind = OP_REG;
                 c.rv = v;
-                c.a.tw = TB.stage_twiddles(P->arith, Builder::parse_radices(v->radices));
+                c.a.tw = TB.stage_twiddles(T.arith, Builder::parse_radices(v->radices));
                 cands.emplace_back(reg_variant_name(v), c);
             }
-            const uint64_t lstr = lds_line_stride(op.n);
-            const size_t es = elem_bytes(P->arith);
+            const uint64_t lstr = lds_line_stride(ref->n);
+            const size_t es = elem_bytes(ref->prec == P_F32 ? P_F32 : P_F64);
             for (uint32_t lpw = 1; lpw <= 32; lpw *= 2) {
-                if (2 * lstr * es * lpw > P->lds_budget || lpw > op.a.lines) break;
-                Op c = op;
+                if (2 * lstr * es * lpw > T.lds_budget || lpw > ref->a.lines) break;
+                Op c = *ref;
                 c.kind = OP_LDS;
                 c.rv = nullptr;
-                c.sc = make_sched(op.n, lpw, op.ld_lf, op.st_lf);
-                c.a.tw = TB.stage_twiddles(P->arith, Builder::sched_radices(c.sc));
+                c.sc = make_sched(ref->n, lpw, ref->ld_lf, ref->st_lf);
+                c.a.tw = TB.stage_twiddles(ref->prec == P_F32 ? P_F32 : P_F64, Builder::sched_radices(c.sc));
                 cands.emplace_back("lds:lpw=" + std::to_string(lpw), c);
             }
             if (cands.size() < 2) continue;
+            auto apply = [&](const Op& c) {
+                for (auto& pv : ops)
+                    for (Op& o : pv)
+                        if (o.tkey == key) {
+                            // keep each op's own buffers, lines and flags; take the kernel choice
+                            o.kind = c.kind;
+                            o.rv = c.rv;
+                            o.sc = c.sc;
+                            o.a.tw = c.a.tw;
+                        }
+            };
             hip_check(hipStreamSynchronize(s), "autotune tables");
-            std::string best;
-            float best_ms = 0.0f;
-            std::ostringstream line;
-            line << op.tkey << ":";
-            for (auto& c : cands) {
-                for (int w = 0; w < 2; w++) run_op(c.second, s, in, out, ws);
-                hip_check(hipEventRecord(e0, s), "autotune event");
-                for (int r = 0; r < reps; r++) run_op(c.second, s, in, out, ws);
-                hip_check(hipEventRecord(e1, s), "autotune event");
-                hip_check(hipEventSynchronize(e1), "autotune event");
-                float ms = 0.0f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "autotune event");
-                ms /= (float)reps;
-                line << " " << c.first << "=" << ms * 1e3f << "us";
-                if (best.empty() || ms < best_ms) {
-                    best = c.first;
-                    best_ms = ms;
+            // three trials with the candidates interleaved; a candidate's time
+            // is its best trial mean per execution of the sequence
+            std::vector<float> tmin(cands.size(), 1e30f);
+            for (int trial = 0; trial < 3; trial++) {
+                for (size_t ci = 0; ci < cands.size(); ci++) {
+                    apply(cands[ci].second);
+                    for (int w = 0; w < 2; w++) run_all(w);
+                    hip_check(hipEventRecord(e0, s), "autotune event");
+                    for (int r = 0; r < reps; r++) run_all(r);
+                    hip_check(hipEventRecord(e1, s), "autotune event");
+                    hip_check(hipEventSynchronize(e1), "autotune event");
+                    float ms = 0.0f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "autotune event");
+                    tmin[ci] = std::min(tmin[ci], ms / (float)reps);
                 }
             }
-            tuning_set(op.tkey, best);
+            size_t bi = 0;
+            std::ostringstream line;
+            line << key << ":";
+            for (size_t ci = 0; ci < cands.size(); ci++) {
+                line << " " << cands[ci].first << "=" << tmin[ci] * 1e3f << "us";
+                if (tmin[ci] < tmin[bi]) bi = ci;
+            }
+            apply(cands[bi].second);
+            tuning_set(key, cands[bi].first);
             tuned++;
-            if (report) *report += line.str() + " -> " + best + "\n";
+            if (report) *report += line.str() + " -> " + cands[bi].first + "\n";
         }
     } catch (...) {
         (void)hipStreamSynchronize(s);
@@ -1245,6 +1306,10 @@ int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return tuned;
+}
+
+int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report) {
+    return autotune_sequence(&cfg, &kind, 1, reps, s, report);
 }
 
 }  // namespace mifft

@@ -127,10 +127,16 @@ std::vector<std::pair<std::string, std::string>> tuning_entries();
 // Parses "key = choice" lines ('#' comments); throws on a malformed line or
 // an unknown variant.  Returns the number of entries read.
 int tuning_parse(const std::string& text);
-// Times every candidate of each line op of the plan of (cfg, kind) on
-// scratch buffers of the plan's sizes (reps launches each) and records the
-// fastest; returns the number of ops tuned and appends one line per op to
-// `report`.
+// Tunes the kernel choice of every line-op shape of the plans of
+// (cfgs[i], kinds[i]), executed back to back as a caller runs them (in place
+// on one array when all plans allow it, else each reading the previous
+// output; arrays rotated beyond the MALL, one workspace per plan): for each
+// shape in launch order, the whole sequence is timed with each candidate
+// applied to all ops of that shape, and the fastest is kept and recorded.
+// Returns the number of shapes tuned; appends one line per shape to `report`.
+int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int count, int reps, hipStream_t s,
+                      std::string* report);
+// autotune_sequence of one plan.
 int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report);
 std::string describe_plan(const Plan& p);
 
