@@ -36,27 +36,31 @@ __device__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t 
     cpx<T>* x0 = bufB + lpw * lstr;  // lpw entries
     cpx<T>* s0 = x0 + lpw;           // lpw entries
     const DivMagic dM = sc.dn;  // sc.n == M
-    // gather a[q] = x[g^q]
+    // gather a[q] = x[g^q] (zero-padded to M), x0 = x[0]; lines past the end
+    // read the last line and are never stored
     const uint32_t total = lpw * M;
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        uint32_t l, q;
-        if (sc.ld_lf) {
-            q = fdiv(e, sc.dlpw);
-            l = e - q * lpw;
-        } else {
-            l = fdiv(e, dM);
-            q = e - l * M;
+    const uint32_t tpl = slot_tpl(lpw), slots = tpl * lpw;
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.ld_lf);
+        const uint32_t g = g0 + m.l;
+        const LineBase b = line_base(a, g < a.lines ? g : a.lines - 1);
+        if (m.t == 0) x0[m.l] = load_point<Fmt, LD_COMPLEX>(a, b.in, 0, p);
+        for (uint32_t q0 = m.t; q0 < M; q0 += tpl * kChunk) {
+            uint32_t src[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t q = q0 + u * tpl;
+                src[u] = c.perm_in[q < p - 1 ? q : p - 2];
+            }
+            cpx<T> v[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) v[u] = load_point<Fmt, LD_COMPLEX>(a, b.in, src[u], p);
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t q = q0 + u * tpl;
+                if (q < M) bufA[m.l * lstr + lpad(q)] = q < p - 1 ? v[u] : mk<T>(T(0), T(0));
+            }
         }
-        const uint32_t g = g0 + l;
-        cpx<T> v = mk<T>(T(0), T(0));
-        if (g < a.lines) {
-            LineBase b = line_base(a, g);
-            if (q < p - 1) v = load_point<Fmt, LD_COMPLEX>(a, b.in, c.perm_in[q], p);
-            if (q == 0) x0[l] = load_point<Fmt, LD_COMPLEX>(a, b.in, 0, p);
-        } else if (q == 0) {
-            x0[l] = mk<T>(T(0), T(0));
-        }
-        bufA[l * lstr + lpad(q)] = v;
     }
     __syncthreads();
     cpx<T>* r = lds_fft_lines<T>(bufA, bufB, lpw, sc, a.tw);
@@ -71,29 +75,23 @@ __device__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t 
     __syncthreads();
     cpx<T>* r2 = lds_fft_lines<T>(r, o, lpw, sc, a.tw);
     // scatter X[g^-u] = x0 + c[u], X[0] = x0 + sum a
-    const uint32_t tot_out = lpw * p;
-    for (uint32_t e = threadIdx.x; e < tot_out; e += blockDim.x) {
-        uint32_t l, u;
-        if (sc.st_lf) {
-            u = fdiv(e, sc.dlpw);
-            l = e - u * lpw;
-        } else {
-            l = e / p;
-            u = e - l * p;
-        }
-        const uint32_t g = g0 + l;
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.st_lf);
+        const uint32_t g = g0 + m.l;
         if (g >= a.lines) continue;
-        LineBase b = line_base(a, g);
-        cpx<T> v;
-        uint32_t k;
-        if (u < p - 1) {
-            v = x0[l] + swap_ri(r2[l * lstr + lpad(u)]);
-            k = c.perm_out[u];
-        } else {
-            v = x0[l] + s0[l];
-            k = 0;
+        const LineBase b = line_base(a, g);
+        for (uint32_t u = m.t; u < p; u += tpl) {
+            cpx<T> v;
+            uint32_t k;
+            if (u < p - 1) {
+                v = x0[m.l] + swap_ri(r2[m.l * lstr + lpad(u)]);
+                k = c.perm_out[u];
+            } else {
+                v = x0[m.l] + s0[m.l];
+                k = 0;
+            }
+            store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, p, v);
         }
-        store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, p, v);
     }
 }
 
@@ -108,22 +106,26 @@ __device__ void bluestein_tile(const ConvArgs& c, unsigned char* smem_raw, uint3
     cpx<T>* bufB = bufA + lpw * lstr;
     const DivMagic dM = sc.dn;
     const uint32_t total = lpw * M;
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        uint32_t l, q;
-        if (sc.ld_lf) {
-            q = fdiv(e, sc.dlpw);
-            l = e - q * lpw;
-        } else {
-            l = fdiv(e, dM);
-            q = e - l * M;
+    const uint32_t tpl = slot_tpl(lpw), slots = tpl * lpw;
+    // gather x[q] c[q] (zero-padded to M); lines past the end read the last
+    // line and are never stored
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.ld_lf);
+        const uint32_t g = g0 + m.l;
+        const LineBase b = line_base(a, g < a.lines ? g : a.lines - 1);
+        for (uint32_t q0 = m.t; q0 < M; q0 += tpl * kChunk) {
+            cpx<T> v[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t q = q0 + u * tpl;
+                v[u] = load_point<Fmt, LD_COMPLEX>(a, b.in, q < n ? q : n - 1, n);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t q = q0 + u * tpl;
+                if (q < M) bufA[m.l * lstr + lpad(q)] = q < n ? cmul(v[u], ld_tw<T>(c.chirp, q)) : mk<T>(T(0), T(0));
+            }
         }
-        const uint32_t g = g0 + l;
-        cpx<T> v = mk<T>(T(0), T(0));
-        if (g < a.lines && q < n) {
-            LineBase b = line_base(a, g);
-            v = cmul(load_point<Fmt, LD_COMPLEX>(a, b.in, q, n), ld_tw<T>(c.chirp, q));
-        }
-        bufA[l * lstr + lpad(q)] = v;
     }
     __syncthreads();
     cpx<T>* r = lds_fft_lines<T>(bufA, bufB, lpw, sc, a.tw);
@@ -135,21 +137,15 @@ __device__ void bluestein_tile(const ConvArgs& c, unsigned char* smem_raw, uint3
     }
     __syncthreads();
     cpx<T>* r2 = lds_fft_lines<T>(r, o, lpw, sc, a.tw);
-    const uint32_t tot_out = lpw * n;
-    for (uint32_t e = threadIdx.x; e < tot_out; e += blockDim.x) {
-        uint32_t l, k;
-        if (sc.st_lf) {
-            k = fdiv(e, sc.dlpw);
-            l = e - k * lpw;
-        } else {
-            l = e / n;
-            k = e - l * n;
-        }
-        const uint32_t g = g0 + l;
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.st_lf);
+        const uint32_t g = g0 + m.l;
         if (g >= a.lines) continue;
-        LineBase b = line_base(a, g);
-        cpx<T> v = cmul(swap_ri(r2[l * lstr + lpad(k)]), ld_tw<T>(c.chirp, k));
-        store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, n, v);
+        const LineBase b = line_base(a, g);
+        for (uint32_t k = m.t; k < n; k += tpl) {
+            const cpx<T> v = cmul(swap_ri(r2[m.l * lstr + lpad(k)]), ld_tw<T>(c.chirp, k));
+            store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, k, n, v);
+        }
     }
 }
 

@@ -626,6 +626,25 @@ __device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched&
     return src;
 }
 
+// Moving `lpw` lines of `len` points between memory and LDS: slot s of
+// lpw * tpl slots (tpl = threads per line) owns line l and points
+// t, t + tpl, ...; with lines-fastest memory order adjacent slots take adjacent
+// lines (coalesced across lines), otherwise adjacent points of one line.  A
+// slot computes its line's base once and moves kChunk points per step with
+// the loads issued together (clamped indices, no branch per load), so they
+// are in flight at once instead of one memory latency per point.
+constexpr uint32_t kChunk = 8;
+struct LineSlot {
+    uint32_t l, t;
+};
+__device__ __forceinline__ uint32_t slot_tpl(uint32_t lpw) {
+    const uint32_t tpl = blockDim.x / lpw;
+    return tpl ? tpl : 1u;
+}
+__device__ __forceinline__ LineSlot line_slot(uint32_t s, uint32_t lpw, uint32_t tpl, bool lf) {
+    return lf ? LineSlot{s % lpw, s / lpw} : LineSlot{s / tpl, s % tpl};
+}
+
 // Lines [g0, g0 + sc.lpw) through LDS with the runtime schedule `sc`.
 template <class Fmt, int LDM, int STM, bool POST>
 __device__ void lds_fft_tile(const LineArgs& a, const Sched& sc, unsigned char* smem_raw,
@@ -634,42 +653,36 @@ __device__ void lds_fft_tile(const LineArgs& a, const Sched& sc, unsigned char* 
     const uint32_t lstr = lds_line_stride(sc.n);
     cpx<T>* bufA = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* bufB = bufA + sc.lpw * lstr;
-    const uint32_t total = sc.lpw * sc.n;
-    // load
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        uint32_t l, p;
-        if (sc.ld_lf) {
-            p = fdiv(e, sc.dlpw);
-            l = e - p * sc.lpw;
-        } else {
-            l = fdiv(e, sc.dn);
-            p = e - l * sc.n;
+    const uint32_t n = sc.n, lpw = sc.lpw, tpl = slot_tpl(lpw), slots = tpl * lpw;
+    // load (lines past the end read the last line; they are never stored)
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.ld_lf);
+        const uint32_t g = g0 + m.l;
+        const LineBase b = line_base(a, g < a.lines ? g : a.lines - 1);
+        for (uint32_t p0 = m.t; p0 < n; p0 += tpl * kChunk) {
+            cpx<T> v[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t p = p0 + u * tpl;
+                v[u] = load_point<Fmt, LDM>(a, b.in, p < n ? p : n - 1, n);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; u++) {
+                const uint32_t p = p0 + u * tpl;
+                if (p < n) bufA[m.l * lstr + lpad(p)] = v[u];
+            }
         }
-        const uint32_t g = g0 + l;
-        cpx<T> v = mk<T>(T(0), T(0));
-        if (g < a.lines) {
-            LineBase b = line_base(a, g);
-            v = load_point<Fmt, LDM>(a, b.in, p, sc.n);
-        }
-        bufA[l * lstr + lpad(p)] = v;
     }
     __syncthreads();
-    cpx<T>* res = lds_fft_lines<T>(bufA, bufB, sc.lpw, sc, a.tw);
+    cpx<T>* res = lds_fft_lines<T>(bufA, bufB, lpw, sc, a.tw);
     // store
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        uint32_t l, p;
-        if (sc.st_lf) {
-            p = fdiv(e, sc.dlpw);
-            l = e - p * sc.lpw;
-        } else {
-            l = fdiv(e, sc.dn);
-            p = e - l * sc.n;
-        }
-        const uint32_t g = g0 + l;
-        if (g < a.lines) {
-            LineBase b = line_base(a, g);
-            store_point<Fmt, STM, POST>(a, b.out, b.j, p, sc.n, res[l * lstr + lpad(p)]);
-        }
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const LineSlot m = line_slot(s, lpw, tpl, sc.st_lf);
+        const uint32_t g = g0 + m.l;
+        if (g >= a.lines) continue;
+        const LineBase b = line_base(a, g);
+        for (uint32_t p = m.t; p < n; p += tpl)
+            store_point<Fmt, STM, POST>(a, b.out, b.j, p, n, res[m.l * lstr + lpad(p)]);
     }
 }
 

@@ -18,31 +18,72 @@
 
 namespace mifft {
 
+// The tile claimed by the workgroup, broadcast through LDS.
+__shared__ uint32_t s_tile;
+
+__device__ __forceinline__ uint32_t claim_tile(uint32_t* counter) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();  // the next claim overwrites s_tile
+    return t;
+}
+
 // Every class is a separate (not inlined) function: inlined into one switch,
 // the register allocation of the kernel would be the worst path's, and the
-// whole persistent kernel would run at one wave per SIMD.
+// whole persistent kernel would run at one wave per SIMD.  A call boundary
+// waits for every outstanding memory operation, so a class function keeps
+// claiming tiles while they are of its own class (tiles are sorted by cost,
+// so a class comes in runs): the stores of one tile stay in flight while the
+// next tile loads.  It returns the first claimed tile of another class (or
+// past the end).
+template <class Body>
+__device__ __forceinline__ uint32_t tile_run(const BatchTile* __restrict__ tiles, uint32_t ntiles,
+                                             uint32_t* counter, const OpDesc* __restrict__ ops, uint32_t t,
+                                             Body&& body) {
+    t = __builtin_amdgcn_readfirstlane(t);
+    const uint32_t cls = tiles[t].cls;
+    for (;;) {
+        const BatchTile tl = tiles[t];
+        body(ops[tl.arg], tl.g0);
+        __syncthreads();  // the next tile reuses LDS
+        t = claim_tile(counter);
+        if (t >= ntiles || tiles[t].cls != cls) return t;
+    }
+}
+
+#define MIFFT_TILE_ARGS const BatchTile *__restrict__ tiles, uint32_t ntiles, uint32_t *counter, \
+                        const OpDesc *__restrict__ ops, unsigned char *smem, uint32_t t
+#define MIFFT_TILE_PASS tiles, ntiles, counter, ops, smem, t
+
 template <class K>
-__device__ __noinline__ void reg_tile(const LineArgs& a, unsigned char* smem, uint32_t g0) {
-    K::tile(a, reinterpret_cast<cpx<typename K::T>*>(smem), g0);
+__device__ __noinline__ uint32_t reg_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t, [&](const OpDesc& d, uint32_t g0) {
+        K::tile(d.a, reinterpret_cast<cpx<typename K::T>*>(smem), g0);
+    });
 }
 template <class Fmt, int LDM, int STM, bool POST>
-__device__ __noinline__ void lds_tile_nc(const LineArgs& a, const Sched& sc, unsigned char* smem, uint32_t g0) {
-    lds_fft_tile<Fmt, LDM, STM, POST>(a, sc, smem, g0);
+__device__ __noinline__ uint32_t lds_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t, [&](const OpDesc& d, uint32_t g0) {
+        lds_fft_tile<Fmt, LDM, STM, POST>(d.a, d.sc, smem, g0);
+    });
 }
 template <class Fmt>
-__device__ __noinline__ void rader_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
-    rader_tile<Fmt>(c, smem, g0);
+__device__ __noinline__ uint32_t rader_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t,
+                    [&](const OpDesc& d, uint32_t g0) { rader_tile<Fmt>(d.conv, smem, g0); });
 }
 template <class Fmt>
-__device__ __noinline__ void bluestein_tile_nc(const ConvArgs& c, unsigned char* smem, uint32_t g0) {
-    bluestein_tile<Fmt>(c, smem, g0);
+__device__ __noinline__ uint32_t bluestein_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t,
+                    [&](const OpDesc& d, uint32_t g0) { bluestein_tile<Fmt>(d.conv, smem, g0); });
 }
 
 // Element-wise step KIND (0 chirp pre, 1 filter, 2 chirp post) over elements
 // [g0, g0 + kEltTile) of the op's index space (lines x m, or lines x n for the
 // chirp post); the host keeps that space below 2^32.
 template <class Fmt, int KIND>
-__device__ __noinline__ void elt_tile(const ConvArgs& c, uint32_t g0) {
+__device__ __forceinline__ void elt_tile(const ConvArgs& c, uint32_t g0) {
     const uint32_t per = KIND == 2 ? c.n : c.m;
     const uint32_t total = c.io.lines * per;
     const uint32_t end = total - g0 < kEltTile ? total : g0 + kEltTile;
@@ -54,6 +95,11 @@ __device__ __noinline__ void elt_tile(const ConvArgs& c, uint32_t g0) {
         else chirp_post_elem<Fmt>(c.io, c.chirp, c.n, c.m, l, p);
     }
 }
+template <class Fmt, int KIND>
+__device__ __noinline__ uint32_t elt_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t,
+                    [&](const OpDesc& d, uint32_t g0) { elt_tile<Fmt, KIND>(d.conv, g0); });
+}
 
 #define MIFFT_BK(N, TPL, LINES, LDLF, STLF, POST, VEC, ...) \
     RegFft<FmtF32, N, TPL, LINES, LDLF, STLF, LD_COMPLEX, ST_COMPLEX, POST, RadixList<__VA_ARGS__>, VEC>
@@ -63,59 +109,51 @@ __device__ __noinline__ void elt_tile(const ConvArgs& c, uint32_t g0) {
 __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                uint32_t* counter, const OpDesc* __restrict__ ops) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ uint32_t s_tile;
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
-        __syncthreads();
-        if (t >= ntiles) break;  // uniform: every wave of the block leaves together
-        const BatchTile tl = tiles[t];
-        const OpDesc& d = ops[tl.arg];
-        switch (tl.cls) {
-            case BC_LDS: lds_tile_nc<FmtF32, LD_COMPLEX, ST_COMPLEX, false>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_POST: lds_tile_nc<FmtF32, LD_COMPLEX, ST_COMPLEX, true>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_R2C: lds_tile_nc<FmtF32, LD_REAL, ST_PACKED, false>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_C2R: lds_tile_nc<FmtF32, LD_HERMITIAN, ST_REAL, false>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_F64: lds_tile_nc<FmtF64, LD_COMPLEX, ST_COMPLEX, false>(d.a, d.sc, smem, tl.g0); break;
-            case BC_LDS_POST_F64: lds_tile_nc<FmtF64, LD_COMPLEX, ST_COMPLEX, true>(d.a, d.sc, smem, tl.g0); break;
-            case BC_RADER: rader_tile_nc<FmtF32>(d.conv, smem, tl.g0); break;
-            case BC_BLUESTEIN: bluestein_tile_nc<FmtF32>(d.conv, smem, tl.g0); break;
-            case BC_RADER_F64: rader_tile_nc<FmtF64>(d.conv, smem, tl.g0); break;
-            case BC_BLUESTEIN_F64: bluestein_tile_nc<FmtF64>(d.conv, smem, tl.g0); break;
+    uint32_t t = claim_tile(counter);
+    while (t < ntiles) {  // uniform: every wave of the block leaves together
+        switch (tiles[t].cls) {
+            case BC_LDS: t = lds_tiles<FmtF32, LD_COMPLEX, ST_COMPLEX, false>(MIFFT_TILE_PASS); break;
+            case BC_LDS_POST: t = lds_tiles<FmtF32, LD_COMPLEX, ST_COMPLEX, true>(MIFFT_TILE_PASS); break;
+            case BC_LDS_R2C: t = lds_tiles<FmtF32, LD_REAL, ST_PACKED, false>(MIFFT_TILE_PASS); break;
+            case BC_LDS_C2R: t = lds_tiles<FmtF32, LD_HERMITIAN, ST_REAL, false>(MIFFT_TILE_PASS); break;
+            case BC_LDS_F64: t = lds_tiles<FmtF64, LD_COMPLEX, ST_COMPLEX, false>(MIFFT_TILE_PASS); break;
+            case BC_LDS_POST_F64: t = lds_tiles<FmtF64, LD_COMPLEX, ST_COMPLEX, true>(MIFFT_TILE_PASS); break;
+            case BC_RADER: t = rader_tiles<FmtF32>(MIFFT_TILE_PASS); break;
+            case BC_BLUESTEIN: t = bluestein_tiles<FmtF32>(MIFFT_TILE_PASS); break;
+            case BC_RADER_F64: t = rader_tiles<FmtF64>(MIFFT_TILE_PASS); break;
+            case BC_BLUESTEIN_F64: t = bluestein_tiles<FmtF64>(MIFFT_TILE_PASS); break;
             case BC_REG_C256_F64:
-                reg_tile<RegFft<FmtF64, 256, 16, 16, false, false, LD_COMPLEX, ST_COMPLEX, false, RadixList<16, 16>, 1>>(
-                    d.a, smem, tl.g0);
+                t = reg_tiles<RegFft<FmtF64, 256, 16, 16, false, false, LD_COMPLEX, ST_COMPLEX, false,
+                                     RadixList<16, 16>, 1>>(MIFFT_TILE_PASS);
                 break;
             case BC_REG_S256_F64:
-                reg_tile<RegFft<FmtF64, 256, 16, 16, true, true, LD_COMPLEX, ST_COMPLEX, false, RadixList<16, 16>, 1>>(
-                    d.a, smem, tl.g0);
+                t = reg_tiles<RegFft<FmtF64, 256, 16, 16, true, true, LD_COMPLEX, ST_COMPLEX, false,
+                                     RadixList<16, 16>, 1>>(MIFFT_TILE_PASS);
                 break;
-            case BC_CHIRP_PRE: elt_tile<FmtF32, 0>(d.conv, tl.g0); break;
-            case BC_CHIRP_PRE_F64: elt_tile<FmtF64, 0>(d.conv, tl.g0); break;
-            case BC_POINTWISE: elt_tile<FmtF32, 1>(d.conv, tl.g0); break;
-            case BC_POINTWISE_F64: elt_tile<FmtF64, 1>(d.conv, tl.g0); break;
-            case BC_CHIRP_POST: elt_tile<FmtF32, 2>(d.conv, tl.g0); break;
-            case BC_CHIRP_POST_F64: elt_tile<FmtF64, 2>(d.conv, tl.g0); break;
+            case BC_CHIRP_PRE: t = elt_tiles<FmtF32, 0>(MIFFT_TILE_PASS); break;
+            case BC_CHIRP_PRE_F64: t = elt_tiles<FmtF64, 0>(MIFFT_TILE_PASS); break;
+            case BC_POINTWISE: t = elt_tiles<FmtF32, 1>(MIFFT_TILE_PASS); break;
+            case BC_POINTWISE_F64: t = elt_tiles<FmtF64, 1>(MIFFT_TILE_PASS); break;
+            case BC_CHIRP_POST: t = elt_tiles<FmtF32, 2>(MIFFT_TILE_PASS); break;
+            case BC_CHIRP_POST_F64: t = elt_tiles<FmtF64, 2>(MIFFT_TILE_PASS); break;
             // register-resident variants with 256 threads (the registry entries of the same shape)
-            case BC_REG_C4096: reg_tile<MIFFT_BK(4096, 256, 1, 0, 0, 0, 1, 16, 16, 16)>(d.a, smem, tl.g0); break;
-            case BC_REG_C2048: reg_tile<MIFFT_BK(2048, 128, 2, 0, 0, 0, 1, 16, 16, 8)>(d.a, smem, tl.g0); break;
-            case BC_REG_C1024: reg_tile<MIFFT_BK(1024, 64, 4, 0, 0, 0, 1, 16, 16, 4)>(d.a, smem, tl.g0); break;
-            case BC_REG_C512: reg_tile<MIFFT_BK(512, 32, 8, 0, 0, 0, 1, 8, 8, 8)>(d.a, smem, tl.g0); break;
-            case BC_REG_C256: reg_tile<MIFFT_BK(256, 16, 16, 0, 0, 0, 1, 16, 16)>(d.a, smem, tl.g0); break;
-            case BC_REG_C128: reg_tile<MIFFT_BK(128, 8, 32, 0, 0, 0, 1, 16, 8)>(d.a, smem, tl.g0); break;
-            case BC_REG_C64: reg_tile<MIFFT_BK(64, 4, 64, 0, 0, 0, 1, 16, 4)>(d.a, smem, tl.g0); break;
-            case BC_REG_C120: reg_tile<MIFFT_BK(120, 8, 32, 0, 0, 0, 1, 15, 8)>(d.a, smem, tl.g0); break;
-            case BC_REG_C343: reg_tile<MIFFT_BK(343, 64, 4, 0, 0, 0, 1, 7, 7, 7)>(d.a, smem, tl.g0); break;
-            case BC_REG_C1001: reg_tile<MIFFT_BK(1001, 128, 2, 0, 0, 0, 1, 13, 11, 7)>(d.a, smem, tl.g0); break;
-            case BC_REG_P256: reg_tile<MIFFT_BK(256, 16, 32, 1, 1, 1, 2, 16, 16)>(d.a, smem, tl.g0); break;
-            case BC_REG_P256V1: reg_tile<MIFFT_BK(256, 16, 16, 1, 1, 1, 1, 16, 16)>(d.a, smem, tl.g0); break;
-            case BC_REG_T256: reg_tile<MIFFT_BK(256, 16, 16, 0, 1, 0, 1, 16, 16)>(d.a, smem, tl.g0); break;
-            case BC_REG_P128: reg_tile<MIFFT_BK(128, 8, 64, 1, 1, 1, 2, 16, 8)>(d.a, smem, tl.g0); break;
-            case BC_REG_T128: reg_tile<MIFFT_BK(128, 8, 32, 0, 1, 0, 1, 16, 8)>(d.a, smem, tl.g0); break;
-            default: break;
+            case BC_REG_C4096: t = reg_tiles<MIFFT_BK(4096, 256, 1, 0, 0, 0, 1, 16, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C2048: t = reg_tiles<MIFFT_BK(2048, 128, 2, 0, 0, 0, 1, 16, 16, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C1024: t = reg_tiles<MIFFT_BK(1024, 64, 4, 0, 0, 0, 1, 16, 16, 4)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C512: t = reg_tiles<MIFFT_BK(512, 32, 8, 0, 0, 0, 1, 8, 8, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C256: t = reg_tiles<MIFFT_BK(256, 16, 16, 0, 0, 0, 1, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C128: t = reg_tiles<MIFFT_BK(128, 8, 32, 0, 0, 0, 1, 16, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C64: t = reg_tiles<MIFFT_BK(64, 4, 64, 0, 0, 0, 1, 16, 4)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C120: t = reg_tiles<MIFFT_BK(120, 8, 32, 0, 0, 0, 1, 15, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C343: t = reg_tiles<MIFFT_BK(343, 64, 4, 0, 0, 0, 1, 7, 7, 7)>(MIFFT_TILE_PASS); break;
+            case BC_REG_C1001: t = reg_tiles<MIFFT_BK(1001, 128, 2, 0, 0, 0, 1, 13, 11, 7)>(MIFFT_TILE_PASS); break;
+            case BC_REG_P256: t = reg_tiles<MIFFT_BK(256, 16, 32, 1, 1, 1, 2, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_P256V1: t = reg_tiles<MIFFT_BK(256, 16, 16, 1, 1, 1, 1, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_T256: t = reg_tiles<MIFFT_BK(256, 16, 16, 0, 1, 0, 1, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_P128: t = reg_tiles<MIFFT_BK(128, 8, 64, 1, 1, 1, 2, 16, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_T128: t = reg_tiles<MIFFT_BK(128, 8, 32, 0, 1, 0, 1, 16, 8)>(MIFFT_TILE_PASS); break;
+            default: t = claim_tile(counter); break;  // unknown class: skip the tile
         }
-        __syncthreads();  // the next tile reuses LDS
     }
 }
 
