@@ -45,7 +45,13 @@ CASES = [
         (reg(2048, 256, 8, "11", 2, "8x8x8x4"), "8x8x8x4"),
         ("lds:lpw=16", "lds N=2048")]),
     ([32, 1024], 1, key(1024, "11", True, 32), [
-        (reg(1024, 64, 16, "11", 2, "16x16x4"), "vec=2 lpw=16")]),
+        (reg(1024, 64, 16, "11", 2, "16x16x4"), "vec=2 lpw=16"),
+        (reg(1024, 64, 8, "11", 2, "16x16x4"), "vec=2 lpw=8")]),
+    # strided line pairs in 256-thread workgroups (also the batched kernel's classes)
+    ([32, 512], 1, key(512, "11", True, 32), [(reg(512, 32, 16, "11", 2, "8x8x8"), "vec=2 lpw=16")]),
+    ([64, 256], 1, key(256, "11", True, 64), [(reg(256, 16, 32, "11", 2, "16x16"), "vec=2 lpw=32")]),
+    ([128, 128], 1, key(128, "11", True, 128), [(reg(128, 8, 64, "11", 2, "16x8"), "vec=2 lpw=64")]),
+    ([256, 64], 1, key(64, "11", True, 256), [(reg(64, 4, 128, "11", 2, "16x4"), "vec=2 lpw=128")]),
 ]
 
 

@@ -1,7 +1,7 @@
 // Host planner of the persistent batched launch (mifft_batch_*).
 //
 // Each transform reuses the planner of standalone plans in batched mode
-// (256-thread kernels, a 40 KiB LDS budget so that four workgroups share a
+// (256-thread kernels, an 80 KiB LDS budget so that at least two workgroups share a
 // CU).  Plans are cached, so the twiddle, chirp and filter tables of equal
 // shapes are shared.  Op k of every transform becomes a run of tiles in phase
 // k; tiles are sorted by cost (longest first) inside their phase and pulled by

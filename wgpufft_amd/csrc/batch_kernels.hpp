@@ -43,6 +43,12 @@ enum BatchClass : uint32_t {
     BC_POINTWISE_F64,
     BC_CHIRP_POST,
     BC_CHIRP_POST_F64,
+    // strided line pairs (axes of ND arrays), 256 threads
+    BC_REG_S1024V2,
+    BC_REG_S512V2,
+    BC_REG_S256V2,
+    BC_REG_S128V2,
+    BC_REG_S64V2,
     BC_COUNT
 };
 

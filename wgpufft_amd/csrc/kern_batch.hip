@@ -104,8 +104,8 @@ __device__ __noinline__ uint32_t elt_tiles(MIFFT_TILE_ARGS) {
 #define MIFFT_BK(N, TPL, LINES, LDLF, STLF, POST, VEC, ...) \
     RegFft<FmtF32, N, TPL, LINES, LDLF, STLF, LD_COMPLEX, ST_COMPLEX, POST, RadixList<__VA_ARGS__>, VEC>
 
-// Four workgroups per CU (the LDS budget of batched plans is 40 KiB): at most
-// 128 VGPRs per lane.
+// Up to four workgroups per CU (a launch's LDS is the largest of its tiles,
+// at most 80 KiB: two per CU then): at most 128 VGPRs per lane.
 __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                uint32_t* counter, const OpDesc* __restrict__ ops) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -152,6 +152,11 @@ __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ 
             case BC_REG_T256: t = reg_tiles<MIFFT_BK(256, 16, 16, 0, 1, 0, 1, 16, 16)>(MIFFT_TILE_PASS); break;
             case BC_REG_P128: t = reg_tiles<MIFFT_BK(128, 8, 64, 1, 1, 1, 2, 16, 8)>(MIFFT_TILE_PASS); break;
             case BC_REG_T128: t = reg_tiles<MIFFT_BK(128, 8, 32, 0, 1, 0, 1, 16, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_S1024V2: t = reg_tiles<MIFFT_BK(1024, 64, 8, 1, 1, 0, 2, 16, 16, 4)>(MIFFT_TILE_PASS); break;
+            case BC_REG_S512V2: t = reg_tiles<MIFFT_BK(512, 32, 16, 1, 1, 0, 2, 8, 8, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_S256V2: t = reg_tiles<MIFFT_BK(256, 16, 32, 1, 1, 0, 2, 16, 16)>(MIFFT_TILE_PASS); break;
+            case BC_REG_S128V2: t = reg_tiles<MIFFT_BK(128, 8, 64, 1, 1, 0, 2, 16, 8)>(MIFFT_TILE_PASS); break;
+            case BC_REG_S64V2: t = reg_tiles<MIFFT_BK(64, 4, 128, 1, 1, 0, 2, 16, 4)>(MIFFT_TILE_PASS); break;
             default: t = claim_tile(counter); break;  // unknown class: skip the tile
         }
     }
@@ -212,6 +217,11 @@ int batch_class_for(const RegVariant* v) {
     switch (key) {
         case 256111: return BC_REG_P256;
         case 128111: return BC_REG_P128;
+        case 1024110: return BC_REG_S1024V2;
+        case 512110: return BC_REG_S512V2;
+        case 256110: return BC_REG_S256V2;
+        case 128110: return BC_REG_S128V2;
+        case 64110: return BC_REG_S64V2;
         default: return -1;
     }
 }

@@ -102,6 +102,12 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 2048, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x8", 16, 16, 8),
     REGV(FmtF32, P_F32, 2048, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8x4", 8, 8, 8, 4),
     REGV(FmtF32, P_F32, 1024, 64, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
+    // strided line pairs in 256-thread workgroups (the batched kernel's shape)
+    REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 512, 32, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 64, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 64, 4, 128, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x4", 16, 4),
     REGV(FmtF64, P_F64, 4096, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 256, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),

@@ -381,6 +381,39 @@ static bool adjacent_lines(const LineSet& ls, const Layout& l) {
     return ls.S2 > 1 && l.sj == 1;
 }
 
+// Estimated cost per point of one four-step pass over lines of length n:
+// register-resident kernels stream at close to the copy rate; runtime-
+// schedule LDS lines run well when many fit a workgroup (short lines) and
+// poorly when only a few do (long f64 lines: narrow strided row segments).
+static double pass_cost(int prec, uint64_t n) {
+    if (n <= (1u << 30) && !find_reg_variants(prec, (int)n, 1, 1, LD_COMPLEX, ST_COMPLEX, 1).empty()) return 1.0;
+    const size_t es = prec == P_F32 ? 8 : 16;
+    const size_t line = 2 * (size_t)lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30)) * es;
+    const size_t per_wg = kLdsBudget / std::max<size_t>(line, 1);
+    return per_wg >= 8 ? 1.5 : per_wg >= 4 ? 2.5 : 6.0;
+}
+
+// Best four-step split n = n1 * n2 (both smooth, both on chip under
+// `budget`) by estimated cost, ties broken toward balance; 0 if none.
+static uint64_t best_split(int prec, uint64_t n, size_t budget, double* cost_out) {
+    uint64_t best1 = 0;
+    double best = 1e30;
+    const size_t es = prec == P_F32 ? 8 : 16;
+    for (uint64_t n1 = 2; n1 * n1 <= n * 16 && n1 < n; n1++) {
+        if (n % n1) continue;
+        const uint64_t n2 = n / n1;
+        auto fits = [&](uint64_t len) { return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= budget; };
+        if (!fits(n1) || !fits(n2)) continue;
+        const double c = pass_cost(prec, n1) + pass_cost(prec, n2) + 0.05 * std::fabs(std::log((double)n1 / (double)n2));
+        if (c < best) {
+            best = c;
+            best1 = n1;
+        }
+    }
+    if (cost_out) *cost_out = best;
+    return best1;
+}
+
 struct Builder {
     Plan& P;
     hipStream_t s;
@@ -494,7 +527,16 @@ struct Builder {
             forced_lpw = (uint32_t)std::strtoul(choice.c_str() + 8, nullptr, 10);
             tuned = forced_lpw > 0;
         }
-        if (!tuned) {
+        if (!tuned && P.batch_mode) {
+            // batched plans: the first registered variant with a batched class
+            // (line pairs first)
+            for (int want_vec = pairable ? 2 : 1; want_vec >= 1 && !rv; want_vec--)
+                for (const RegVariant* v : find_reg_variants(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post))
+                    if (v->vec == want_vec && batch_class_for(v) >= 0) {
+                        rv = v;
+                        break;
+                    }
+        } else if (!tuned) {
             rv = pairable ? find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 2) : nullptr;
             if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
         }
@@ -660,6 +702,26 @@ struct Builder {
         P.ops.push_back(op);
     }
 
+    // Convolution length of a multi-pass Bluestein axis: any smooth M >= 2N-1
+    // serves (bluestein_axis.rs:1035-1041 takes the smallest); among the smooth
+    // lengths up to 6 % above the smallest, the one whose padded transform
+    // splits into the cheapest passes (for example 4096 x 520 with a
+    // register-resident side instead of 1440 x 1458 through LDS).
+    uint64_t multipass_conv_len(uint64_t n, uint64_t m0) const {
+        uint64_t best_m = m0;
+        double best = 1e30;
+        for (uint64_t m = m0; m <= m0 + m0 / 16; m = next_smooth(m + 1)) {
+            double c = 0;
+            if (!best_split(P.arith, m, P.lds_budget, &c)) continue;
+            const double total = c * (double)m;
+            if (total < best) {
+                best = total;
+                best_m = m;
+            }
+        }
+        return best_m;
+    }
+
     // Multi-pass Bluestein for convolutions that do not fit LDS; returns the
     // workspace bytes used from ws_off.
     uint64_t bluestein_multi(uint64_t n, uint64_t m, const LineSet& ls, const Layout& in,
@@ -749,23 +811,9 @@ struct Builder {
     bool four_step(uint64_t n, uint64_t S, uint64_t O, int src, uint64_t src_off, int dst,
                    uint64_t dst_off, uint64_t ws_off, bool conj_in, bool conj_out, double scale,
                    uint64_t* ws_used) {
-        // Prefer a balanced split with register variants on both sides.
-        uint64_t best1 = 0;
-        double best_cost = 1e30;
-        for (uint64_t n1 = 2; n1 * n1 <= n * 16 && n1 < n; n1++) {
-            if (n % n1) continue;
-            const uint64_t n2 = n / n1;
-            const size_t es = elem_bytes(P.arith);
-            auto fits = [&](uint64_t len) {
-                return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= P.lds_budget;
-            };
-            if (!fits(n1) || !fits(n2)) continue;
-            const double cost = std::fabs(std::log((double)n1 / (double)n2));
-            if (cost < best_cost) {
-                best_cost = cost;
-                best1 = n1;
-            }
-        }
+        // the split with the cheapest estimated passes (register variants
+        // where possible), ties broken toward balance
+        const uint64_t best1 = best_split(P.arith, n, P.lds_budget, nullptr);
         if (!best1) return false;
         const uint64_t n1 = best1, n2 = n / n1;
         const uint64_t lines_total = S * O;
@@ -828,8 +876,8 @@ struct Builder {
             bluestein_fused_op(n, m, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale);
             return 0;
         }
-        return bluestein_multi(n, m, ls, lay, lay, src, src_off, dst, dst_off, ws_off, inverse,
-                               inverse, scale);
+        return bluestein_multi(n, multipass_conv_len(n, m), ls, lay, lay, src, src_off, dst, dst_off, ws_off,
+                               inverse, inverse, scale);
     }
 };
 

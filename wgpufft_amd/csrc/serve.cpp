@@ -44,6 +44,8 @@ PlanKey key_of(const mifft_server_request& r) {
 struct Slot {
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr;
+    hipStream_t s2 = nullptr;   // the tick's requests outside the batched launch
+    hipEvent_t ev2 = nullptr;
     bool busy = false;
     std::vector<int> reqs;
     uint64_t mem = 0;
@@ -75,6 +77,8 @@ struct mifft_server {
     ~mifft_server() {
         for (Slot& s : slots) {
             if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.ev2) (void)hipEventDestroy(s.ev2);
+            if (s.s2) (void)hipStreamDestroy(s.s2);
             if (s.s) (void)hipStreamDestroy(s.s);
             if (s.ws) (void)hipFree(s.ws);
             if (s.desc) (void)hipFree(s.desc);
@@ -140,6 +144,8 @@ static void prepare(mifft_server& S, const mifft_server_request* reqs, int32_t n
         for (Slot& sl : S.slots) {
             hip_check(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking), "server stream");
             hip_check(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming), "server event");
+            hip_check(hipStreamCreateWithFlags(&sl.s2, hipStreamNonBlocking), "server stream");
+            hip_check(hipEventCreateWithFlags(&sl.ev2, hipEventDisableTiming), "server event");
             sl.ws_cap = std::max(ws_cap, max_plan_ws);
             hip_check(hipMalloc(&sl.ws, sl.ws_cap), "server workspace");
             sl.desc_cap = desc_cap;
@@ -190,7 +196,14 @@ static void launch_tick(mifft_server& S, Slot& sl, const mifft_server_request* r
         batch_enqueue(d_tiles, counts, (const OpDesc*)sl.desc, sl.counters, batch_grid(b.lds_bytes), b.lds_bytes,
                       sl.s);
     }
-    for (int r : plan_reqs) run_plan(standalone(S, reqs[r]), sl.s, reqs[r].in, reqs[r].out, plan_ws);
+    // Requests outside the batched launch run concurrently with it on the
+    // slot's second stream (own workspace region); the tick completes when
+    // both streams have.
+    if (!plan_reqs.empty()) {
+        for (int r : plan_reqs) run_plan(standalone(S, reqs[r]), sl.s2, reqs[r].in, reqs[r].out, plan_ws);
+        hip_check(hipEventRecord(sl.ev2, sl.s2), "tick event");
+        hip_check(hipStreamWaitEvent(sl.s, sl.ev2, 0), "tick join");
+    }
     hip_check(hipEventRecord(sl.ev, sl.s), "tick event");
 }
 
