@@ -47,6 +47,12 @@ CASES = [
     ([32, 1024], 1, key(1024, "11", True, 32), [
         (reg(1024, 64, 16, "11", 2, "16x16x4"), "vec=2 lpw=16"),
         (reg(1024, 64, 8, "11", 2, "16x16x4"), "vec=2 lpw=8")]),
+    # persistent pipelined forms, more tiles than workgroups (each loops)
+    ([4096], 1200, key(4096, "00", False, 1200), [
+        (reg(4096, 256, 1, "00", 1, "16x16x16") + ":pipe2", "tpl=256"),
+        (reg(4096, 512, 1, "00", 1, "8x8x8x8") + ":pipe1", "8x8x8x8")]),
+    ([4400, 2048], 1, key(2048, "11", True, 4400), [
+        (reg(2048, 256, 4, "11", 2, "8x8x8x4") + ":pipe2", "vec=2 lpw=4")]),
     # strided line pairs in 256-thread workgroups (also the batched kernel's classes)
     ([32, 512], 1, key(512, "11", True, 32), [(reg(512, 32, 16, "11", 2, "8x8x8"), "vec=2 lpw=16")]),
     ([64, 256], 1, key(256, "11", True, 64), [(reg(256, 16, 32, "11", 2, "16x16"), "vec=2 lpw=32")]),

@@ -5,6 +5,7 @@
 // typed key at plan time, every register-resident variant is instantiated here
 // at build time and looked up by (precision, N, layout, load/store mode).
 // Lengths without a variant run the runtime-schedule LDS kernel.
+#include <algorithm>
 #include <mutex>
 #include <unordered_map>
 
@@ -44,6 +45,29 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
      (const void*)&reg_fft_kernel<MIFFT_K(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC,  \
                                           __VA_ARGS__)>,                                        \
      RADSTR}
+
+// Persistent pipelined form (twiddles held in registers, see
+// RegFft::run_pipelined), MINB workgroups per CU.
+#define MIFFT_KP(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, ...) \
+    RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__), VEC, true>
+#define REGP(FMT, PREC, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, MINB, RADSTR, ...)         \
+    {PREC,                                                                                      \
+     N,                                                                                         \
+     TPL,                                                                                       \
+     LINES,                                                                                     \
+     LDLF,                                                                                      \
+     STLF,                                                                                      \
+     LDM,                                                                                       \
+     STM,                                                                                       \
+     POST,                                                                                      \
+     VEC,                                                                                       \
+     MIFFT_KP(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, __VA_ARGS__)::NT,            \
+     MIFFT_KP(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, __VA_ARGS__)::lds_bytes(),   \
+     (const void*)&reg_fft_pipe_kernel<MIFFT_KP(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, \
+                                                VEC, __VA_ARGS__),                              \
+                                       MINB>,                                                   \
+     RADSTR,                                                                                    \
+     MINB}
 
 // f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
 #define F32_POW2(LDLF, STLF, LDM, STM, POST, VEC, L4096, L2048, L1024, L512, L256, L128, L64)          \
@@ -108,6 +132,10 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16", 16, 16),
     REGV(FmtF32, P_F32, 128, 8, 64, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x8", 16, 8),
     REGV(FmtF32, P_F32, 64, 4, 128, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x4", 16, 4),
+    // persistent pipelined forms of the 4096x2048 passes
+    REGP(FmtF32, P_F32, 4096, 256, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x16", 16, 16, 16),
+    REGP(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, "8x8x8x8", 8, 8, 8, 8),
+    REGP(FmtF32, P_F32, 2048, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 2, "8x8x8x4", 8, 8, 8, 4),
     REGV(FmtF64, P_F64, 4096, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 256, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
@@ -134,7 +162,9 @@ std::string reg_variant_name(const RegVariant* v) {
     char b[160];
     snprintf(b, sizeof(b), "reg:%s:n%d:t%d:l%d:lf%d%d:m%d/%d:p%d:v%d:%s", prec[v->prec], v->n, v->tpl, v->lines,
              v->ld_lf, v->st_lf, v->ldm, v->stm, v->post, v->vec, v->radices);
-    return b;
+    std::string name = b;
+    if (v->persist) name += ":pipe" + std::to_string(v->persist);
+    return name;
 }
 
 const RegVariant* find_reg_variant_named(const std::string& name) {
@@ -153,10 +183,24 @@ const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ld
     return nullptr;
 }
 
+static uint32_t cu_count() {
+    static uint32_t cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = (uint32_t)n;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
 hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s) {
     hipError_t e = ensure_lds_attr(v->fn, v->lds_bytes);
     if (e != hipSuccess) return e;
-    const uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
+    uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
+    if (v->persist) blocks = std::min<uint32_t>(blocks, cu_count() * (uint32_t)v->persist);
     LineArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchKernel(v->fn, dim3(blocks), dim3(v->nthreads), args, v->lds_bytes, s);

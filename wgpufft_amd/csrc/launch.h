@@ -28,6 +28,10 @@ struct RegVariant {
     size_t lds_bytes;
     const void* fn;
     const char* radices;  // e.g. "16x16x16"
+    // 0: one tile per workgroup.  k > 0: persistent pipelined form (k
+    // workgroups per CU, each loading its next tile before transforming the
+    // current one).
+    int persist = 0;
 };
 
 // Registry lookup; nullptr when no compiled variant matches.

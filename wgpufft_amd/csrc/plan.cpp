@@ -557,6 +557,7 @@ struct Builder {
             d << "reg N=" << n << " " << rv->radices << " lines=" << ls.lines << " tpl=" << rv->tpl
               << " vec=" << rv->vec
               << " lpw=" << rv->lines;
+            if (rv->persist) d << " pipe" << rv->persist;
         } else {
             op.kind = OP_LDS;
             uint32_t fit = (uint32_t)(P.lds_budget / (2 * lstr * es));
