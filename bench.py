@@ -499,7 +499,7 @@ def profiled_traffic(desc, prec="f32"):
             if not line.startswith("traffic_bytes_per_launch:"):
                 continue
             if (f" {prec} N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line
-                    and (" post" in line) == (" post" in desc)):
+                    and (" post" in line) == (" post" in desc) and ("pipe" in line) == ("pipe" in desc)):
                 return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
     return None, None
 
