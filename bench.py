@@ -70,8 +70,10 @@ def workload_spec(name):
                     flops_per_step=5 * n * 10 * b,
                     label="1D C2C f32 N=1024 batch=4096 forward")
     if name in ("c4a_f64", "c4a_df64", "c4b_f64"):
-        # 3D 256^3 (three axis passes) or 1D 2^24 (four-step, two passes); one
-        # complex element is 16 bytes in both f64 and df64 [re_hi,re_lo,im_hi,im_lo]
+        # 3D 256^3 (three axis passes) or 1D 2^24 (counted as two read+write
+        # sweeps of the array per FFT, the traffic of a two-pass four-step,
+        # whichever split the plan uses); one complex element is 16 bytes in
+        # both f64 and df64 [re_hi,re_lo,im_hi,im_lo]
         shape = [256, 256, 256] if name.startswith("c4a") else [1 << 24]
         n_elems = 1 << 24
         passes = 3 if name.startswith("c4a") else 2
@@ -79,7 +81,7 @@ def workload_spec(name):
         return dict(shape=shape, batch=1, kind="c2c_pair", elems=n_elems, precision=prec, elem_bytes=16,
                     bytes_per_step=2 * passes * 16 * n_elems * 2,  # 2*P*S per FFT, two FFTs
                     flops_per_step=2 * 5 * n_elems * 24,
-                    label=f"{'3D C2C [256,256,256]' if passes == 3 else '1D C2C [2^24] four-step'} {prec} "
+                    label=f"{'3D C2C [256,256,256]' if passes == 3 else '1D C2C [2^24]'} {prec} "
                           "forward+inverse in place")
     if name == "r2c":
         n = 2048
@@ -494,11 +496,13 @@ def profiled_traffic(desc, prec="f32"):
     if not m:
         return None, None
     n, rad, vec = m.groups()
+    lf = "lf=" + ("t" if "ldLF" in desc else "f") + ("t" if "stLF" in desc else "f")
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             if not line.startswith("traffic_bytes_per_launch:"):
                 continue
             if (f" {prec} N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line
+                    and f" {lf} " in line
                     and (" post" in line) == (" post" in desc) and ("pipe" in line) == ("pipe" in desc)):
                 return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
     return None, None

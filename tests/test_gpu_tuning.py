@@ -143,3 +143,21 @@ def test_autotune_sequence_pair_and_real_chain(torch_cuda):
     c2r.execute(pk, back)
     torch.cuda.synchronize()
     assert oracle.rel_l2(from_dev(back, np.float32), xr) <= 1e-6
+
+
+# Long 1D axes: the two-pass (n1 x n2) and three-pass (n3 x n1 x n2) splits,
+# forced by the table's split entries, against the oracle in both directions.
+@pytest.mark.parametrize("n,batch", [(1 << 18, 3), (1 << 19, 2), (1 << 20, 1)])
+@pytest.mark.parametrize("split", ["2", "3"])
+def test_forced_split_matches_oracle(torch_cuda, n, batch, split):
+    x = seeded_complex(n + int(split), n * batch)
+    W.parse_tuning(f"split|f32|n{n} = {split}\n")
+    for inverse in (False, True):
+        cfg = (FftConfig.inverse_nd([n]) if inverse else FftConfig.new_nd([n])).with_batch(batch)
+        plan, y = _run(cfg, x)
+        desc = plan.describe()
+        assert desc.count("post") == int(split) - 1, desc
+        want = c2c_expected(x, [n], batch=batch, inverse=inverse)
+        e = oracle.rel_l2(y, want)
+        print(f"n={n} b={batch} split={split} inverse={inverse} rel_l2={e:.3e} [{desc}]", flush=True)
+        assert e <= 1e-6, (split, inverse, e)

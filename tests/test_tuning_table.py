@@ -39,6 +39,7 @@ def test_parse_save_load_round_trip(tmp_path):
     f"{K1} = lds:lpw=x",
     f"{K1} = lds:lpw=99999",
     f"c64|n8 = {V1}",                              # unknown precision prefix
+    "split|f32|n65536 = 4",                        # splits are two or three passes
 ])
 def test_malformed_rejected_atomically(bad):
     W.parse_tuning(f"{K2} = lds:lpw=2\n")
@@ -52,3 +53,8 @@ def test_load_missing_file(tmp_path):
     with pytest.raises(W.FftError) as e:
         W.load_tuning(str(tmp_path / "absent.txt"))
     assert e.value.kind == "InvalidTuning"
+
+
+def test_split_entries():
+    assert W.parse_tuning("split|f64|n16777216 = 3\nsplit|f32|n65536 = 2\n") == 2
+    assert W.tuning_size() == 2

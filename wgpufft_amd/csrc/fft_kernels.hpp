@@ -42,13 +42,15 @@ struct LineArgs {
     uint32_t post_shift, post_mask;
     double scale;
     uint32_t conj_in, conj_out;
+    uint32_t pair_dj;  // line pairs: j of the second line minus j of the first (0 or 1)
 };
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-    // run of logical tiles so neighbouring tiles share its L2.
-    if ((n & 7u) != 0u) return b;
-    return (b & 7u) * (n >> 3) + (b >> 3);
+    // run of logical tiles so neighbouring tiles share its L2.  XCD x holds
+    // blocks x, x + 8, ...: q + (x < r) of them for n = 8q + r.
+    const uint32_t x = b & 7u, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 struct LineBase {
@@ -67,15 +69,37 @@ __device__ __forceinline__ LineBase line_base(const LineArgs& a, uint32_t g) {
     return b;
 }
 
-// W_P^t rounded once from an FP64 product of two FP64 table entries.
+// W_P^t as the FP64 product of two FP64 table entries.
+__device__ __forceinline__ double2 post_f64(const LineArgs& a, uint64_t t) {
+    const double2 h = a.post_hi[t >> a.post_shift];
+    const double2 l = a.post_lo[t & a.post_mask];
+    return make_double2(h.x * l.x - h.y * l.y, h.x * l.y + h.y * l.x);
+}
+
+// W_P^t rounded once from that product.
 template <class T>
 __device__ __forceinline__ cpx<T> post_twiddle(const LineArgs& a, uint64_t t) {
-    double2 h = a.post_hi[t >> a.post_shift];
-    double2 l = a.post_lo[t & a.post_mask];
-    double re = h.x * l.x - h.y * l.y;
-    double im = h.x * l.y + h.y * l.x;
-    return mk<T>((T)re, (T)im);
+    const double2 w = post_f64(a, t);
+    return mk<T>((T)w.x, (T)w.y);
 }
+
+// Four-step twiddles W_P^{jline (p0 + m L)} of one butterfly's outputs m = 0,
+// 1, ...: two table products (for p0 and for the step L) and an FP64
+// recurrence, so that a radix-R butterfly reads 4 table entries instead of 2R.
+// The R - 1 products add about R ulp of FP64 rounding (2e-15 for R = 16).
+struct PostSeq {
+    double2 w, step;
+    __device__ __forceinline__ void init(const LineArgs& a, uint32_t jline, uint32_t p0, uint32_t L) {
+        w = post_f64(a, (uint64_t)jline * p0);
+        step = post_f64(a, (uint64_t)jline * L);
+    }
+    template <class T>
+    __device__ __forceinline__ cpx<T> next() {
+        const cpx<T> r = mk<T>((T)w.x, (T)w.y);
+        w = make_double2(w.x * step.x - w.y * step.y, w.x * step.y + w.y * step.x);
+        return r;
+    }
+};
 
 // Complex or real point at element index idx (not the Hermitian mode); NT
 // selects the non-temporal load.
@@ -207,9 +231,10 @@ __device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uin
                                               cpx<typename Fmt::T> v0, cpx<typename Fmt::T> v1) {
     using T = typename Fmt::T;
     if constexpr (POST) {
+        // pairs along j (adjacent j at unit stride) take the twiddle of j + 1
         const cpx<T> w = post_twiddle<T>(a, (uint64_t)jline * k);
         v0 = cmul(v0, w);
-        v1 = cmul(v1, w);
+        v1 = cmul(v1, a.pair_dj ? post_twiddle<T>(a, (uint64_t)(jline + 1) * k) : w);
     }
     const OutScale<T> s = out_scale<T>(a);
     v0 = mk<T>(v0.x * s.re, v0.y * s.im);
@@ -451,14 +476,30 @@ struct RegFft {
                         const uint32_t k = j % L;
                         const uint32_t p0 = (j - k) * R + k;
                         const uint32_t o0 = b.out + p0 * sk;
+                        // outputs p0 + m L: twiddles by recurrence (pairs along j
+                        // take the twiddles of line j + 1 for their second line)
+                        PostSeq ps0, ps1;
+                        if constexpr (POST) {
+                            ps0.init(a, b.j, p0, L);
+                            if constexpr (VEC == 2)
+                                if (a.pair_dj) ps1.init(a, b.j + 1, p0, L);
+                        }
 #pragma unroll
                         for (int m = 0; m < R; m++) {
                             const uint32_t idx = o0 + (uint32_t)(m * L) * sk;
-                            if constexpr (VEC == 1)
-                                store_point_at<Fmt, STM, POST>(a, idx, b.j, p0 + m * L, N, w[0][q * R + m]);
-                            else
-                                store_pair_at<Fmt, POST>(a, idx, b.j, p0 + m * L, w[0][q * R + m],
-                                                         w[1][q * R + m]);
+                            cpx<T> v0 = w[0][q * R + m];
+                            cpx<T> t0;
+                            if constexpr (POST) {
+                                t0 = ps0.template next<T>();
+                                v0 = cmul(v0, t0);
+                            }
+                            if constexpr (VEC == 1) {
+                                store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0);
+                            } else {
+                                cpx<T> v1 = w[1][q * R + m];
+                                if constexpr (POST) v1 = cmul(v1, a.pair_dj ? ps1.template next<T>() : t0);
+                                store_pair_at<Fmt, false>(a, idx, b.j, p0 + m * L, v0, v1);
+                            }
                         }
                     }
                 }

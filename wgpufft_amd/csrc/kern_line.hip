@@ -104,6 +104,12 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    // f64 / df64 256-point passes of the three-pass split of long axes (2^24 =
+    // 256^3): columns with twiddle on store; rows with line-first store
+    REGV(FmtF64, P_F64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     // f64 / df64 4096-point lines: the two passes of the 2^24 four-step
     // (columns with twiddle on store; rows with transposed store), strided
     // and contiguous axes; component-split LDS exchange at four lines
@@ -126,6 +132,14 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 2048, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x8", 16, 16, 8),
     REGV(FmtF32, P_F32, 2048, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8x4", 8, 8, 8, 4),
     REGV(FmtF32, P_F32, 1024, 64, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
+    // strided lines one per thread, fewer lines per workgroup: more
+    // workgroups for short line counts (1025 packed columns of a 2048^2 R2C)
+    REGV(FmtF32, P_F32, 2048, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x8", 16, 16, 8),
+    REGV(FmtF32, P_F32, 2048, 128, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x8", 16, 16, 8),
+    REGV(FmtF32, P_F32, 2048, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x4", 8, 8, 8, 4),
+    REGV(FmtF32, P_F32, 2048, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x4", 8, 8, 8, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
     // strided line pairs in 256-thread workgroups (the batched kernel's shape)
     REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
     REGV(FmtF32, P_F32, 512, 32, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8", 8, 8, 8),

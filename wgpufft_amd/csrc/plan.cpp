@@ -284,8 +284,11 @@ int tuning_parse(const std::string& text) {
         if (eq == std::string::npos)
             fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": expected 'key = choice'");
         const std::string key = trim(line.substr(0, eq)), choice = trim(line.substr(eq + 1));
-        const bool key_ok = (key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 || key.rfind("df64|n", 0) == 0);
-        if (!key_ok || !valid_choice(choice))
+        const bool split_key = key.rfind("split|", 0) == 0;
+        const bool key_ok = split_key || key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 ||
+                            key.rfind("df64|n", 0) == 0;
+        const bool choice_ok = split_key ? (choice == "2" || choice == "3") : valid_choice(choice);
+        if (!key_ok || !choice_ok)
             fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": unknown key or choice");
         got.emplace_back(key, choice);
     }
@@ -414,6 +417,54 @@ static uint64_t best_split(int prec, uint64_t n, size_t budget, double* cost_out
     return best1;
 }
 
+// Three-pass split n = n3 * n1 * n2 (each smooth, on chip and with a register
+// variant for its pass), as balanced as possible; false if none.  For long
+// axes whose two-pass factors are long lines: a 4096-point f64 line fills
+// 64 KiB, so only two to four fit a workgroup and its strided side moves 32- to
+// 64-byte row segments, while 256-point lines go sixteen to a workgroup.
+static bool best_split3(int prec, uint64_t n, size_t budget, uint64_t* o3, uint64_t* o1, uint64_t* o2) {
+    const size_t es = prec == P_F32 ? 8 : 16;
+    auto fits = [&](uint64_t len, int ld_lf, int st_lf, int post) {
+        return is_smooth(len) && 2 * lds_line_stride((uint32_t)len) * es <= budget &&
+               !find_reg_variants(prec, (int)len, ld_lf, st_lf, LD_COMPLEX, ST_COMPLEX, post).empty();
+    };
+    double best = 1e30;
+    for (uint64_t a = 2; a * a <= n && a <= 4096; a++) {
+        if (n % a || !fits(a, 1, 1, 1)) continue;
+        const uint64_t m = n / a;
+        for (uint64_t b = 2; b < m && b <= 4096; b++) {
+            if (m % b) continue;
+            const uint64_t c = m / b;
+            if (c > 4096 || !fits(b, 0, 1, 0) || !fits(c, 1, 1, 1)) continue;
+            const double hi = (double)std::max(a, std::max(b, c)), lo = (double)std::min(a, std::min(b, c));
+            const double cost = std::log(hi / lo);
+            if (cost < best) {
+                best = cost;
+                *o3 = a;
+                *o1 = b;
+                *o2 = c;
+            }
+        }
+    }
+    return best < 1e29;
+}
+
+// Whether a long axis takes the three-pass split: by the tuning-table entry
+// "split|<prec>|n<n>" ("2" or "3") when present, else when the two-pass
+// split's longer factor fills more than 32 KiB per line or the axis has at
+// least 2^20 points.  Measured on MI355X (forward+inverse, f32): three passes
+// win from 2^20 points on (1.1-3x; the two passes of 2^20 have 1024 lines
+// each, too few workgroups for 256 CUs), lose up to 20 % below 2^19; f64 2^24
+// 1.3x.
+static bool prefer_three_pass(int prec, uint64_t n, uint64_t n1, uint64_t n2) {
+    static const char* pn[] = {"f32", "f64", "df64"};
+    const std::string forced = tuning_lookup(std::string("split|") + pn[prec] + "|n" + std::to_string(n));
+    if (forced == "2") return false;
+    if (forced == "3") return true;
+    const size_t es = prec == P_F32 ? 8 : 16;
+    return std::max(n1, n2) * es > 32768 || n >= (1u << 20);
+}
+
 struct Builder {
     Plan& P;
     hipStream_t s;
@@ -501,10 +552,13 @@ struct Builder {
         op.post = post;
         // Pairs of adjacent lines (16-byte accesses) need both sides to place line
         // g+1 right after line g with every stride even (16-byte alignment).
-        const bool pairable = ld_lf && st_lf && ldm == LD_COMPLEX && stm == ST_COMPLEX &&
-                              ls.lines % 2 == 0 && ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 &&
-                              in.sj % 2 == 0 && in.so % 2 == 0 && in.sn % 2 == 0 &&
-                              out.sj % 2 == 0 && out.so % 2 == 0 && out.sn % 2 == 0;
+        // With S1 = 1 the adjacent lines are j and j + 1 (unit j stride), which
+        // share their o when S2 is even.
+        const bool pairs_i = ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 && in.sj % 2 == 0 && out.sj % 2 == 0;
+        const bool pairs_j = ls.S1 == 1 && ls.S2 % 2 == 0 && in.sj == 1 && out.sj == 1;
+        const bool pairable = ld_lf && st_lf && ldm == LD_COMPLEX && stm == ST_COMPLEX && ls.lines % 2 == 0 &&
+                              (pairs_i || pairs_j) && in.so % 2 == 0 && in.sn % 2 == 0 && out.so % 2 == 0 &&
+                              out.sn % 2 == 0;
         op.n = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
         op.ld_lf = ld_lf;
         op.st_lf = st_lf;
@@ -545,6 +599,7 @@ struct Builder {
         if (rv && P.batch_mode && batch_class_for(rv) < 0) rv = nullptr;
         if (!rv && 2 * lstr * es > P.lds_budget) return false;
         op.a = base_args(ls, in, out, conj_in, conj_out, scale);
+        op.a.pair_dj = pairable && !pairs_i ? 1u : 0u;
         op.a.post_hi = post_hi;
         op.a.post_lo = post_lo;
         op.a.post_shift = post_shift;
@@ -805,6 +860,63 @@ struct Builder {
         return region + child->ws_bytes;
     }
 
+    // Four-step twiddle table W_n^t = hi[t >> sh] * lo[t & mask], t < n.
+    struct PostTable {
+        const double2* hi;
+        const double2* lo;
+        uint32_t sh, mask;
+    };
+    PostTable post_table(uint64_t n) {
+        uint32_t sh = 0;
+        while ((1ull << (2 * sh)) < n) sh++;
+        const uint64_t lo_n = 1ull << sh;
+        const uint64_t hi_n = (n + lo_n - 1) / lo_n;
+        double2* lo = (double2*)alloc(lo_n * sizeof(double2));
+        double2* hi = (double2*)alloc(hi_n * sizeof(double2));
+        hip_check(gen_twiddles_f64_strided(lo, n, lo_n, 1, s), "post twiddles");
+        hip_check(gen_twiddles_f64_strided(hi, n, hi_n, lo_n, s), "post twiddles");
+        return PostTable{hi, lo, sh, (uint32_t)(lo_n - 1)};
+    }
+
+    // Three-pass split n = n3 * n1 * n2 of a contiguous axis (stride 1, O
+    // lines 'n' apart), with q = q1 + n1 q2 + n1 n2 q3 and k = k3 + n3 k2 +
+    // n3 n2 k1, since W_n^{qk} = W_n3^{q3 k3} W_n^{(q1 + n1 q2) k3}
+    // W_n2^{q2 k2} W_{n1 n2}^{q1 k2} W_n1^{q1 k1}:
+    //   pass 1: n3-point DFTs over q3 (stride n1 n2), W_n^{(q1 + n1 q2) k3} on
+    //           store, in place into the workspace;
+    //   pass 2: n2-point DFTs over q2 (stride n1), W_{n1 n2}^{q1 k2} on store,
+    //           in place;
+    //   pass 3: n1-point DFTs over q1 (contiguous), stored at k3 + n3 k2 +
+    //           n3 n2 k1 with adjacent lines (k3, k3 + 1) adjacent in the output.
+    // Every pass moves whole row segments of its short lines.
+    bool three_pass(uint64_t n, uint64_t O, int src, uint64_t src_off, int dst, uint64_t dst_off,
+                    uint64_t ws_off, bool conj_in, bool conj_out, double scale, uint64_t n3, uint64_t n1,
+                    uint64_t n2, uint64_t* ws_used) {
+        const uint64_t m = n1 * n2;
+        const PostTable tn = post_table(n), tm = post_table(m);
+        // pass 1: lines (j = q1 + n1 q2 < m, o < O), points q3 at stride m
+        const LineSet l1{(uint32_t)(O * m), 1, (uint32_t)m};
+        const Layout in1{1, 1, (int64_t)n, (int64_t)m};
+        if (!line_op(n3, l1, in1, in1, src, src_off, B_WS, ws_off, conj_in, false, 1.0, LD_COMPLEX, ST_COMPLEX,
+                     tn.hi, tn.lo, tn.sh, tn.mask))
+            return false;
+        // pass 2: lines (j = q1 < n1, o' = k3 + n3 o at stride m), points q2 at stride n1
+        const LineSet l2{(uint32_t)(O * n3 * n1), 1, (uint32_t)n1};
+        const Layout in2{1, 1, (int64_t)m, (int64_t)n1};
+        if (!line_op(n2, l2, in2, in2, B_WS, ws_off, B_WS, ws_off, false, false, 1.0, LD_COMPLEX, ST_COMPLEX,
+                     tm.hi, tm.lo, tm.sh, tm.mask))
+            return false;
+        // pass 3: lines (i = k3 < n3, j = k2 < n2, o < O), points q1 contiguous
+        const LineSet l3{(uint32_t)(O * n3 * n2), (uint32_t)n3, (uint32_t)n2};
+        const Layout in3{(int64_t)m, (int64_t)n1, (int64_t)n, 1};
+        const Layout out3{1, (int64_t)n3, (int64_t)n, (int64_t)(n3 * n2)};
+        if (!line_op(n1, l3, in3, out3, B_WS, ws_off, dst, dst_off, false, conj_out, scale, LD_COMPLEX,
+                     ST_COMPLEX))
+            return false;
+        *ws_used = O * n * elem_bytes(P.prec);
+        return true;
+    }
+
     // Four-step split n = n1*n2 of an axis too long for one workgroup: pass 1
     // transforms n1-point columns (stride n2) with W_n^(q2 k1) on store into the
     // workspace, pass 2 transforms n2-point rows and stores transposed
@@ -818,22 +930,24 @@ struct Builder {
         if (!best1) return false;
         const uint64_t n1 = best1, n2 = n / n1;
         const uint64_t lines_total = S * O;
-        // post twiddle tables: W_n^t = hi[t >> sh] * lo[t & mask]
-        uint32_t sh = 0;
-        while ((1ull << (2 * sh)) < n) sh++;
-        const uint64_t lo_n = 1ull << sh;
-        const uint64_t hi_n = (n + lo_n - 1) / lo_n;
-        double2* lo = (double2*)alloc(lo_n * sizeof(double2));
-        double2* hi = (double2*)alloc(hi_n * sizeof(double2));
-        hip_check(gen_twiddles_f64_strided(lo, n, lo_n, 1, s), "post twiddles");
-        hip_check(gen_twiddles_f64_strided(hi, n, hi_n, lo_n, s), "post twiddles");
+        // Batched plans keep two passes (measured: no gain in the batched
+        // kernel, which already runs many transforms side by side).
+        uint64_t t3 = 0, t1 = 0, t2 = 0;
+        if (S == 1 && !P.batch_mode && prefer_three_pass(P.prec, n, n1, n2) &&
+            best_split3(P.prec, n, P.lds_budget, &t3, &t1, &t2)) {
+            const size_t mark = P.ops.size();
+            if (three_pass(n, O, src, src_off, dst, dst_off, ws_off, conj_in, conj_out, scale, t3, t1, t2, ws_used))
+                return true;
+            P.ops.resize(mark);
+        }
+        const PostTable pt = post_table(n);
         const uint64_t N = n;
         // pass 1: lines (i < S, q2 < n2, o < O), points q1 at stride S*n2
         LineSet l1{(uint32_t)(lines_total * n2), (uint32_t)S, (uint32_t)n2};
         Layout in1{1, (int64_t)S, (int64_t)(S * N), (int64_t)(S * n2)};
         Layout ws1 = in1;
         if (!line_op(n1, l1, in1, ws1, src, src_off, B_WS, ws_off, conj_in, false, 1.0, LD_COMPLEX,
-                     ST_COMPLEX, hi, lo, sh, (uint32_t)(lo_n - 1)))
+                     ST_COMPLEX, pt.hi, pt.lo, pt.sh, pt.mask))
             return false;
         // pass 2: lines (i < S, k1 < n1, o < O), points q2 at stride S
         LineSet l2{(uint32_t)(lines_total * n1), (uint32_t)S, (uint32_t)n1};
