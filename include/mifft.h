@@ -228,8 +228,9 @@ void mifft_batch_destroy(mifft_batch* b);
 /* -------------------------------------------------------------- serving */
 /* Continuous-batching request loop on one GPU (no counterpart in the
  * reference, which leaves submission to the caller).  A tick packs the
- * admitted 1D f32 C2C requests into one persistent batched launch and runs the
- * others through resident plans; up to `streams` ticks overlap on the device;
+ * admitted requests the batched kernel can run (1D / 2D C2C f32 and f64, R2C,
+ * below the batch limit) into one persistent batched launch per pass and runs
+ * the others through resident plans; up to `streams` ticks overlap on the device;
  * requests are admitted while input + output + workspace bytes of the work on
  * the GPU stay within the budget.  Request results do not depend on what else
  * is in flight. */
@@ -260,6 +261,11 @@ typedef struct mifft_server mifft_server;
 
 mifft_status mifft_server_create(uint64_t memory_budget_bytes, int32_t streams, int32_t max_tick_requests,
                                  mifft_server** out);
+/* Requests of at least `max_points` points (shape product x batch) run through
+ * their resident plans beside the batched launch instead of inside it
+ * (default 2^19: such a transform fills the GPU by itself).  0 batches nothing;
+ * UINT64_MAX batches everything the batched kernel can run. */
+mifft_status mifft_server_set_batch_limit(mifft_server* s, uint64_t max_points);
 /* Builds and keeps resident the plans of every distinct request shape, plus
  * the per-stream workspaces, descriptor arenas and pinned staging. */
 mifft_status mifft_server_prepare(mifft_server* s, const mifft_server_request* reqs, int32_t n);

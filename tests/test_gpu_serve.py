@@ -90,3 +90,22 @@ def test_serve_rejects_request_over_budget(torch_cuda):
     res = srv.replay(specs, bufs, [0.0, 0.0], mode=0)
     assert res["status"][0] != 0 and res["path"][0] == 2
     assert res["status"][1] == 0
+
+
+def test_serve_batch_limit_routes_large_requests(torch_cuda):
+    """Requests at or above the batch limit run through their resident plans
+    beside the batched launch; every result still matches the oracle."""
+    import torch
+    specs = [s for s in small_mix(30, 5) if s["precision"] == S.F32]
+    bufs = S.allocate(specs)
+    torch.cuda.synchronize()
+    inputs = [host(x, np.uint8).copy() for x, _ in bufs]
+    limit = 4096
+    srv = S.FftServer(budget_bytes=8 << 30, streams=4, max_tick_requests=32, batch_max_points=limit)
+    res = srv.replay(specs, bufs, S.poisson_arrivals(len(specs), 20_000.0, seed=9), mode=0)
+    assert (res["status"] == 0).all()
+    points = np.array([int(np.prod(s["shape"])) * s["batch"] for s in specs])
+    assert (res["path"][points >= limit] == 1).all()
+    assert (res["path"][points < limit] == 0).any() and (points >= limit).any()
+    for s, inp, (_, y) in zip(specs, inputs, bufs):
+        assert oracle.rel_l2(host(y, np.complex64), expected(s, inp)) <= 1e-6, s

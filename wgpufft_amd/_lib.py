@@ -127,6 +127,7 @@ SIGNATURES = [
                                ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     ("mifft_batch_destroy", None, [vp]),
     ("mifft_server_create", i32, [u64, i32, i32, ctypes.POINTER(vp)]),
+    ("mifft_server_set_batch_limit", i32, [vp, u64]),
     ("mifft_server_prepare", i32, [vp, ctypes.POINTER(ServerRequest), i32]),
     ("mifft_server_replay", i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(ctypes.c_double), i32, i32,
                                   ctypes.POINTER(ServerResult)]),

@@ -122,12 +122,18 @@ def allocate(specs, seed0=1000):
 
 
 class FftServer:
-    def __init__(self, budget_bytes: int = 64 << 30, streams: int = 4, max_tick_requests: int = 256):
+    def __init__(self, budget_bytes: int = 64 << 30, streams: int = 4, max_tick_requests: int = 256,
+                 batch_max_points: int | None = None):
+        """batch_max_points: requests of at least this many points run through
+        their resident plans beside the batched launch (None keeps the
+        library default, 2^19)."""
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         _check(self._lib.mifft_server_create(int(budget_bytes), int(streams), int(max_tick_requests),
                                              ctypes.byref(h)))
         self._h = h
+        if batch_max_points is not None:
+            _check(self._lib.mifft_server_set_batch_limit(h, int(batch_max_points)))
 
     @staticmethod
     def _requests(specs, bufs):
