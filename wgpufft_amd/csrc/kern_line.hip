@@ -128,6 +128,9 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 2048, 128, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x8", 16, 16, 8),
     REGV(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4x4", 8, 8, 4, 4),
     REGV(FmtF32, P_F32, 1024, 64, 8, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 128, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4x4", 8, 8, 4, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
     REGV(FmtF32, P_F32, 4096, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x16", 16, 16, 16),
     REGV(FmtF32, P_F32, 2048, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x8", 16, 16, 8),
     REGV(FmtF32, P_F32, 2048, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8x4", 8, 8, 8, 4),

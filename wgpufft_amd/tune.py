@@ -42,12 +42,17 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--reps", type=int, default=40)
+    ap.add_argument("--fresh", action="store_true",
+                    help="start from an empty table instead of the one at --out (whose entries the "
+                         "tuning does not cover, such as split choices, are otherwise kept)")
     args = ap.parse_args(argv)
     import torch  # device runtime set-up, as in every other entry point
 
     torch.cuda.init()
-    from . import autotune_sequence, clear_tuning, save_tuning, tuning_size
+    from . import autotune_sequence, clear_tuning, load_tuning, save_tuning, tuning_size
     clear_tuning()
+    if not args.fresh and os.path.exists(args.out):
+        load_tuning(args.out)
     for name, jobs in workloads():
         n, report = autotune_sequence(jobs, reps=args.reps)
         print(f"== {name}: {n} shape(s) tuned", flush=True)
