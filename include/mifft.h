@@ -53,8 +53,12 @@ typedef enum mifft_status {
     MIFFT_ERR_RECORDER_STATE = 21,             /* recorder used after end */
     MIFFT_ERR_OVER_BUDGET = 22,                /* server admission: request exceeds the memory budget */
     MIFFT_ERR_BUSY = 23,                       /* server: no free slot, try again */
-    MIFFT_ERR_UNALIGNED = 24                   /* FftError::BufferViewOffsetUnaligned: device
+    MIFFT_ERR_UNALIGNED = 24,                  /* FftError::BufferViewOffsetUnaligned: device
                                                   buffers must be 16-byte aligned */
+    MIFFT_ERR_LAYOUT_ZERO_STRIDE = 25,         /* FftError::BufferLayoutZeroStride */
+    MIFFT_ERR_LAYOUT_BATCH_STRIDE = 26,        /* FftError::BufferLayoutBatchStrideTooSmall */
+    MIFFT_ERR_LAYOUT_OUT_OF_BOUNDS = 27,       /* FftError::BufferLayoutOutOfBounds */
+    MIFFT_ERR_LAYOUT_TOO_LARGE = 28            /* FftError::BufferLayoutTooLarge */
 } mifft_status;
 
 /* FftTransformKind (plan.rs:24-29). */
@@ -274,6 +278,50 @@ mifft_status mifft_server_prepare(mifft_server* s, const mifft_server_request* r
 mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* reqs, const double* arrival_s,
                                  int32_t n, int32_t mode, mifft_server_result* results);
 void mifft_server_destroy(mifft_server* s);
+
+/* ------------------------------------------------------ logical views */
+/* FftLogicalLayout (src/runtime/logical_io.rs:33-38): logical element i of
+ * batch b sits at element element_offset + b * batch_stride + i * element_stride
+ * of its buffer, in elements of the endpoint's format (complex for C2C and the
+ * packed side of R2C / C2R, f32 for the real side).  batch_stride 0 means the
+ * per-batch span, (elements per batch - 1) * element_stride + 1. */
+typedef struct mifft_logical_layout {
+    uint64_t element_offset;
+    uint64_t element_stride;
+    uint64_t batch_stride;
+} mifft_logical_layout;
+
+/* FftLogicalLayout::contiguous (logical_io.rs:90-96). */
+void mifft_logical_layout_init(mifft_logical_layout* l);
+/* FftLogicalLayout::required_element_span (logical_io.rs:123-145): the
+ * elements the layout spans for `batch` batches of `per_batch` elements;
+ * MIFFT_ERR_LAYOUT_ZERO_STRIDE / _BATCH_STRIDE / _TOO_LARGE on a bad layout. */
+mifft_status mifft_logical_layout_span(const mifft_logical_layout* l, uint64_t per_batch, uint64_t batch,
+                                       uint64_t* elements);
+/* Workspace bytes of mifft_execute_logical_with_workspace for these layouts:
+ * the plan's workspace plus contiguous staging for each strided side. */
+mifft_status mifft_plan_logical_workspace_bytes(const mifft_plan* p, const mifft_logical_layout* in_layout,
+                                                const mifft_logical_layout* out_layout, uint64_t* bytes);
+/* FftPlan::execute_logical_views (plan.rs:734-746, c2c.rs:1564-1584,
+ * real.rs:997-1030): the plan over caller layouts.  A side that is contiguous
+ * (stride 1, batches back to back, 16-byte aligned start) is used in place; a
+ * strided side is gathered into / scattered from contiguous staging, and the
+ * elements between strided ones are left untouched.  NULL layouts are
+ * contiguous.  Without a workspace the plan grows its own staging (one
+ * in-flight execution per plan). */
+mifft_status mifft_execute_logical(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
+                                   const mifft_logical_layout* in_layout, void* out, uint64_t out_bytes,
+                                   const mifft_logical_layout* out_layout);
+/* FftPlan::execute_logical_views_with_workspace (plan.rs:748-763). */
+mifft_status mifft_execute_logical_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
+                                                  uint64_t in_bytes, const mifft_logical_layout* in_layout,
+                                                  void* out, uint64_t out_bytes,
+                                                  const mifft_logical_layout* out_layout, void* workspace,
+                                                  uint64_t workspace_bytes);
+/* FftPlan::record_logical_views (plan.rs:917-936). */
+mifft_status mifft_record_logical(mifft_recorder* r, const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                  const mifft_logical_layout* in_layout, void* out, uint64_t out_bytes,
+                                  const mifft_logical_layout* out_layout);
 
 /* ------------------------------------------------------------ autotuner */
 /* No single counterpart in the reference: wgpuFFT compiles in per-shape

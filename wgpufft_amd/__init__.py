@@ -24,6 +24,7 @@ __all__ = [
     "FftPrecision", "FftDirection", "Normalization", "FftTransformKind", "AxisKind", "C2cRoute",
     "FftTuning", "FftConfig", "FftPlan", "FftRecorder", "FftBatch", "FftError", "LibraryMissing",
     "create_plan", "create_r2c_plan", "create_c2r_plan", "fill_uniform_f32", "fill_normal_f32",
+    "FftLogicalLayout",
 ]
 
 
@@ -80,7 +81,8 @@ _STATUS_NAMES = {
     12: "BufferViewTooSmall", 13: "WorkspaceTooSmall", 14: "InPlaceUnsupported",
     15: "GpuPlanResourceAllocationFailed", 16: "HostBufferLength", 17: "NullArgument",
     18: "InvalidEnum", 19: "RankTooLarge", 20: "Hip", 21: "RecorderState", 22: "OverBudget",
-    23: "Busy", 24: "BufferViewOffsetUnaligned",
+    23: "Busy", 24: "BufferViewOffsetUnaligned", 25: "BufferLayoutZeroStride",
+    26: "BufferLayoutBatchStrideTooSmall", 27: "BufferLayoutOutOfBounds", 28: "BufferLayoutTooLarge",
 }
 
 
@@ -293,6 +295,42 @@ def _stream(s) -> int:
     return int(s)
 
 
+class FftLogicalLayout:
+    """FftLogicalLayout (src/runtime/logical_io.rs:33-38): logical element i of
+    batch b sits at element ``element_offset + b * batch_stride + i *
+    element_stride`` of its buffer (elements of the endpoint's format);
+    ``batch_stride=None`` is the per-batch span."""
+
+    def __init__(self, element_offset: int = 0, element_stride: int = 1, batch_stride: Optional[int] = None):
+        self.element_offset = int(element_offset)
+        self.element_stride = int(element_stride)
+        self.batch_stride = None if batch_stride is None else int(batch_stride)
+
+    @classmethod
+    def contiguous(cls) -> "FftLogicalLayout":
+        return cls()
+
+    def with_batch_stride(self, batch_stride: int) -> "FftLogicalLayout":
+        return FftLogicalLayout(self.element_offset, self.element_stride, batch_stride)
+
+    def _c(self):
+        l = _lib.LogicalLayout()
+        l.element_offset, l.element_stride = self.element_offset, self.element_stride
+        l.batch_stride = self.batch_stride or 0
+        return l
+
+    def required_element_span(self, per_batch: int, batch: int) -> int:
+        """Elements the layout spans (FftLogicalLayout::required_element_span)."""
+        n = ctypes.c_uint64()
+        c = self._c()
+        _check(_lib.load().mifft_logical_layout_span(ctypes.byref(c), int(per_batch), int(batch), ctypes.byref(n)))
+        return int(n.value)
+
+
+def _layout_ptr(layout: Optional[FftLogicalLayout]):
+    return None if layout is None else ctypes.byref(layout._c())
+
+
 class FftPlan:
     """FftPlan (plan.rs:32-936) over HIP device buffers."""
 
@@ -394,6 +432,38 @@ class FftPlan:
     def execute_in_place(self, buffer, stream=None) -> None:
         p, b = _buf(buffer)
         _check(self._lib.mifft_execute_in_place(self._h, _stream(stream), p, b))
+
+    # logical views (plan.rs:734-806, 917-936)
+    def logical_workspace_size_bytes(self, input_layout: Optional[FftLogicalLayout] = None,
+                                     output_layout: Optional[FftLogicalLayout] = None) -> int:
+        n = ctypes.c_uint64()
+        _check(self._lib.mifft_plan_logical_workspace_bytes(self._h, _layout_ptr(input_layout),
+                                                            _layout_ptr(output_layout), ctypes.byref(n)))
+        return int(n.value)
+
+    def execute_logical_views(self, input, input_layout: Optional[FftLogicalLayout], output,
+                              output_layout: Optional[FftLogicalLayout], stream=None) -> None:
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        _check(self._lib.mifft_execute_logical(self._h, _stream(stream), ip, ib, _layout_ptr(input_layout),
+                                               op, ob, _layout_ptr(output_layout)))
+
+    def execute_logical_views_with_workspace(self, input, input_layout: Optional[FftLogicalLayout], output,
+                                             output_layout: Optional[FftLogicalLayout], workspace,
+                                             stream=None) -> None:
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        wp, wb = _buf(workspace)
+        _check(self._lib.mifft_execute_logical_with_workspace(self._h, _stream(stream), ip, ib,
+                                                              _layout_ptr(input_layout), op, ob,
+                                                              _layout_ptr(output_layout), wp, wb))
+
+    def record_logical_views(self, recorder: "FftRecorder", input, input_layout: Optional[FftLogicalLayout],
+                             output, output_layout: Optional[FftLogicalLayout]) -> None:
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        _check(self._lib.mifft_record_logical(recorder._h, self._h, ip, ib, _layout_ptr(input_layout), op, ob,
+                                              _layout_ptr(output_layout)))
 
     def record(self, recorder: "FftRecorder", input, output) -> None:
         recorder._record(self, input, output)

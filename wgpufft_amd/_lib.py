@@ -57,6 +57,10 @@ class BatchItem(ctypes.Structure):
     ]
 
 
+class LogicalLayout(ctypes.Structure):
+    _fields_ = [("element_offset", u64), ("element_stride", u64), ("batch_stride", u64)]
+
+
 class ServerRequest(ctypes.Structure):
     _fields_ = [
         ("kind", i32),
@@ -112,6 +116,16 @@ SIGNATURES = [
     ("mifft_execute", i32, [vp, vp, vp, u64, vp, u64]),
     ("mifft_execute_with_workspace", i32, [vp, vp, vp, u64, vp, u64, vp, u64]),
     ("mifft_execute_in_place", i32, [vp, vp, vp, u64]),
+    ("mifft_logical_layout_init", None, [ctypes.POINTER(LogicalLayout)]),
+    ("mifft_logical_layout_span", i32, [ctypes.POINTER(LogicalLayout), u64, u64, ctypes.POINTER(u64)]),
+    ("mifft_plan_logical_workspace_bytes", i32, [vp, ctypes.POINTER(LogicalLayout), ctypes.POINTER(LogicalLayout),
+                                                 ctypes.POINTER(u64)]),
+    ("mifft_execute_logical", i32, [vp, vp, vp, u64, ctypes.POINTER(LogicalLayout), vp, u64,
+                                    ctypes.POINTER(LogicalLayout)]),
+    ("mifft_execute_logical_with_workspace", i32, [vp, vp, vp, u64, ctypes.POINTER(LogicalLayout), vp, u64,
+                                                   ctypes.POINTER(LogicalLayout), vp, u64]),
+    ("mifft_record_logical", i32, [vp, vp, vp, u64, ctypes.POINTER(LogicalLayout), vp, u64,
+                                   ctypes.POINTER(LogicalLayout)]),
     ("mifft_recorder_begin", i32, [vp, i32, ctypes.POINTER(vp)]),
     ("mifft_recorder_stream", vp, [vp]),
     ("mifft_record", i32, [vp, vp, vp, u64, vp, u64]),

@@ -230,6 +230,151 @@ static void exec_impl(const mifft_plan* p, hipStream_t s, const void* in, uint64
     run_plan(P, s, in, out, ws ? ws : P.own_ws.p);
 }
 
+// ------------------------------------------------------------ logical views
+// FftLogicalLayout semantics (src/runtime/logical_io.rs:100-178).
+static uint64_t checked_mul(uint64_t a, uint64_t b) {
+    if (a && b > ~0ull / a) fail(MIFFT_ERR_LAYOUT_TOO_LARGE, "layout value overflows 64 bits");
+    return a * b;
+}
+static uint64_t checked_add(uint64_t a, uint64_t b) {
+    if (b > ~0ull - a) fail(MIFFT_ERR_LAYOUT_TOO_LARGE, "layout value overflows 64 bits");
+    return a + b;
+}
+
+struct LogicalSide {
+    uint64_t off = 0, se = 1, sb = 0;  // resolved batch stride
+    uint64_t span = 0;                 // elements spanned
+    bool direct = true;                // usable as a contiguous buffer at base + off
+};
+
+static LogicalSide resolve_layout(const mifft_logical_layout* l, uint64_t per_batch, uint64_t batch) {
+    LogicalSide r;
+    if (l) {
+        r.off = l->element_offset;
+        r.se = l->element_stride;
+    }
+    if (r.se == 0) fail(MIFFT_ERR_LAYOUT_ZERO_STRIDE, "layout element stride must be greater than zero");
+    if (per_batch == 0 || batch == 0) return r;
+    const uint64_t per_span = checked_add(checked_mul(r.se, per_batch - 1), 1);
+    r.sb = (l && l->batch_stride) ? l->batch_stride : per_span;
+    if (r.sb < per_span)
+        fail(MIFFT_ERR_LAYOUT_BATCH_STRIDE, "layout batch stride is too small: required at least " +
+                                                std::to_string(per_span) + " elements, got " + std::to_string(r.sb));
+    r.span = checked_add(r.off, checked_add(checked_mul(batch - 1, r.sb), per_span));
+    r.direct = r.se == 1 && (batch <= 1 || r.sb == per_batch);
+    return r;
+}
+
+// Elements per batch and element bytes of a plan's input / output side.
+static void endpoint(const Plan& P, bool input, uint64_t* per_batch, uint32_t* es) {
+    const bool real = (P.kind == MIFFT_R2C && input) || (P.kind == MIFFT_C2R && !input);
+    *es = real ? 4u : (uint32_t)(P.prec == P_F32 ? 8 : 16);
+    const uint64_t bytes = input ? P.in_bytes : P.out_bytes;
+    *per_batch = bytes / *es / std::max<uint64_t>(1, P.cfg.batch);
+}
+
+static uint64_t stage_align(uint64_t b) { return (b + 255) & ~255ull; }
+
+struct LogicalExec {
+    LogicalSide in, out;
+    uint64_t in_pb = 0, out_pb = 0;
+    uint32_t in_es = 0, out_es = 0;
+    uint64_t ws_bytes = 0, stage_in = 0, stage_out = 0;  // byte offsets / sizes within the workspace
+};
+
+static LogicalExec plan_logical(const Plan& P, const void* in, const void* out, const mifft_logical_layout* li,
+                                const mifft_logical_layout* lo) {
+    LogicalExec e;
+    const uint64_t batch = std::max<uint64_t>(1, P.cfg.batch);
+    endpoint(P, true, &e.in_pb, &e.in_es);
+    endpoint(P, false, &e.out_pb, &e.out_es);
+    e.in = resolve_layout(li, e.in_pb, batch);
+    e.out = resolve_layout(lo, e.out_pb, batch);
+    // a contiguous side at an offset keeps the kernels' 16-byte alignment
+    if (e.in.direct && (((uintptr_t)in + e.in.off * e.in_es) & 15u)) e.in.direct = false;
+    if (e.out.direct && (((uintptr_t)out + e.out.off * e.out_es) & 15u)) e.out.direct = false;
+    // contiguous sides that overlap without coinciding: read the input from staging
+    if (e.in.direct && e.out.direct && in && out) {
+        const uintptr_t a0 = (uintptr_t)in + e.in.off * e.in_es, a1 = a0 + P.in_bytes;
+        const uintptr_t b0 = (uintptr_t)out + e.out.off * e.out_es, b1 = b0 + P.out_bytes;
+        if (a0 != b0 && a0 < b1 && b0 < a1) e.in.direct = false;
+    }
+    uint64_t b = stage_align(P.ws_bytes);
+    if (!e.in.direct) {
+        e.stage_in = b;
+        b += stage_align(P.in_bytes);
+    }
+    if (!e.out.direct) {
+        e.stage_out = b;
+        b += stage_align(P.out_bytes);
+    }
+    e.ws_bytes = b;
+    return e;
+}
+
+static void exec_logical(const mifft_plan* p, hipStream_t s, const void* in, uint64_t in_bytes,
+                         const mifft_logical_layout* li, void* out, uint64_t out_bytes,
+                         const mifft_logical_layout* lo, void* ws, uint64_t ws_bytes, bool caller_ws) {
+    if (!p) fail(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    if (!in || !out) fail(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    if (((uintptr_t)in | (uintptr_t)out) & 15u) fail(MIFFT_ERR_UNALIGNED, "device buffers must be 16-byte aligned");
+    Plan& P = *p->p;
+    const LogicalExec e = plan_logical(P, in, out, li, lo);
+    const uint64_t batch = std::max<uint64_t>(1, P.cfg.batch);
+    const uint64_t need_in = checked_mul(e.in.span, e.in_es), need_out = checked_mul(e.out.span, e.out_es);
+    if (in_bytes < need_in)
+        fail(MIFFT_ERR_LAYOUT_OUT_OF_BOUNDS, "input layout is out of bounds: required " + std::to_string(need_in) +
+                                                 " bytes, got " + std::to_string(in_bytes));
+    if (out_bytes < need_out)
+        fail(MIFFT_ERR_LAYOUT_OUT_OF_BOUNDS, "output layout is out of bounds: required " +
+                                                 std::to_string(need_out) + " bytes, got " + std::to_string(out_bytes));
+    if (e.in.direct && e.out.direct) {
+        // both sides contiguous: the plain execution at the offsets
+        exec_impl(p, s, (const char*)in + e.in.off * e.in_es, P.in_bytes, (char*)out + e.out.off * e.out_es,
+                  P.out_bytes, caller_ws ? ws : nullptr, caller_ws ? ws_bytes : 0);
+        return;
+    }
+    char* base = nullptr;
+    if (caller_ws) {
+        if (!ws) fail(MIFFT_ERR_NULL_ARGUMENT, "null workspace");
+        if (ws_bytes < e.ws_bytes)
+            fail(MIFFT_ERR_WORKSPACE_TOO_SMALL, "workspace needs " + std::to_string(e.ws_bytes) + " bytes, got " +
+                                                    std::to_string(ws_bytes));
+        base = (char*)ws;
+    } else {
+        std::lock_guard<std::mutex> g(P.ws_mu);
+        if (P.stage.bytes < e.ws_bytes) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            (void)hipStreamIsCapturing(s, &cs);
+            if (cs != hipStreamCaptureStatusNone)
+                fail(MIFFT_ERR_RECORDER_STATE, "first strided logical execution cannot be graph-captured");
+            if (P.stage.p) hip_check(hipFree(P.stage.p), "staging free");
+            P.stage.p = nullptr;
+            P.stage.bytes = 0;
+            hip_check(hipMalloc(&P.stage.p, e.ws_bytes), "logical view staging");
+            P.stage.bytes = e.ws_bytes;
+        }
+        base = (char*)P.stage.p;
+    }
+    void* plan_ws = P.ws_bytes ? (void*)base : nullptr;
+    const void* src = (const char*)in + e.in.off * e.in_es;
+    void* dst = (char*)out + e.out.off * e.out_es;
+    if (!e.in.direct) {
+        src = base + e.stage_in;
+        hip_check(launch_strided_copy(e.in_es, in, (void*)src, e.in_pb, batch, e.in.off, e.in.se, e.in.sb, 0, 1,
+                                      e.in_pb, s),
+                  "logical input gather");
+    }
+    if (!e.out.direct) dst = base + e.stage_out;
+    // src and dst never alias here (one side is staging)
+    if (plan_ws) run_plan(P, s, src, dst, plan_ws);
+    else run_plan(P, s, src, dst, nullptr);
+    if (!e.out.direct)
+        hip_check(launch_strided_copy(e.out_es, dst, out, e.out_pb, batch, 0, 1, e.out_pb, e.out.off, e.out.se,
+                                      e.out.sb, s),
+                  "logical output scatter");
+}
+
 mifft_status mifft_execute(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
                            void* out, uint64_t out_bytes) {
     return guard([&] { exec_impl(p, stream, in, in_bytes, out, out_bytes, nullptr, 0); });
@@ -268,6 +413,56 @@ mifft_status mifft_record(mifft_recorder* r, const mifft_plan* p, const void* in
     if (!r) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null recorder");
     if (r->ended) return set_err(MIFFT_ERR_RECORDER_STATE, "recorder already ended");
     return guard([&] { exec_impl(p, r->stream, in, in_bytes, out, out_bytes, nullptr, 0); });
+}
+
+mifft_status mifft_record_logical(mifft_recorder* r, const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                  const mifft_logical_layout* in_layout, void* out, uint64_t out_bytes,
+                                  const mifft_logical_layout* out_layout) {
+    if (!r) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null recorder");
+    if (r->ended) return set_err(MIFFT_ERR_RECORDER_STATE, "recorder already ended");
+    return guard([&] {
+        exec_logical(p, r->stream, in, in_bytes, in_layout, out, out_bytes, out_layout, nullptr, 0, false);
+    });
+}
+
+void mifft_logical_layout_init(mifft_logical_layout* l) {
+    if (l) *l = mifft_logical_layout{0, 1, 0};
+}
+
+mifft_status mifft_logical_layout_span(const mifft_logical_layout* l, uint64_t per_batch, uint64_t batch,
+                                       uint64_t* elements) {
+    if (!elements) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null output");
+    return guard([&] { *elements = resolve_layout(l, per_batch, batch).span; });
+}
+
+mifft_status mifft_plan_logical_workspace_bytes(const mifft_plan* p, const mifft_logical_layout* in_layout,
+                                                const mifft_logical_layout* out_layout, uint64_t* bytes) {
+    if (!p || !bytes) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    return guard([&] {
+        // alignment of the caller's buffers is unknown here: count staging for
+        // every side that is not contiguous by its layout alone
+        const LogicalExec e = plan_logical(*p->p, nullptr, nullptr, in_layout, out_layout);
+        *bytes = (e.in.direct && e.out.direct) ? p->p->ws_bytes : e.ws_bytes;
+    });
+}
+
+mifft_status mifft_execute_logical(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
+                                   const mifft_logical_layout* in_layout, void* out, uint64_t out_bytes,
+                                   const mifft_logical_layout* out_layout) {
+    return guard([&] {
+        exec_logical(p, stream, in, in_bytes, in_layout, out, out_bytes, out_layout, nullptr, 0, false);
+    });
+}
+
+mifft_status mifft_execute_logical_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
+                                                  uint64_t in_bytes, const mifft_logical_layout* in_layout,
+                                                  void* out, uint64_t out_bytes,
+                                                  const mifft_logical_layout* out_layout, void* workspace,
+                                                  uint64_t workspace_bytes) {
+    return guard([&] {
+        exec_logical(p, stream, in, in_bytes, in_layout, out, out_bytes, out_layout, workspace, workspace_bytes,
+                     true);
+    });
 }
 
 mifft_status mifft_record_in_place(mifft_recorder* r, const mifft_plan* p, void* buf, uint64_t bytes) {
@@ -330,6 +525,10 @@ const char* mifft_status_string(mifft_status s) {
         case MIFFT_ERR_OVER_BUDGET: return "over memory budget";
         case MIFFT_ERR_BUSY: return "busy";
         case MIFFT_ERR_UNALIGNED: return "buffer not 16-byte aligned";
+        case MIFFT_ERR_LAYOUT_ZERO_STRIDE: return "layout element stride is zero";
+        case MIFFT_ERR_LAYOUT_BATCH_STRIDE: return "layout batch stride too small";
+        case MIFFT_ERR_LAYOUT_OUT_OF_BOUNDS: return "layout out of bounds";
+        case MIFFT_ERR_LAYOUT_TOO_LARGE: return "layout value too large";
     }
     return "unknown status";
 }

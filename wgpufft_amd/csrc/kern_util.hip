@@ -382,6 +382,58 @@ hipError_t fill_normal_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s)
     return hipGetLastError();
 }
 
+// Logical-view copies: element (b, i) of a batch of `per_batch` elements moves
+// from src[s.off + b s.sb + i s.se] to dst[d.off + b d.sb + i d.se] (elements
+// of E).  Gathers a strided caller layout into a contiguous staging array and
+// scatters one back (execute_logical_views, src/runtime/c2c.rs:1696-1860).
+struct CopyLayout {
+    uint64_t off, se, sb;
+};
+template <class E, bool WIDE>
+__global__ void k_strided_copy(const E* __restrict__ src, E* __restrict__ dst, uint32_t per_batch, DivMagic dp,
+                               uint64_t total, CopyLayout sl, CopyLayout dl) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t b, i;
+        if constexpr (WIDE) {
+            b = t / per_batch;
+            i = t - b * per_batch;
+        } else {
+            b = fdiv((uint32_t)t, dp);
+            i = t - b * per_batch;
+        }
+        dst[dl.off + b * dl.sb + i * dl.se] = src[sl.off + b * sl.sb + i * sl.se];
+    }
+}
+
+template <class E>
+static hipError_t strided_copy_as(const void* src, void* dst, uint64_t per_batch, uint64_t batch, const CopyLayout& sl,
+                                  const CopyLayout& dl, hipStream_t s) {
+    const uint64_t total = per_batch * batch;
+    const DivMagic dp = make_magic((uint32_t)per_batch);
+    if (total < (1ull << 31))
+        k_strided_copy<E, false><<<grid_for(total), 256, 0, s>>>((const E*)src, (E*)dst, (uint32_t)per_batch, dp,
+                                                                 total, sl, dl);
+    else
+        k_strided_copy<E, true><<<grid_for(total), 256, 0, s>>>((const E*)src, (E*)dst, (uint32_t)per_batch, dp,
+                                                                total, sl, dl);
+    return hipGetLastError();
+}
+
+hipError_t launch_strided_copy(uint32_t elem_bytes, const void* src, void* dst, uint64_t per_batch, uint64_t batch,
+                               uint64_t s_off, uint64_t s_se, uint64_t s_sb, uint64_t d_off, uint64_t d_se,
+                               uint64_t d_sb, hipStream_t s) {
+    if (per_batch == 0 || batch == 0) return hipSuccess;
+    if (per_batch > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const CopyLayout sl{s_off, s_se, s_sb}, dl{d_off, d_se, d_sb};
+    switch (elem_bytes) {
+        case 4: return strided_copy_as<uint32_t>(src, dst, per_batch, batch, sl, dl, s);
+        case 8: return strided_copy_as<uint2>(src, dst, per_batch, batch, sl, dl, s);
+        case 16: return strided_copy_as<uint4>(src, dst, per_batch, batch, sl, dl, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 DivMagic make_magic(uint32_t d) {
     DivMagic r;
     if (d <= 1) {

@@ -93,6 +93,11 @@ hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uin
 // k <= n0/2 with a scale, Hermitian extension of packed lines, real part with a
 // scale.  Lines are contiguous.
 hipError_t launch_real_widen(const void* in, void* out, uint64_t count, hipStream_t s);
+// dst[d_off + b d_sb + i d_se] = src[s_off + b s_sb + i s_se] for i < per_batch,
+// b < batch, in elements of elem_bytes (4, 8 or 16).
+hipError_t launch_strided_copy(uint32_t elem_bytes, const void* src, void* dst, uint64_t per_batch, uint64_t batch,
+                               uint64_t s_off, uint64_t s_se, uint64_t s_sb, uint64_t d_off, uint64_t d_se,
+                               uint64_t d_sb, hipStream_t s);
 hipError_t launch_pack(const void* full, void* packed, uint32_t n0, uint64_t lines, double scale, hipStream_t s);
 hipError_t launch_herm_extend(const void* packed, void* full, uint32_t n0, uint64_t lines, hipStream_t s);
 hipError_t launch_real_part(const void* full, void* out, uint64_t count, double scale, hipStream_t s);

@@ -30,6 +30,7 @@ Plan::~Plan() {
         if (b.p) (void)hipFree(b.p);
     if (own_ws.p) (void)hipFree(own_ws.p);
     if (in_copy.p) (void)hipFree(in_copy.p);
+    if (stage.p) (void)hipFree(stage.p);
 }
 
 // ------------------------------------------------------------ integer helpers

@@ -96,6 +96,7 @@ struct Plan {
     std::mutex ws_mu;
     DevBuf own_ws;
     DevBuf in_copy;  // lazily created input copy for in-place real transforms
+    DevBuf stage;    // lazily created staging of strided logical views
     ~Plan();
 };
 
