@@ -42,7 +42,6 @@ struct LineArgs {
     uint32_t post_shift, post_mask;
     double scale;
     uint32_t conj_in, conj_out;
-    uint32_t pair_dj;  // line pairs: j of the second line minus j of the first (0 or 1)
 };
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
@@ -231,10 +230,10 @@ __device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uin
                                               cpx<typename Fmt::T> v0, cpx<typename Fmt::T> v1) {
     using T = typename Fmt::T;
     if constexpr (POST) {
-        // pairs along j (adjacent j at unit stride) take the twiddle of j + 1
+        // the two lines of a pair share j (adjacent i)
         const cpx<T> w = post_twiddle<T>(a, (uint64_t)jline * k);
         v0 = cmul(v0, w);
-        v1 = cmul(v1, a.pair_dj ? post_twiddle<T>(a, (uint64_t)(jline + 1) * k) : w);
+        v1 = cmul(v1, w);
     }
     const OutScale<T> s = out_scale<T>(a);
     v0 = mk<T>(v0.x * s.re, v0.y * s.im);
@@ -476,14 +475,10 @@ struct RegFft {
                         const uint32_t k = j % L;
                         const uint32_t p0 = (j - k) * R + k;
                         const uint32_t o0 = b.out + p0 * sk;
-                        // outputs p0 + m L: twiddles by recurrence (pairs along j
-                        // take the twiddles of line j + 1 for their second line)
-                        PostSeq ps0, ps1;
-                        if constexpr (POST) {
-                            ps0.init(a, b.j, p0, L);
-                            if constexpr (VEC == 2)
-                                if (a.pair_dj) ps1.init(a, b.j + 1, p0, L);
-                        }
+                        // outputs p0 + m L: four-step twiddles by recurrence (the
+                        // lines of a pair share j)
+                        PostSeq ps0;
+                        if constexpr (POST) ps0.init(a, b.j, p0, L);
 #pragma unroll
                         for (int m = 0; m < R; m++) {
                             const uint32_t idx = o0 + (uint32_t)(m * L) * sk;
@@ -497,7 +492,7 @@ struct RegFft {
                                 store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0);
                             } else {
                                 cpx<T> v1 = w[1][q * R + m];
-                                if constexpr (POST) v1 = cmul(v1, a.pair_dj ? ps1.template next<T>() : t0);
+                                if constexpr (POST) v1 = cmul(v1, t0);
                                 store_pair_at<Fmt, false>(a, idx, b.j, p0 + m * L, v0, v1);
                             }
                         }

@@ -453,17 +453,17 @@ static bool best_split3(int prec, uint64_t n, size_t budget, uint64_t* o3, uint6
 // Whether a long axis takes the three-pass split: by the tuning-table entry
 // "split|<prec>|n<n>" ("2" or "3") when present, else when the two-pass
 // split's longer factor fills more than 32 KiB per line or the axis has at
-// least 2^20 points.  Measured on MI355X (forward+inverse, f32): three passes
-// win from 2^20 points on (1.1-3x; the two passes of 2^20 have 1024 lines
-// each, too few workgroups for 256 CUs), lose up to 20 % below 2^19; f64 2^24
-// 1.3x.
+// least 2^19 points.  Measured on MI355X (forward+inverse pairs, f32): three
+// passes win at 2^19-2^21 (1.1-1.4x: the two passes of 2^20 have 1024 lines
+// each, too few workgroups for 256 CUs) and 2^23-2^26 (1.0-3.7x), lose 13 %
+// at 2^22 (the MI355X table keeps two there); f64 / df64 2^24 1.4-1.5x.
 static bool prefer_three_pass(int prec, uint64_t n, uint64_t n1, uint64_t n2) {
     static const char* pn[] = {"f32", "f64", "df64"};
     const std::string forced = tuning_lookup(std::string("split|") + pn[prec] + "|n" + std::to_string(n));
     if (forced == "2") return false;
     if (forced == "3") return true;
     const size_t es = prec == P_F32 ? 8 : 16;
-    return std::max(n1, n2) * es > 32768 || n >= (1u << 20);
+    return std::max(n1, n2) * es > 32768 || n >= (1u << 19);
 }
 
 struct Builder {
@@ -553,12 +553,12 @@ struct Builder {
         op.post = post;
         // Pairs of adjacent lines (16-byte accesses) need both sides to place line
         // g+1 right after line g with every stride even (16-byte alignment).
-        // With S1 = 1 the adjacent lines are j and j + 1 (unit j stride), which
-        // share their o when S2 is even.
-        const bool pairs_i = ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 && in.sj % 2 == 0 && out.sj % 2 == 0;
-        const bool pairs_j = ls.S1 == 1 && ls.S2 % 2 == 0 && in.sj == 1 && out.sj == 1;
+        // (Pairing lines j and j + 1 of the four-step passes, whose i index
+        // is trivial, measured slower: the pair variants' wider workgroups
+        // leave too few of them for these line counts.)
         const bool pairable = ld_lf && st_lf && ldm == LD_COMPLEX && stm == ST_COMPLEX && ls.lines % 2 == 0 &&
-                              (pairs_i || pairs_j) && in.so % 2 == 0 && in.sn % 2 == 0 && out.so % 2 == 0 &&
+                              ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 && in.sj % 2 == 0 &&
+                              in.so % 2 == 0 && in.sn % 2 == 0 && out.sj % 2 == 0 && out.so % 2 == 0 &&
                               out.sn % 2 == 0;
         op.n = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
         op.ld_lf = ld_lf;
@@ -600,7 +600,6 @@ struct Builder {
         if (rv && P.batch_mode && batch_class_for(rv) < 0) rv = nullptr;
         if (!rv && 2 * lstr * es > P.lds_budget) return false;
         op.a = base_args(ls, in, out, conj_in, conj_out, scale);
-        op.a.pair_dj = pairable && !pairs_i ? 1u : 0u;
         op.a.post_hi = post_hi;
         op.a.post_lo = post_lo;
         op.a.post_shift = post_shift;
