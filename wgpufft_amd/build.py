@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmifft.so")
-SOURCES = ["kern_line.hip", "kern_conv.hip", "kern_util.hip", "kern_batch.hip", "plan.cpp",
+SOURCES = ["kern_line.hip", "kern_conv.hip", "kern_util.hip", "kern_batch.hip", "kern_small.hip", "plan.cpp",
            "api.cpp", "batch.cpp", "serve.cpp"]
 ARCH = os.environ.get("MIFFT_ARCH", "gfx950")
 

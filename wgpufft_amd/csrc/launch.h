@@ -93,6 +93,25 @@ hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uin
 // k <= n0/2 with a scale, Hermitian extension of packed lines, real part with a
 // scale.  Lines are contiguous.
 hipError_t launch_real_widen(const void* in, void* out, uint64_t count, hipStream_t s);
+// Small-volume kernel (kern_small.hip): all axes of an f32 volume of
+// elems <= 4096 points per workgroup, axis a of length sc[a].n at stride S[a].
+constexpr int kSmallVolMaxRank = 8;  // MIFFT_MAX_RANK
+struct SmallVolArgs {
+    const void* in;
+    void* out;
+    uint32_t elems, buf_elems, rank;
+    uint32_t S[kSmallVolMaxRank];
+    DivMagic dS[kSmallVolMaxRank];
+    DivMagic dL[kSmallVolMaxRank];  // divide by elems / n of the axis (its line count)
+    const void* tw;                     // every axis's stage twiddles, back to back
+    uint32_t tw_len[kSmallVolMaxRank];  // stage-twiddle entries of each axis
+    uint32_t tw_total;                  // their sum (copied to LDS at start)
+    const Sched* sc;  // device array of rank schedules
+    float scale;
+    uint32_t conj_in, conj_out;
+};
+size_t small_volume_lds_bytes(const SmallVolArgs& v);
+hipError_t launch_small_volume(const SmallVolArgs& v, uint32_t volumes, hipStream_t s);
 // dst[d_off + b d_sb + i d_se] = src[s_off + b s_sb + i s_se] for i < per_batch,
 // b < batch, in elements of elem_bytes (4, 8 or 16).
 hipError_t launch_strided_copy(uint32_t elem_bytes, const void* src, void* dst, uint64_t per_batch, uint64_t batch,

@@ -36,6 +36,7 @@ enum OpKind {
     OP_PACK,        // general R2C: complex lines -> packed bins k <= N0/2
     OP_HERM_EXTEND, // general C2R: packed lines -> Hermitian-extended complex lines
     OP_REAL_PART,   // general C2R: complex lines -> real part
+    OP_SMALL_VOLUME // every axis of small f32 volumes, one workgroup per volume
 };
 
 struct Plan;
@@ -48,6 +49,7 @@ struct Op {
     LineArgs a{};
     Sched sc{};
     ConvArgs conv{};
+    SmallVolArgs sv{};  // OP_SMALL_VOLUME (volumes in count)
     const RegVariant* rv = nullptr;
     int ldm = LD_COMPLEX, stm = ST_COMPLEX, post = 0;
     Plan* sub = nullptr;     // OP_SUBPLAN
