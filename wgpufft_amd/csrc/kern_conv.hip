@@ -47,4 +47,72 @@ hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s) {
     }
 }
 
+// Direct DFT of prime lines (src/runtime/direct_prime.rs:25-110), for the
+// primes up to direct_max_prime whose Rader convolution would need zero
+// padding.  A task computes the output pair (k, p - k) from the symmetric and
+// antisymmetric sums s_n = x_n + x_{p-n}, d_n = x_n - x_{p-n}, n = 1..(p-1)/2:
+//   X_k     = x_0 + sum s_n cos(2 pi n k / p) - i sum d_n sin(2 pi n k / p),
+//   X_{p-k} = x_0 + sum s_n cos(2 pi n k / p) + i sum d_n sin(2 pi n k / p);
+// task k = 0 sums the line.  The lines and W_p^t (t < p) sit in LDS; slots and
+// tasks run lines-fastest on strided axes so that loads and stores coalesce
+// across lines.
+__global__ void __launch_bounds__(256) k_direct_prime(LineArgs a, uint32_t p, uint32_t lpw, uint32_t lf,
+                                                      const void* wtab) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<float>* w = reinterpret_cast<cpx<float>*>(smem_raw);
+    cpx<float>* xs = w + p;
+    const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x) * lpw;
+    for (uint32_t t = threadIdx.x; t < p; t += blockDim.x) w[t] = ld_tw<float>(wtab, t);
+    const uint32_t slots = lpw * p;
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const uint32_t l = lf ? s % lpw : s / p, n = lf ? s / lpw : s % p;
+        const uint32_t g = g0 + l;
+        const LineBase b = line_base(a, g < a.lines ? g : a.lines - 1);
+        xs[l * p + n] = load_point<FmtF32, LD_COMPLEX>(a, b.in, n, p);
+    }
+    __syncthreads();
+    const uint32_t h = (p - 1) / 2, tasks = lpw * (h + 1);
+    for (uint32_t s = threadIdx.x; s < tasks; s += blockDim.x) {
+        const uint32_t l = lf ? s % lpw : s / (h + 1), k = lf ? s / lpw : s % (h + 1);
+        const uint32_t g = g0 + l;
+        if (g >= a.lines) continue;
+        const LineBase b = line_base(a, g);
+        const cpx<float>* x = xs + l * p;
+        const cpx<float> x0 = x[0];
+        if (k == 0) {
+            cpx<float> sum = x0;
+            for (uint32_t n = 1; n < p; n++) sum = mk<float>(sum.x + x[n].x, sum.y + x[n].y);
+            store_point<FmtF32, ST_COMPLEX, false>(a, b.out, b.j, 0, p, sum);
+            continue;
+        }
+        float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+        uint32_t t = 0;
+        for (uint32_t n = 1; n <= h; n++) {
+            t += k;
+            if (t >= p) t -= p;
+            const cpx<float> u = x[n], v = x[p - n];
+            const float c = w[t].x, sn = -w[t].y;  // W_p^t = cos - i sin
+            ar += (u.x + v.x) * c;
+            ai += (u.y + v.y) * c;
+            br += (u.x - v.x) * sn;
+            bi += (u.y - v.y) * sn;
+        }
+        // -i B = (B.y, -B.x)
+        store_point<FmtF32, ST_COMPLEX, false>(a, b.out, b.j, k, p, mk<float>(x0.x + ar + bi, x0.y + ai - br));
+        store_point<FmtF32, ST_COMPLEX, false>(a, b.out, b.j, p - k, p, mk<float>(x0.x + ar - bi, x0.y + ai + br));
+    }
+}
+
+size_t direct_prime_bytes(uint32_t p, uint32_t lpw) { return ((size_t)lpw * p + p) * sizeof(cpx<float>); }
+
+hipError_t launch_direct_prime(const LineArgs& a, uint32_t p, uint32_t lpw, bool lf, const void* wtab,
+                               hipStream_t s) {
+    const size_t bytes = direct_prime_bytes(p, lpw);
+    hipError_t e = ensure_lds_attr((const void*)&k_direct_prime, bytes);
+    if (e != hipSuccess) return e;
+    const uint32_t blocks = (a.lines + lpw - 1) / lpw;
+    k_direct_prime<<<blocks, 256, bytes, s>>>(a, p, lpw, lf ? 1u : 0u, wtab);
+    return hipGetLastError();
+}
+
 }  // namespace mifft

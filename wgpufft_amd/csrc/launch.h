@@ -67,6 +67,11 @@ struct ConvArgs {
 };
 hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s);
 hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s);
+// Direct f32 DFT of prime lines of length p (kern_conv.hip), lpw lines per
+// workgroup, lines-fastest slots when lf; wtab = W_p^t, t < p.
+size_t direct_prime_bytes(uint32_t p, uint32_t lpw);
+hipError_t launch_direct_prime(const LineArgs& a, uint32_t p, uint32_t lpw, bool lf, const void* wtab,
+                               hipStream_t s);
 size_t conv_kernel_bytes(int prec, uint32_t m, uint32_t lpw);
 
 // Table generation (FP64 on the device, rounded once to the table type).

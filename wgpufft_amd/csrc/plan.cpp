@@ -926,6 +926,44 @@ struct Builder {
         return true;
     }
 
+    // The direct prime DFT (src/runtime/c2c.rs:245-254, rader_axis.rs:290-320):
+    // f32 primes up to direct_max_prime whose Rader convolution is not the
+    // cyclic one of length p - 1 (p - 1 not smooth), unless Rader is forced
+    // on the axis; not in batched plans (no batched class).
+    int cur_axis = -1;  // axis being planned, for the forced-axis tunings
+    bool direct_prime_op(uint64_t p, const LineSet& ls, const Layout& lay, int src, uint64_t src_off, int dst,
+                         uint64_t dst_off, bool inverse, double scale) {
+        const mifft_tuning& t = P.cfg.tuning;
+        if (P.batch_mode || P.prec != P_F32 || p < 3 || p > t.direct_max_prime || rader_conv_len(p) == p - 1)
+            return false;
+        for (int i = 0; i < t.n_force_rader_axes; i++)
+            if (t.force_rader_axes[i] == cur_axis) return false;
+        const bool lf = adjacent_lines(ls, lay);
+        const uint32_t tasks = (uint32_t)((p - 1) / 2 + 1);
+        uint32_t lpw = std::max<uint32_t>(1, 256 / tasks);
+        if (lf) lpw = std::max<uint32_t>(lpw, 8);
+        lpw = std::min<uint32_t>(lpw, std::max<uint32_t>(1, ls.lines / 256));  // keep 256 workgroups
+        if (direct_prime_bytes((uint32_t)p, lpw) > P.lds_budget) return false;
+        void* wt = alloc(p * sizeof(cpx<float>));
+        hip_check(gen_twiddles(P_F32, wt, p, p, s), "direct twiddles");
+        Op op;
+        op.kind = OP_DIRECT;
+        op.src = src;
+        op.src_off = src_off;
+        op.dst = dst;
+        op.dst_off = dst_off;
+        op.prec = P_F32;
+        op.a = base_args(ls, lay, lay, inverse, inverse, scale);
+        op.n = (uint32_t)p;
+        op.m = lpw;
+        op.ld_lf = op.st_lf = lf;
+        op.filt = wt;
+        op.desc = "direct N=" + std::to_string(p) + " lines=" + std::to_string(ls.lines) + " lpw=" +
+                  std::to_string(lpw) + (lf ? " LF" : "");
+        P.ops.push_back(op);
+        return true;
+    }
+
     // Four-step twiddle table W_n^t = hi[t >> sh] * lo[t & mask], t < n.
     struct PostTable {
         const double2* hi;
@@ -1042,6 +1080,8 @@ struct Builder {
                 return used;
             fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "no on-chip split for length " + std::to_string(n));
         }
+        if (kind == MIFFT_AXIS_RADER && direct_prime_op(n, ls, lay, src, src_off, dst, dst_off, inverse, scale))
+            return 0;
         if (kind == MIFFT_AXIS_RADER) {
             const uint64_t m = rader_conv_len(n);
             uint32_t lpw = 0;
@@ -1161,6 +1201,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                     const uint64_t O = total / (S * n);
                     const bool last = k + 1 == cfg.naxes;
                     const int dst = (!pp || (cfg.naxes - 1 - k) % 2 == 0) ? B_OUT : B_WS;
+                    B.cur_axis = axis;
                     ws = std::max(ws, scratch + B.axis_ops(n, P->axes[k].kind, S, O, src, 0, dst, 0,
                                                            scratch, inverse, last ? scale : 1.0));
                     src = dst;
@@ -1212,6 +1253,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 if (!(smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false, s0, LD_REAL,
                                            ST_PACKED))) {
                     elem_op(OP_REAL_WIDEN, B_IN, 0, B_WS, 0, total, 1.0, "real widen");
+                    B.cur_axis = 0;
                     ws = full_bytes + B.axis_ops(N0, P->axes[0].kind, 1, outer, B_WS, 0, B_WS, 0, full_bytes,
                                                  false, 1.0);
                     elem_op(OP_PACK, B_WS, 0, B_OUT, 0, outer, s0, "pack");
@@ -1220,6 +1262,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 for (int axis = 1; axis < cfg.rank; axis++) {
                     const uint64_t n = cfg.shape[axis];
                     const uint64_t O = packed / (S * n);
+                    B.cur_axis = axis;
                     ws = std::max(ws, B.axis_ops(n, P->axes[axis].kind, S, O, B_OUT, 0, B_OUT, 0, 0,
                                                  false, axis + 1 == cfg.rank ? scale : 1.0));
                     S *= n;
@@ -1233,6 +1276,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                     for (int axis = 1; axis < cfg.rank; axis++) {
                         const uint64_t n = cfg.shape[axis];
                         const uint64_t O = packed / (S * n);
+                        B.cur_axis = axis;
                         ws = std::max(ws, B.axis_ops(n, P->axes[axis].kind, S, O, src, src_off, B_WS,
                                                      0, pk_bytes, true, 1.0));
                         src = B_WS;
@@ -1246,6 +1290,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                     // full complex lines after the packed copy of the outer passes
                     const uint64_t X = cfg.rank > 1 ? pk_bytes : 0;
                     elem_op(OP_HERM_EXTEND, src, src_off, B_WS, X, outer, 1.0, "hermitian extend");
+                    B.cur_axis = 0;
                     ws = std::max(ws, X + full_bytes +
                                           B.axis_ops(N0, P->axes[0].kind, 1, outer, B_WS, X, B_WS, X,
                                                      X + full_bytes, true, 1.0));
@@ -1295,6 +1340,13 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             c.io.out = ptr(op.dst, op.dst_off);
             hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
                       "convolution kernel");
+            break;
+        }
+        case OP_DIRECT: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_direct_prime(a, op.n, op.m, op.ld_lf, op.filt, s), "direct prime kernel");
             break;
         }
         case OP_SMALL_VOLUME: {
