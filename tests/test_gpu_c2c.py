@@ -258,3 +258,39 @@ def test_high_precision_full_size(torch_cuda, shape, prec):
     e = oracle.rel_l2(y, ref)
     print(f"{prec} {shape} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
     assert e <= (1e-13 if prec == "f64" else 1e-11)
+
+
+# fuse_long_axes off (FftTuning::fuse_long_axes, tuning.rs:159-165): axes too
+# long for one workgroup run as one Stockham pass per radix over HBM instead
+# of the four-step split (axis_plan.rs:3540-3665).
+@pytest.mark.parametrize("shape,batch,prec", [([65536], 3, "f32"), ([59049], 1, "f32"), ([4, 65536], 1, "f32"),
+                                              ([1 << 18], 1, "f32"), ([65536], 2, "f64"), ([40000], 1, "df64")])
+@pytest.mark.parametrize("direction", [FftDirection.Forward, FftDirection.Inverse])
+def test_unfused_long_axes(torch_cuda, shape, batch, prec, direction):
+    import torch
+    cnt = int(np.prod(shape)) * batch
+    tuning = FftTuning().with_fuse_long_axes(False)
+    cfg = FftConfig.new_nd(shape).with_batch(batch).with_direction(direction).with_tuning(tuning)
+    inverse = direction == FftDirection.Inverse
+    if prec == "f32":
+        x = seeded_complex(900 + cnt, cnt)
+        plan, y = run_c2c(cfg, x, in_place=batch > 1)
+        assert "stockham" in plan.describe(), plan.describe()
+        check_f32(y, c2c_expected(x, shape, batch=batch, inverse=inverse), tag=f"unfused {shape} [{plan.describe()}]")
+        return
+    x = _f64_input(51, cnt)
+    if prec == "f64":
+        plan = FftPlan.c2c(cfg.with_precision(FftPrecision.F64))
+        d = to_dev(x)
+    else:
+        plan = FftPlan.c2c(cfg.with_precision(FftPrecision.Df64))
+        hilo = oracle.df64_split(x.view(np.float64))
+        x = oracle.df64_join(hilo).view(np.complex128)
+        d = to_dev(hilo)
+    assert "stockham" in plan.describe(), plan.describe()
+    plan.execute_in_place(d)
+    torch.cuda.synchronize()
+    y = from_dev(d, np.complex128) if prec == "f64" else oracle.df64_join(from_dev(d, np.float32)).view(np.complex128)
+    e = oracle.rel_l2(y, c2c_expected(x, shape, batch=batch, inverse=inverse))
+    print(f"unfused {prec} {shape} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= (1e-13 if prec == "f64" else 1e-11)

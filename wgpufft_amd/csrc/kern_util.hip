@@ -382,6 +382,82 @@ hipError_t fill_normal_f32(float* out, uint64_t n, uint64_t seed, hipStream_t s)
     return hipGetLastError();
 }
 
+// One Stockham radix-R stage over HBM, out of place (the reference's
+// per-radix pass for axes too long for one workgroup when fuse_long_axes is
+// off, src/runtime/axis_plan.rs:3540-3665).  Butterfly j < N/R of line
+// (i < S, o) reads points j + m N/R at stride S, multiplies point m by
+// W_{L R}^{(j mod L) m} from the stage-twiddle table, and writes point m to
+// (j - j mod L) R + (j mod L) + m L.  On a contiguous axis adjacent threads take
+// adjacent butterflies of one line; on a strided axis, adjacent lines.
+template <class Fmt, int R>
+__device__ __forceinline__ void stockham_butterfly(const LineArgs& a, const StockhamStage& st, uint32_t line,
+                                                   uint32_t j) {
+    using T = typename Fmt::T;
+    const LineBase b = line_base(a, line);
+    const uint32_t nr = st.n / R;
+    cpx<T> w[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) w[m] = load_point<Fmt, LD_COMPLEX>(a, b.in, j + m * nr, st.n);
+    const uint32_t k = fdiv(j, st.dl);
+    const uint32_t kk = j - k * st.L;
+    if (st.L > 1) {
+#pragma unroll
+        for (int m = 1; m < R; m++) w[m] = cmul(w[m], ld_tw<T>(a.tw, st.twoff + (m - 1) * st.L + kk));
+    }
+    Dft<R, T>::run(w);
+    const uint32_t ob = k * st.L * R + kk;
+#pragma unroll
+    for (int m = 0; m < R; m++) store_point<Fmt, ST_COMPLEX, false>(a, b.out, b.j, ob + m * st.L, st.n, w[m]);
+}
+
+template <class Fmt>
+__global__ void __launch_bounds__(256) k_stockham_pass(LineArgs a, StockhamStage st) {
+    const uint32_t nr = st.n / st.R;
+    const uint64_t total = (uint64_t)a.lines * nr;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t line, j;
+        if (st.lf) {
+            j = (uint32_t)(t / a.lines);
+            line = (uint32_t)(t - (uint64_t)j * a.lines);
+        } else {
+            line = (uint32_t)(t / nr);
+            j = (uint32_t)(t - (uint64_t)line * nr);
+        }
+        switch (st.R) {
+#define MIFFT_SCASE(RR) \
+    case RR: stockham_butterfly<Fmt, RR>(a, st, line, j); break;
+            MIFFT_SCASE(2)
+            MIFFT_SCASE(3)
+            MIFFT_SCASE(4)
+            MIFFT_SCASE(5)
+            MIFFT_SCASE(6)
+            MIFFT_SCASE(7)
+            MIFFT_SCASE(8)
+            MIFFT_SCASE(9)
+            MIFFT_SCASE(10)
+            MIFFT_SCASE(11)
+            MIFFT_SCASE(12)
+            MIFFT_SCASE(13)
+            MIFFT_SCASE(14)
+            MIFFT_SCASE(15)
+            MIFFT_SCASE(16)
+#undef MIFFT_SCASE
+            default: break;
+        }
+    }
+}
+
+hipError_t launch_stockham_pass(int prec, const LineArgs& a, const StockhamStage& st, hipStream_t s) {
+    const uint64_t total = (uint64_t)a.lines * (st.n / st.R);
+    switch (prec) {
+        case P_F32: k_stockham_pass<FmtF32><<<grid_for(total), 256, 0, s>>>(a, st); break;
+        case P_F64: k_stockham_pass<FmtF64><<<grid_for(total), 256, 0, s>>>(a, st); break;
+        default: k_stockham_pass<FmtDF64><<<grid_for(total), 256, 0, s>>>(a, st); break;
+    }
+    return hipGetLastError();
+}
+
 // Logical-view copies: element (b, i) of a batch of `per_batch` elements moves
 // from src[s.off + b s.sb + i s.se] to dst[d.off + b d.sb + i d.se] (elements
 // of E).  Gathers a strided caller layout into a contiguous staging array and

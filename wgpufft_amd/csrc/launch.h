@@ -98,6 +98,15 @@ hipError_t launch_chirp_post(int prec, const LineArgs& a, const void* chirp, uin
 // k <= n0/2 with a scale, Hermitian extension of packed lines, real part with a
 // scale.  Lines are contiguous.
 hipError_t launch_real_widen(const void* in, void* out, uint64_t count, hipStream_t s);
+// One Stockham radix stage over HBM (kern_util.hip): radix R, L = product of
+// the earlier radices, twiddles at twoff in the stage-twiddle table a.tw; lf:
+// adjacent threads take adjacent lines (strided axes).
+struct StockhamStage {
+    uint32_t n, R, L, twoff, lf;
+    DivMagic dl;  // divide by L
+};
+hipError_t launch_stockham_pass(int prec, const LineArgs& a, const StockhamStage& st, hipStream_t s);
+
 // Small-volume kernel (kern_small.hip): all axes of an f32 volume of
 // elems <= 4096 points per workgroup, axis a of length sc[a].n at stride S[a].
 constexpr int kSmallVolMaxRank = 8;  // MIFFT_MAX_RANK

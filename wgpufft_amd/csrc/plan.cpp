@@ -964,6 +964,48 @@ struct Builder {
         return true;
     }
 
+    // Long axis as one Stockham pass per radix over HBM (fuse_long_axes off,
+    // or no on-chip split): pass 1 reads the source, the middle passes
+    // ping-pong between two workspace copies of the array, the last writes
+    // the destination, so no pass reads what it writes.  Returns the
+    // workspace bytes used beyond ws_off.
+    uint64_t stockham_passes(uint64_t n, uint64_t S, uint64_t O, int src, uint64_t src_off, int dst,
+                             uint64_t dst_off, uint64_t ws_off, bool inverse, double scale) {
+        if (n > 0xFFFFFFFFull) fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "length beyond 32 bits");
+        const std::vector<int> rad = lds_schedule(n);
+        const LineSet ls{(uint32_t)(S * O), (uint32_t)S, 1};
+        const Layout lay{1, 0, (int64_t)(S * n), (int64_t)S};
+        const bool lf = adjacent_lines(ls, lay);
+        const void* tw = stage_twiddles(P.arith, rad);
+        const uint64_t arr = S * O * n * elem_bytes(P.prec);
+        const int ns = (int)rad.size();
+        uint32_t L = 1, twoff = 0;
+        for (int k = 0; k < ns; k++) {
+            const bool first = k == 0, last = k + 1 == ns;
+            // ws regions: A at ws_off, B at ws_off + arr; pass k writes A for even k, B for odd k
+            const int from = first ? src : B_WS;
+            const uint64_t from_off = first ? src_off : ws_off + ((k - 1) % 2 ? arr : 0);
+            const int to = last ? dst : B_WS;
+            const uint64_t to_off = last ? dst_off : ws_off + (k % 2 ? arr : 0);
+            Op op;
+            op.kind = OP_STOCKHAM;
+            op.src = from;
+            op.src_off = from_off;
+            op.dst = to;
+            op.dst_off = to_off;
+            op.prec = P.prec;
+            op.a = base_args(ls, lay, lay, first && inverse, last && inverse, last ? scale : 1.0);
+            op.a.tw = tw;
+            op.st = StockhamStage{(uint32_t)n, (uint32_t)rad[k], L, twoff, lf ? 1u : 0u, make_magic(L)};
+            op.desc = "stockham N=" + std::to_string(n) + " R=" + std::to_string(rad[k]) + " L=" + std::to_string(L) +
+                      " lines=" + std::to_string(ls.lines);
+            P.ops.push_back(op);
+            if (k >= 1) twoff += (uint32_t)((rad[k] - 1) * L);
+            L *= (uint32_t)rad[k];
+        }
+        return ns == 1 ? 0 : (ns == 2 ? arr : 2 * arr);
+    }
+
     // Four-step twiddle table W_n^t = hi[t >> sh] * lo[t & mask], t < n.
     struct PostTable {
         const double2* hi;
@@ -1076,9 +1118,10 @@ struct Builder {
                         LD_COMPLEX, ST_COMPLEX))
                 return 0;
             uint64_t used = 0;
-            if (four_step(n, S, O, src, src_off, dst, dst_off, ws_off, inverse, inverse, scale, &used))
+            if (P.cfg.tuning.fuse_long_axes &&
+                four_step(n, S, O, src, src_off, dst, dst_off, ws_off, inverse, inverse, scale, &used))
                 return used;
-            fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "no on-chip split for length " + std::to_string(n));
+            return stockham_passes(n, S, O, src, src_off, dst, dst_off, ws_off, inverse, scale);
         }
         if (kind == MIFFT_AXIS_RADER && direct_prime_op(n, ls, lay, src, src_off, dst, dst_off, inverse, scale))
             return 0;
@@ -1340,6 +1383,13 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             c.io.out = ptr(op.dst, op.dst_off);
             hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
                       "convolution kernel");
+            break;
+        }
+        case OP_STOCKHAM: {
+            LineArgs a = op.a;
+            a.in = ptr(op.src, op.src_off);
+            a.out = ptr(op.dst, op.dst_off);
+            hip_check(launch_stockham_pass(op.prec, a, op.st, s), "Stockham pass");
             break;
         }
         case OP_DIRECT: {

@@ -37,7 +37,8 @@ enum OpKind {
     OP_HERM_EXTEND, // general C2R: packed lines -> Hermitian-extended complex lines
     OP_REAL_PART,   // general C2R: complex lines -> real part
     OP_SMALL_VOLUME, // every axis of small f32 volumes, one workgroup per volume
-    OP_DIRECT        // direct DFT of short prime lines (n = p, m = lines per workgroup, filt = W_p table)
+    OP_DIRECT,       // direct DFT of short prime lines (n = p, m = lines per workgroup, filt = W_p table)
+    OP_STOCKHAM      // one radix stage of a long axis over HBM (st)
 };
 
 struct Plan;
@@ -51,6 +52,7 @@ struct Op {
     Sched sc{};
     ConvArgs conv{};
     SmallVolArgs sv{};  // OP_SMALL_VOLUME (volumes in count)
+    StockhamStage st{}; // OP_STOCKHAM
     const RegVariant* rv = nullptr;
     int ldm = LD_COMPLEX, stm = ST_COMPLEX, post = 0;
     Plan* sub = nullptr;     // OP_SUBPLAN
