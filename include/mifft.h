@@ -160,6 +160,13 @@ int32_t mifft_plan_supports_in_place(const mifft_plan* p);
  * of them ("reg N=4096 16x16x16 lines=2048; ..."), for diagnostics. */
 int32_t mifft_plan_launch_count(const mifft_plan* p);
 mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap);
+/* FftPlan::diagnostics (plan.rs:938-1004, diagnostics.rs:8-140) as a JSON
+ * object: route {transform, route, execution_kind, reason_codes,
+ * attempted_routes}, stages [{label, kind, route}] (one per launch),
+ * blockers [], buffer_requirements [{role, required_bytes, format}],
+ * active_tuning {FftTuning fields}, device_limits.  Writes at most cap bytes
+ * (NUL-terminated) and the full size including the NUL to *needed. */
+mifft_status mifft_plan_diagnostics(const mifft_plan* p, char* buf, size_t cap, size_t* needed);
 /* Text of launch `index` (0 <= index < mifft_plan_launch_count). */
 mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char* buf, size_t cap);
 /* Enqueues only launch `index` of one execution (no counterpart in the

@@ -407,6 +407,16 @@ class FftPlan:
         _check(self._lib.mifft_plan_launch_describe(self._h, int(index), buf, 1024))
         return buf.value.decode()
 
+    def diagnostics(self) -> dict:
+        """FftPlan::diagnostics (plan.rs:938-1004): route, stages, blockers,
+        buffer requirements, active tuning and device limits."""
+        import json
+        n = ctypes.c_size_t()
+        _check(self._lib.mifft_plan_diagnostics(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(self._lib.mifft_plan_diagnostics(self._h, buf, n.value, ctypes.byref(n)))
+        return json.loads(buf.value.decode())
+
     def execute_launch(self, index: int, input, output, workspace=None, stream=None) -> None:
         """Enqueue only launch `index` of one execution (per-kernel timing)."""
         ip, _ = _buf(input)

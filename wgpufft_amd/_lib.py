@@ -111,6 +111,7 @@ SIGNATURES = [
     ("mifft_plan_supports_in_place", i32, [vp]),
     ("mifft_plan_launch_count", i32, [vp]),
     ("mifft_plan_describe", i32, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+    ("mifft_plan_diagnostics", i32, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("mifft_plan_launch_describe", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t]),
     ("mifft_execute_launch", i32, [vp, vp, i32, vp, vp, vp]),
     ("mifft_execute", i32, [vp, vp, vp, u64, vp, u64]),

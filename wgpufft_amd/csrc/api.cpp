@@ -190,6 +190,19 @@ mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap) {
     return MIFFT_SUCCESS;
 }
 
+mifft_status mifft_plan_diagnostics(const mifft_plan* p, char* buf, size_t cap, size_t* needed) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    return guard([&] {
+        const std::string d = plan_diagnostics_json(*p->p);
+        if (needed) *needed = d.size() + 1;
+        if (buf && cap) {
+            const size_t n = std::min(cap - 1, d.size());
+            std::memcpy(buf, d.data(), n);
+            buf[n] = '\0';
+        }
+    });
+}
+
 static void check_sizes(const Plan& P, const void* in, uint64_t in_bytes, const void* out,
                         uint64_t out_bytes) {
     if (!in || !out) fail(MIFFT_ERR_NULL_ARGUMENT, "null buffer");

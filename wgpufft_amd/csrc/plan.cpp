@@ -1498,6 +1498,98 @@ std::string describe_plan(const Plan& p) {
     return d.str();
 }
 
+// FftPlan::diagnostics (src/plan.rs:938-1004, src/diagnostics.rs:8-140) as
+// a JSON document: route summary, one stage per launch (child plans
+// expanded), buffer requirements (input, output, workspace, helper tables) and
+// the active tuning.  A plan that exists has no blockers.
+static const char* op_kind_name(OpKind k) {
+    switch (k) {
+        case OP_REG: return "register-lines";
+        case OP_LDS: return "lds-lines";
+        case OP_RADER: return "rader";
+        case OP_BLUESTEIN: return "bluestein";
+        case OP_CHIRP_PRE: return "bluestein-chirp-pre";
+        case OP_POINTWISE: return "bluestein-filter";
+        case OP_CHIRP_POST: return "bluestein-chirp-post";
+        case OP_SUBPLAN: return "subplan";
+        case OP_SCALE_COPY: return "direct-dft";
+        case OP_REAL_WIDEN: return "real-widen";
+        case OP_PACK: return "real-pack";
+        case OP_HERM_EXTEND: return "hermitian-extend";
+        case OP_REAL_PART: return "real-part";
+        case OP_SMALL_VOLUME: return "small-volume";
+        case OP_DIRECT: return "direct-prime";
+        case OP_STOCKHAM: return "stockham-pass";
+    }
+    return "unknown";
+}
+
+static std::string json_str(const std::string& v) {
+    std::string o = "\"";
+    for (char c : v) {
+        if (c == '"' || c == '\\') o += '\\';
+        if ((unsigned char)c < 0x20) continue;
+        o += c;
+    }
+    return o + "\"";
+}
+
+static void diag_stages(const Plan& p, const std::string& route, std::ostringstream& d, bool& first) {
+    for (const Op& op : p.ops) {
+        if (op.kind == OP_SUBPLAN && op.sub) {
+            diag_stages(*op.sub, route, d, first);
+            continue;
+        }
+        d << (first ? "" : ",") << "{\"label\":" << json_str(op.desc) << ",\"kind\":\"" << op_kind_name(op.kind)
+          << "\",\"route\":" << json_str(route) << "}";
+        first = false;
+    }
+}
+
+static uint64_t table_bytes(const Plan& p) {
+    uint64_t b = 0;
+    for (const DevBuf& o : p.owned) b += o.bytes;
+    for (const auto& c : p.children) b += table_bytes(*c);
+    return b;
+}
+
+std::string plan_diagnostics_json(const Plan& p) {
+    static const char* routes[] = {"direct-dft", "mixed-radix", "rader", "bluestein", "axis-sequence"};
+    static const char* kinds[] = {"c2c", "r2c", "c2r"};
+    static const char* cfmt[] = {"complex-f32", "complex-f64", "complex-df64"};
+    const std::string route = (p.route >= 0 && p.route <= 4) ? routes[p.route] : "unknown";
+    const std::string cf = cfmt[p.prec];
+    const std::string in_fmt = p.kind == MIFFT_R2C ? "real-f32" : p.kind == MIFFT_C2R ? "packed-complex-f32" : cf;
+    const std::string out_fmt = p.kind == MIFFT_R2C ? "packed-complex-f32" : p.kind == MIFFT_C2R ? "real-f32" : cf;
+    std::ostringstream d;
+    d << "{\"route\":{\"transform\":\"" << kinds[p.kind] << "\",\"route\":" << json_str(route)
+      << ",\"execution_kind\":\"normal\",\"reason_codes\":[],\"attempted_routes\":[" << json_str(route) << "]},";
+    d << "\"stages\":[";
+    bool first = true;
+    diag_stages(p, route, d, first);
+    d << "],\"blockers\":[],";
+    d << "\"buffer_requirements\":[{\"role\":\"input\",\"required_bytes\":" << p.in_bytes << ",\"format\":\""
+      << in_fmt << "\"},{\"role\":\"output\",\"required_bytes\":" << p.out_bytes << ",\"format\":\"" << out_fmt
+      << "\"}";
+    if (p.ws_bytes)
+        d << ",{\"role\":\"workspace\",\"required_bytes\":" << p.ws_bytes << ",\"format\":\"" << cf << "\"}";
+    const uint64_t tb = table_bytes(p);
+    if (tb) d << ",{\"role\":\"helper:twiddle-luts-total\",\"required_bytes\":" << tb << ",\"format\":\"" << cf << "\"}";
+    d << "],";
+    const mifft_tuning& t = p.cfg.tuning;
+    d << "\"active_tuning\":{\"workgroup_size\":" << t.workgroup_size << ",\"fused_workgroup_size\":"
+      << t.fused_workgroup_size << ",\"rader_max_prime\":" << t.rader_max_prime << ",\"direct_max_prime\":"
+      << t.direct_max_prime << ",\"fuse_long_axes\":" << (t.fuse_long_axes ? "true" : "false")
+      << ",\"fuse_small_volumes\":" << (t.fuse_small_volumes ? "true" : "false")
+      << ",\"fused_min_convolution_length\":" << t.fused_min_convolution_length << ",\"force_rader_axes\":[";
+    for (int i = 0; i < t.n_force_rader_axes; i++) d << (i ? "," : "") << t.force_rader_axes[i];
+    d << "],\"force_bluestein_axes\":[";
+    for (int i = 0; i < t.n_force_bluestein_axes; i++) d << (i ? "," : "") << t.force_bluestein_axes[i];
+    d << "]},";
+    d << "\"device_limits\":{\"max_workgroup_lds_bytes\":" << kLdsBudget << ",\"device\":" << p.device << "}}";
+    return d.str();
+}
+
 // ------------------------------------------------------------- autotuner
 int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int count, int reps, hipStream_t s,
                       std::string* report) {

@@ -145,6 +145,8 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
 // autotune_sequence of one plan.
 int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report);
 std::string describe_plan(const Plan& p);
+// FftPlan::diagnostics as JSON.
+std::string plan_diagnostics_json(const Plan& p);
 
 // Host-side integer helpers (the reference's route-choice arithmetic).
 std::vector<uint64_t> factor_supported(uint64_t n, bool* ok);  // runtime/mod.rs:33-54
