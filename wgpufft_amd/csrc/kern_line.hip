@@ -110,6 +110,21 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 256, 16, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
     REGV(FmtDF64, P_DF64, 256, 16, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    // autotuner alternatives of the f64 three-pass passes: 8 and 32 lines per
+    // workgroup, and 32 threads per line (8 points each)
+    REGV(FmtF64, P_F64, 256, 16, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 32, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "8x8x4", 8, 8, 4),
+    REGV(FmtF64, P_F64, 256, 16, 8, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 32, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 32, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4", 8, 8, 4),
+    REGV(FmtDF64, P_DF64, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 32, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    // 32 lines per workgroup for the 256^3 axis passes
+    REGV(FmtF64, P_F64, 256, 16, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF64, P_F64, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtDF64, P_DF64, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
     // f64 / df64 4096-point lines: the two passes of the 2^24 four-step
     // (columns with twiddle on store; rows with transposed store), strided
     // and contiguous axes; component-split LDS exchange at four lines
