@@ -169,6 +169,9 @@ static const RegVariant kRegistry[] = {
     REGP(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, "8x8x8x8", 8, 8, 8, 8),
     REGP(FmtF32, P_F32, 2048, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 2, "8x8x8x4", 8, 8, 8, 4),
     REGP(FmtF32, P_F32, 1024, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
+    // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
+    REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
+    REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),
     REGV(FmtF64, P_F64, 4096, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 256, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
