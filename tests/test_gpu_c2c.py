@@ -128,7 +128,7 @@ def test_normalizations(torch_cuda, norm, direction):
     check_f32(y, ref, tag=f"norm={norm.name} {direction.name}")
 
 
-@pytest.mark.parametrize("shape", [[512, 256], [720, 480], [65536], [4241], [1009, 4], [96, 10007]])
+@pytest.mark.parametrize("shape", [[512, 256], [720, 480], [65536], [4241], [65537], [1009, 4], [96, 10007]])
 def test_in_place_matches_out_of_place(torch_cuda, shape):
     """tests/gpu_in_place.rs: in-place equals out-of-place; here bit-exact (same kernels)."""
     cnt = int(np.prod(shape))
@@ -294,3 +294,39 @@ def test_unfused_long_axes(torch_cuda, shape, batch, prec, direction):
     e = oracle.rel_l2(y, c2c_expected(x, shape, batch=batch, inverse=inverse))
     print(f"unfused {prec} {shape} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
     assert e <= (1e-13 if prec == "f64" else 1e-11)
+
+
+# tests/gpu_accuracy.rs (:94-104): 4096, 3000, Rader 2999, Bluestein 3256 and a
+# batched 2D, against FP64 with the reference's accuracy envelope.
+@pytest.mark.parametrize("shape,batch,force", [([4096], 16, None), ([3000], 16, None), ([2999], 16, None),
+                                               ([3256], 16, None), ([3256], 8, "bluestein"), ([120, 48], 6, None)])
+def test_accuracy_envelope(torch_cuda, shape, batch, force):
+    n = int(np.prod(shape))
+    x = seeded_complex(1234 + n, n * batch)
+    cfg = FftConfig.new_nd(shape).with_batch(batch)
+    if force:
+        cfg = cfg.with_tuning(FftTuning().with_force_bluestein_axes([0]))
+    plan, y = run_c2c(cfg, x)
+    ref = c2c_expected(x, shape, batch=batch)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"{shape} b={batch} {force} rms_rel={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 < 5e-7, e2   # the reference's RMS-rel bound for non-fused routes
+    assert em < 5e-7, em   # and its max-rel bound
+
+
+def test_large_grid_dispatch(torch_cuda):
+    """tests/gpu_dispatch_split.rs: a launch of more than 65535 workgroups (2^22
+    + 64 lines of 64 points, 64 lines per workgroup).  Every line is one of 64
+    seeded patterns; each must equal the FFT of its pattern bit for bit."""
+    import torch
+    n, lines = 64, (1 << 22) + 64
+    pat = seeded_complex(77, n * 64).reshape(64, n)
+    d = torch.from_numpy(pat.view(np.uint8).reshape(64, n * 8)).to("cuda").repeat(lines // 64, 1).reshape(-1)
+    plan = FftPlan.c2c(FftConfig.new(n).with_batch(lines))
+    plan.execute_in_place(d)
+    torch.cuda.synchronize()
+    y = d.view(torch.int32).reshape(lines // 64, 64 * n * 2)
+    first = y[0:1]
+    assert bool((y == first).all()), "a repeated line differs"
+    got = from_dev(d[: 64 * n * 8], np.complex64).reshape(64, n)
+    check_f32(got.reshape(-1), c2c_expected(pat.reshape(-1), [n], batch=64), tag=f"grid [{plan.describe()}]")
