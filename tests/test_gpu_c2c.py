@@ -330,3 +330,17 @@ def test_large_grid_dispatch(torch_cuda):
     assert bool((y == first).all()), "a repeated line differs"
     got = from_dev(d[: 64 * n * 8], np.complex64).reshape(64, n)
     check_f32(got.reshape(-1), c2c_expected(pat.reshape(-1), [n], batch=64), tag=f"grid [{plan.describe()}]")
+
+
+# tests/gpu_long_axes.rs: long 1D axes (two- and three-pass splits) and 1024^2,
+# against FP64 with its envelope (max-rel < 2e-6, RMS-rel < 5e-7).
+@pytest.mark.parametrize("shape,batch", [([8192], 4), ([1 << 17], 2), ([1 << 19], 1), ([1 << 21], 1),
+                                         ([1024, 1024], 1)])
+def test_long_axes_envelope(torch_cuda, shape, batch):
+    n = int(np.prod(shape))
+    x = seeded_complex(4321 + n, n * batch)
+    plan, y = run_c2c(FftConfig.new_nd(shape).with_batch(batch), x)
+    ref = c2c_expected(x, shape, batch=batch)
+    e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
+    print(f"{shape} b={batch} rms_rel={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
+    assert e2 < 5e-7 and em < 2e-6, (e2, em)
