@@ -1,0 +1,229 @@
+"""CPU simulations of the kernels' index maps against numpy's FFT (the
+schedule-simulation tests the reference keeps for its generated kernels:
+register_fft.rs:1119-1228, axis_plan.rs:4246-4306).  Each restates, in numpy,
+the addressing a HIP kernel uses, so that a wrong index, sign or twiddle
+exponent in that map fails here without a GPU:
+
+* Stockham stages with the stage-twiddle table layout (RegFft, lds_stage,
+  k_stockham_pass): butterfly j reads j + m N/R, twiddles W_{LR}^{(j mod L) m}
+  at table entry twoff(s) + (m-1) L + (j mod L), writes (j - j mod L) R +
+  (j mod L) + m L;
+* the two-pass four-step and the three-pass split with the line
+  decomposition i = g % S1, j = (g / S1) % S2, o = g / (S1 S2) and the planner's
+  strides (plan.cpp four_step / three_pass);
+* the twiddle recurrence of a butterfly's outputs (PostSeq);
+* the direct prime DFT's output pairs (k_direct_prime);
+* the packed R2C store and the Hermitian-extended C2R load;
+* the XCD tile remap (a bijection for every grid size);
+* the small-volume gather order (k_small_volume).
+"""
+import numpy as np
+import pytest
+
+rng = np.random.default_rng(7)
+
+
+def crand(n):
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def W(N, e):
+    # exact integer phase reduction before the exponential (large exponents
+    # would otherwise carry the angle's own rounding)
+    return np.exp(-2j * np.pi * (np.asarray(e, dtype=np.int64) % N) / N)
+
+
+def stage_table(rad):
+    """Stage-twiddle table: for s >= 1 a block of (R_s - 1) L_s entries,
+    entry (m-1) L_s + k = W_{L_s R_s}^{k m}."""
+    tab, L = [], rad[0]
+    for R in rad[1:]:
+        for m in range(1, R):
+            for k in range(L):
+                tab.append(W(L * R, k * m))
+        L *= R
+    return np.array(tab, dtype=complex)
+
+
+def twoff(rad, s):
+    o, L = 0, rad[0]
+    for i in range(1, s):
+        o += (rad[i] - 1) * L
+        L *= rad[i]
+    return o
+
+
+def stockham(x, rad, tab):
+    N = len(x)
+    L = 1
+    for s, R in enumerate(rad):
+        nr = N // R
+        y = np.empty_like(x)
+        for j in range(nr):
+            k = j % L
+            w = np.array([x[j + m * nr] for m in range(R)])
+            if L > 1:
+                for m in range(1, R):
+                    w[m] *= tab[twoff(rad, s) + (m - 1) * L + k]
+            w = np.fft.fft(w)
+            for m in range(R):
+                y[(j - k) * R + k + m * L] = w[m]
+        x = y
+        L *= R
+    return x
+
+
+@pytest.mark.parametrize("rad", [[16, 16, 8], [8, 8, 8, 4], [15, 8], [7, 7, 7], [13, 11, 7], [2, 9, 7],
+                                 [16, 4, 5, 5], [16, 16, 16], [3], [9, 9, 5, 5]])
+def test_stockham_stages(rad):
+    N = int(np.prod(rad))
+    x = crand(N)
+    y = stockham(x, rad, stage_table(rad))
+    assert np.abs(y - np.fft.fft(x)).max() <= 1e-9 * np.abs(np.fft.fft(x)).max()
+    assert len(stage_table(rad)) == N - rad[0]
+
+
+def line_base(g, S1, S2, si, sj, so):
+    i = g % S1
+    r = g // S1
+    j = r % S2
+    o = r // S2
+    return i * si + j * sj + o * so, j
+
+
+def run_lines(src, dst, n, lines, S1, S2, lin, lout, post=None):
+    """One line pass: line g of n points at lin = (si, sj, so, sn) -> lout,
+    optional four-step twiddle W_P^{j k} on store (post = P)."""
+    for g in range(lines):
+        bi, j = line_base(g, S1, S2, *lin[:3])
+        bo, _ = line_base(g, S1, S2, *lout[:3])
+        v = np.fft.fft(src[bi + lin[3] * np.arange(n)])
+        if post:
+            v = v * W(post, j * np.arange(n))
+        dst[bo + lout[3] * np.arange(n)] = v
+
+
+@pytest.mark.parametrize("n1,n2,S,O", [(8, 16, 1, 3), (16, 8, 4, 2), (4, 32, 1, 1), (12, 10, 3, 2)])
+def test_four_step_two_pass(n1, n2, S, O):
+    """plan.cpp four_step: pass 1 transforms n1-point columns (points at stride
+    S n2) with W_n^{q2 k1} on store; pass 2 transforms n2-point rows and stores
+    transposed (k = k1 + n1 k2)."""
+    n = n1 * n2
+    x = crand(S * n * O)
+    ws = np.zeros_like(x)
+    out = np.zeros_like(x)
+    run_lines(x, ws, n1, S * O * n2, S, n2, (1, S, S * n, S * n2), (1, S, S * n, S * n2), post=n)
+    run_lines(ws, out, n2, S * O * n1, S, n1, (1, S * n2, S * n, S), (1, S, S * n, S * n1))
+    ref = np.fft.fft(x.reshape(O, n, S), axis=1).reshape(-1)
+    assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n3,n1,n2,O", [(4, 8, 16, 3), (16, 4, 8, 1), (8, 8, 8, 2)])
+def test_three_pass_split(n3, n1, n2, O):
+    """plan.cpp three_pass with the planner's line sets and layouts."""
+    n, m = n3 * n1 * n2, n1 * n2
+    x = crand(n * O)
+    ws = np.zeros_like(x)
+    ws2 = np.zeros_like(x)
+    out = np.zeros_like(x)
+    run_lines(x, ws, n3, O * m, 1, m, (1, 1, n, m), (1, 1, n, m), post=n)
+    run_lines(ws, ws2, n2, O * n3 * n1, 1, n1, (1, 1, m, n1), (1, 1, m, n1), post=m)
+    run_lines(ws2, out, n1, O * n3 * n2, n3, n2, (m, n1, n, 1), (1, n3, n, n3 * n2))
+    ref = np.fft.fft(x.reshape(O, n), axis=1).reshape(-1)
+    assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_post_twiddle_recurrence():
+    """PostSeq: W_P^{j (p0 + m L)} for m < R from W_P^{j p0} and the step
+    W_P^{j L} by repeated multiplication, within ~R ulp of the direct values."""
+    P, L, R = 1 << 24, 4096, 16
+    for j, p0 in [(1, 0), (65535, 4095), (12345, 17), (65535, 1)]:
+        w, step = W(P, j * p0), W(P, j * L)
+        for m in range(R):
+            assert abs(w - W(P, j * (p0 + m * L))) < 4 * R * 2.2e-16
+            w = w * step
+
+
+@pytest.mark.parametrize("p", [47, 59, 83, 103, 107])
+def test_direct_prime_pairs(p):
+    """k_direct_prime: X_k = x_0 + sum s_n cos - i sum d_n sin and X_{p-k} with
+    the opposite sign, s_n = x_n + x_{p-n}, d_n = x_n - x_{p-n}."""
+    x = crand(p)
+    X = np.zeros(p, complex)
+    X[0] = x.sum()
+    h = (p - 1) // 2
+    for k in range(1, h + 1):
+        A = B = 0
+        t = 0
+        for n in range(1, h + 1):
+            t = (t + k) % p
+            w = W(p, t)
+            c, sn = w.real, -w.imag
+            A += (x[n] + x[p - n]) * c
+            B += (x[n] - x[p - n]) * sn
+        X[k] = x[0] + A - 1j * B
+        X[p - k] = x[0] + A + 1j * B
+    assert np.abs(X - np.fft.fft(x)).max() <= 1e-9 * np.abs(X).max()
+
+
+@pytest.mark.parametrize("n", [16, 15, 2048])
+def test_real_modes(n):
+    """LD_REAL + ST_PACKED stores bins k <= n/2; LD_HERMITIAN extends the packed
+    half (bin p > n/2 = conj of bin n - p; the imaginary part of bins 0 and n/2
+    dropped)."""
+    x = rng.standard_normal(n)
+    full = np.fft.fft(x)
+    packed = full[: n // 2 + 1]
+    assert np.allclose(packed, np.fft.rfft(x))
+    ext = np.empty(n, complex)
+    for p in range(n):
+        up = p > n // 2
+        v = packed[n - p if up else p]
+        im = 0.0 if (p == 0 or 2 * p == n) else (-v.imag if up else v.imag)
+        ext[p] = v.real + 1j * im
+    assert np.allclose(np.fft.ifft(ext).imag, 0, atol=1e-12)
+    assert np.allclose(np.fft.ifft(ext).real, x)
+
+
+def test_xcd_remap_bijective():
+    """xcd_remap: XCD x = b mod 8 holds q + (x < r) blocks (n = 8q + r), given
+    the contiguous tiles x q + min(x, r) + b / 8."""
+    for n in list(range(1, 300)) + [1025, 4096, 65537]:
+        q, r = n >> 3, n & 7
+        tiles = sorted((b & 7) * q + min(b & 7, r) + (b >> 3) for b in range(n))
+        assert tiles == list(range(n))
+
+
+@pytest.mark.parametrize("dims", [[16, 16], [12, 10, 9], [4, 256], [2, 2, 2, 2, 2, 2, 2, 2]])
+def test_small_volume_gather(dims):
+    """k_small_volume: per axis a (length n, stride S = prod(dims[:a])), thread
+    t takes line l and point p (p fastest on axis 0, l fastest otherwise), line
+    l = (i, o) with i = l % S, point at o S n + i + p S.  Every element is
+    gathered exactly once and the axis FFTs give the full DFT."""
+    vol = crand(int(np.prod(dims)))
+    x0 = vol.copy()
+    e = vol.size
+    S = 1
+    for n in dims:
+        lines = e // n
+        buf = np.empty((lines, n), complex)
+        seen = np.zeros(e, int)
+        for t in range(e):
+            if S == 1:
+                l, p = t // n, t % n
+            else:
+                p, l = t // lines, t % lines
+            o, i = l // S, l % S
+            idx = o * S * n + i + p * S
+            buf[l, p] = vol[idx]
+            seen[idx] += 1
+        assert (seen == 1).all()
+        res = np.fft.fft(buf, axis=1)
+        for t in range(e):
+            l, p = t // n, t % n
+            o, i = l // S, l % S
+            vol[o * S * n + i + p * S] = res[l, p]
+        S *= n
+    # axis 0 is the fastest-varying index: numpy's C order reversed
+    ref = np.fft.fftn(x0.reshape(dims[::-1])).reshape(-1)
+    assert np.abs(vol - ref).max() <= 1e-9 * np.abs(ref).max()
