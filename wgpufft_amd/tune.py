@@ -1,6 +1,6 @@
 """Autotune the configurations of the benchmark workloads and save the table.
 
-    python -m wgpufft_amd.tune [--out wgpufft_amd/tuning/mi355x.txt] [--reps 40]
+    python -m wgpufft_amd.tune [--out wgpufft_amd/tuning/mi355x.txt] [--reps 40] [--only TEXT]
 
 Runs mifft_autotune_sequence on each benchmark workload below, as the
 benchmark runs it (every candidate kernel of each line-pass shape, the whole
@@ -45,6 +45,9 @@ def main(argv=None) -> int:
     ap.add_argument("--fresh", action="store_true",
                     help="start from an empty table instead of the one at --out (whose entries the "
                          "tuning does not cover, such as split choices, are otherwise kept)")
+    ap.add_argument("--only", default=None,
+                    help="tune only the workloads whose name contains this text (the table's other "
+                         "entries are kept)")
     args = ap.parse_args(argv)
     import torch  # device runtime set-up, as in every other entry point
 
@@ -54,6 +57,8 @@ def main(argv=None) -> int:
     if not args.fresh and os.path.exists(args.out):
         load_tuning(args.out)
     for name, jobs in workloads():
+        if args.only is not None and args.only not in name:
+            continue
         n, report = autotune_sequence(jobs, reps=args.reps)
         print(f"== {name}: {n} shape(s) tuned", flush=True)
         print(report, end="", flush=True)
