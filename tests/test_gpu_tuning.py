@@ -161,3 +161,32 @@ def test_forced_split_matches_oracle(torch_cuda, n, batch, split):
         e = oracle.rel_l2(y, want)
         print(f"n={n} b={batch} split={split} inverse={inverse} rel_l2={e:.3e} [{desc}]", flush=True)
         assert e <= 1e-6, (split, inverse, e)
+
+
+# Persistent pipelined R2C / C2R axis-0 passes (real load + packed store,
+# Hermitian-extended load + real store), more rows than workgroups so that
+# each workgroup loops over several tiles, against the FP64 oracle.
+@pytest.mark.parametrize("lines,minb", [(1, 2), (2, 2)])
+def test_forced_pipelined_real_passes(torch_cuda, lines, minb):
+    import torch
+    n, batch = 2048, 1500
+    lb = (batch - 1).bit_length()
+    xr = oracle.normal_f32(94 + lines, n * batch)
+    want = oracle.r2c_nd(xr, [n], batch=batch, nthreads=8)
+    W.parse_tuning(
+        f"f32|n{n}|lf00|m1/1|p0|pair0|lines2^{lb} = reg:f32:n{n}:t128:l{lines}:lf00:m1/1:p0:v1:16x16x8:pipe{minb}\n"
+        f"f32|n{n}|lf00|m2/2|p0|pair0|lines2^{lb} = reg:f32:n{n}:t128:l{lines}:lf00:m2/2:p0:v1:16x16x8:pipe{minb}\n")
+    r2c = FftPlan.r2c(FftConfig.new(n).with_batch(batch))
+    c2r = FftPlan.c2r(FftConfig.inverse(n).with_batch(batch))
+    assert "pipe" in r2c.describe() and "pipe" in c2r.describe(), (r2c.describe(), c2r.describe())
+    d = to_dev(xr)
+    pk = empty_dev(r2c.required_output_buffer_size_bytes())
+    back = empty_dev(xr.nbytes)
+    r2c.execute(d, pk)
+    c2r.execute(pk, back)
+    torch.cuda.synchronize()
+    e1 = oracle.rel_l2(from_dev(pk, np.complex64), want)
+    want_back = oracle.c2r_nd(want, [n], batch=batch, nthreads=8)
+    e2 = oracle.rel_l2(from_dev(back, np.float32), want_back)
+    print(f"pipelined real l{lines} pipe{minb}: r2c rel_l2={e1:.3e} c2r rel_l2={e2:.3e}", flush=True)
+    assert e1 <= 1e-6 and e2 <= 1e-6

@@ -169,6 +169,11 @@ static const RegVariant kRegistry[] = {
     REGP(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, "8x8x8x8", 8, 8, 8, 8),
     REGP(FmtF32, P_F32, 2048, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 2, "8x8x8x4", 8, 8, 8, 4),
     REGP(FmtF32, P_F32, 1024, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
+    // persistent pipelined forms of the 2048^2 real passes (axis 0)
+    REGP(FmtF32, P_F32, 2048, 128, 1, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
+    REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
+    REGP(FmtF32, P_F32, 2048, 128, 1, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
+    REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
     // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
     REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
     REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),
