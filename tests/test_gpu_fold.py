@@ -1,0 +1,92 @@
+"""GPU tests of the 2D fold (plan.cpp fold_2d): axis 0 fused with the first
+radix-2 stage of axis 1, then axis 1 as half-length lines over both halves.
+Every compiled fold variant, forced through the tuning table, in both
+directions, in place and out of place, against the FP64 oracle (relative
+L2 <= 1e-6) and against the unfolded plan of the same shape."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import c2c_expected, empty_dev, from_dev, seeded_complex, to_dev
+import wgpufft_amd as W
+from wgpufft_amd import FftConfig, FftPlan
+
+pytestmark = pytest.mark.gpu
+
+FOLD_VARIANTS = [
+    "reg:f32:n4096:t256:l2:lf00:m0/0:p0:v2:16x16x16:fold",
+    "reg:f32:n4096:t512:l2:lf00:m0/0:p0:v2:8x8x8x8:fold",
+    "reg:f32:n4096:t512:l2:lf00:m0/0:p0:v2:8x8x8x8:pipe1:fold",
+    "reg:f32:n4096:t512:l2:lf00:m0/0:p0:v2:8x8x8x8:pipe2:fold",
+    "reg:f32:n4096:t256:l2:lf00:m0/0:p0:v2:16x16x16:pipe1:fold",
+]
+
+
+@pytest.fixture(autouse=True)
+def _clean():
+    W.clear_tuning()
+    yield
+    W.clear_tuning()
+
+
+def fold_key(n0, lines):
+    return f"f32|n{n0}|lf00|m0/0|p0|pair0|lines2^{(lines - 1).bit_length()}|fold"
+
+
+def _run(cfg, x, in_place):
+    import torch
+    plan = FftPlan.c2c(cfg)
+    d = to_dev(x)
+    if in_place:
+        plan.execute_in_place(d)
+        out = d
+    else:
+        out = empty_dev(x.nbytes)
+        plan.execute(d, out)
+    torch.cuda.synchronize()
+    return plan, from_dev(out, np.complex64)
+
+
+@pytest.mark.parametrize("shape,batch", [([4096, 128], 3), ([4096, 2048], 1), ([4096, 8192], 1)])
+def test_fold_matches_oracle_and_unfolded(torch_cuda, shape, batch):
+    n0, n1 = shape
+    x = seeded_complex(500 + n1, n0 * n1 * batch)
+    for inverse in (False, True):
+        want = c2c_expected(x, shape, batch=batch, inverse=inverse)
+        cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+        W.clear_tuning()
+        plan, y = _run(cfg, x, in_place=inverse)
+        desc = plan.describe()
+        assert "fold M=%d" % n1 in desc, desc
+        e = oracle.rel_l2(y, want)
+        W.parse_tuning(f"fold|f32|n{n0}x{n1} = 0\n")
+        plain, y0 = _run(cfg, x, in_place=inverse)
+        assert "fold" not in plain.describe(), plain.describe()
+        e0 = oracle.rel_l2(y0, want)
+        d = oracle.rel_l2(y, y0)
+        print(f"{shape} b={batch} inverse={inverse} fold rel_l2={e:.3e} unfolded {e0:.3e} diff {d:.3e} [{desc}]",
+              flush=True)
+        assert e <= 1e-6 and e0 <= 1e-6 and d <= 1e-6
+
+
+@pytest.mark.parametrize("variant", FOLD_VARIANTS)
+def test_forced_fold_variants(torch_cuda, variant):
+    shape, batch = [4096, 256], 5  # 640 row pairs: persistent forms loop over several tiles
+    x = seeded_complex(77, 4096 * 256 * batch)
+    W.parse_tuning(f"{fold_key(4096, 2 * 128 * batch)} = {variant}\n")
+    for inverse in (False, True):
+        cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+        plan, y = _run(cfg, x, in_place=False)
+        desc = plan.describe()
+        assert "fold" in desc and ("pipe" in desc) == (":pipe" in variant), (variant, desc)
+        e = oracle.rel_l2(y, c2c_expected(x, shape, batch=batch, inverse=inverse))
+        print(f"{variant} inverse={inverse} rel_l2={e:.3e}", flush=True)
+        assert e <= 1e-6, (variant, inverse, e)
+
+
+def test_fold_is_off_where_it_does_not_apply(torch_cuda):
+    # odd second axis, f64, axes in the other order, a single axis of a 2D array
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 3])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_precision(W.FftPrecision.F64)).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([1, 0])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([0])).describe()

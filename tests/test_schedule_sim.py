@@ -133,6 +133,29 @@ def test_three_pass_split(n3, n1, n2, O):
     assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n0,n1,O", [(8, 8, 1), (4, 16, 3), (16, 6, 2), (6, 10, 1)])
+def test_fold_2d(n0, n1, O):
+    """plan.cpp fold_2d: pass 1 transforms row pairs (r, r + H) of axis 0
+    (pair q at line_base(q) with S1 = 1, S2 = H, the second row fold_in =
+    H n0 elements on) and stores y[r] + y[r + H] and W_M^r (y[r] - y[r + H]);
+    pass 2 runs H-point lines over both halves (S1 = n0 columns, S2 = 2
+    halves) and stores half c's bin k at row 2k + c."""
+    H = n1 // 2
+    x = crand(n0 * n1 * O)
+    ws = np.zeros_like(x)
+    out = np.zeros_like(x)
+    fold = H * n0
+    for q in range(H * O):  # the kernel's lines = 2 H O, pair q = g // 2
+        b, r = line_base(q, 1, H, 0, n0, n0 * n1)
+        ya = np.fft.fft(x[b + np.arange(n0)])
+        yb = np.fft.fft(x[b + fold + np.arange(n0)])
+        ws[b + np.arange(n0)] = ya + yb
+        ws[b + fold + np.arange(n0)] = W(n1, r) * (ya - yb)
+    run_lines(ws, out, H, n0 * 2 * O, n0, 2, (1, H * n0, n0 * n1, n0), (1, n0, n0 * n1, 2 * n0))
+    ref = np.fft.fft2(x.reshape(O, n1, n0), axes=(1, 2)).reshape(-1)
+    assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
 def test_post_twiddle_recurrence():
     """PostSeq: W_P^{j (p0 + m L)} for m < R from W_P^{j p0} and the step
     W_P^{j L} by repeated multiplication, within ~R ulp of the direct values."""

@@ -42,6 +42,9 @@ struct LineArgs {
     uint32_t post_shift, post_mask;
     double scale;
     uint32_t conj_in, conj_out;
+    // Fold passes: element offsets of the second line of each pair (the
+    // first line at line_base(g / 2)).
+    uint32_t fold_in, fold_out;
 };
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
@@ -255,8 +258,12 @@ __device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uin
 //   the LDS exchange then runs in VEC rounds through one slot per line group).
 //   PF: load every stage's twiddles into registers together with the data, so
 //   their latency overlaps the data loads instead of following each exchange.
+//   FOLD (with VEC 2, contiguous lines): the two lines of a thread are rows r
+//   and r + H of the next axis (M = 2H points, fold_in / fold_out apart),
+//   and the store applies that axis's first radix-2 stage, so that its pass
+//   runs M/2-point lines (half the tile of an M-point line).
 template <class Fmt, int N, int TPL, int LINES, bool LD_LF, bool ST_LF, int LDM, int STM, bool POST,
-          class RL, int VEC = 1, bool PF = false>
+          class RL, int VEC = 1, bool PF = false, bool FOLD = false>
 struct RegFft {
     using T = typename Fmt::T;
     static constexpr int NS = RL::n;
@@ -275,6 +282,7 @@ struct RegFft {
     static constexpr int NT = TPL * GROUPS;
     static constexpr int LSTR = lds_pad(N) + 1;
     static_assert(VEC == 1 || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
+    static_assert(!FOLD || (VEC == 2 && !LD_LF && !ST_LF && !POST), "fold pairs are contiguous lines");
     // Lines of 16-byte points whose exchange buffer would exceed 80 KiB go
     // through LDS one component at a time: half the space, twice the rounds
     // (four 4096-point f64 lines per workgroup instead of two).
@@ -404,7 +412,7 @@ struct RegFft {
         const Map mp = map_of<0>();
         const uint32_t g = g0 + mp.lg * VEC;
         const uint32_t gl = g < a.lines ? g : (a.lines - VEC);
-        LineBase b = line_base(a, gl);
+        LineBase b = line_base(a, FOLD ? gl >> 1 : gl);
         // point j + m*NR sits at o0 + (m*NR)*sn: one add per point, the
         // products of constants and the uniform stride stay scalar
         const uint32_t sn = (uint32_t)a.in_sn;
@@ -416,7 +424,10 @@ struct RegFft {
 #pragma unroll
             for (int m = 0; m < R; m++) {
                 const uint32_t idx = o0 + (uint32_t)(m * NR) * sn;
-                if constexpr (VEC == 2)
+                if constexpr (FOLD) {
+                    w[0][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx);
+                    w[1][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx + a.fold_in);
+                } else if constexpr (VEC == 2)
                     load_pair_at<Fmt, NTL>(a, idx, w[0][q * R + m], w[1][q * R + m]);
                 else if constexpr (LDM == LD_HERMITIAN)
                     w[0][q * R + m] = load_point<Fmt, LDM>(a, b.in, j + m * NR, N);
@@ -466,8 +477,13 @@ struct RegFft {
                 }
             }
             if constexpr (LAST) {
-                LineBase b = valid ? line_base(a, g) : LineBase{0, 0, 0};
+                LineBase b = valid ? line_base(a, FOLD ? g >> 1 : g) : LineBase{0, 0, 0};
                 const uint32_t sk = (uint32_t)a.out_sk;
+                // fold: the pair's rows y[r], y[r + H] of the next axis (length
+                // M = 2H) leave as y[r] + y[r + H] and W_M^r (y[r] - y[r + H]),
+                // its first radix-2 decimation-in-frequency stage
+                cpx<T> tf{};
+                if constexpr (FOLD) tf = post_twiddle<T>(a, b.j);
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
                     const uint32_t j = mp.t + q * TPL;
@@ -488,7 +504,12 @@ struct RegFft {
                                 t0 = ps0.template next<T>();
                                 v0 = cmul(v0, t0);
                             }
-                            if constexpr (VEC == 1) {
+                            if constexpr (FOLD) {
+                                const cpx<T> v1 = w[1][q * R + m];
+                                store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0 + v1);
+                                store_point_at<Fmt, STM, false>(a, idx + a.fold_out, b.j, p0 + m * L, N,
+                                                                cmul(v0 - v1, tf));
+                            } else if constexpr (VEC == 1) {
                                 store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0);
                             } else {
                                 cpx<T> v1 = w[1][q * R + m];

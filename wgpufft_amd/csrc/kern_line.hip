@@ -69,6 +69,23 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
      RADSTR,                                                                                    \
      MINB}
 
+// Fold form (two contiguous lines H rows apart per thread, the next axis's
+// first radix-2 stage on store): REGF one tile per workgroup, REGFP the
+// persistent pipelined form with MINB workgroups per CU.
+#define MIFFT_KF(FMT, N, TPL, LINES, PF, ...) \
+    RegFft<FMT, N, TPL, LINES, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), 2, PF, true>
+#define REGF(FMT, PREC, N, TPL, LINES, RADSTR, ...)                                                   \
+    {PREC, N, TPL, LINES, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 2,                                      \
+     MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)::NT,                                         \
+     MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)::lds_bytes(),                                \
+     (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)>, RADSTR, 0, 1}
+#define REGFP(FMT, PREC, N, TPL, LINES, MINB, RADSTR, ...)                                            \
+    {PREC, N, TPL, LINES, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 2,                                      \
+     MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__)::NT,                                          \
+     MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__)::lds_bytes(),                                 \
+     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__), MINB>,     \
+     RADSTR, MINB, 1}
+
 // f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
 #define F32_POW2(LDLF, STLF, LDM, STM, POST, VEC, L4096, L2048, L1024, L512, L256, L128, L64)          \
     REGV(FmtF32, P_F32, 4096, 256, L4096, LDLF, STLF, LDM, STM, POST, VEC, "16x16x16", 16, 16, 16),    \
@@ -174,6 +191,12 @@ static const RegVariant kRegistry[] = {
     REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
     REGP(FmtF32, P_F32, 2048, 128, 1, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
     REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
+    // fold passes of 2D transforms (rows r and r + H of the second axis per thread)
+    REGF(FmtF32, P_F32, 4096, 256, 2, "16x16x16", 16, 16, 16),
+    REGF(FmtF32, P_F32, 4096, 512, 2, "8x8x8x8", 8, 8, 8, 8),
+    REGFP(FmtF32, P_F32, 4096, 512, 2, 1, "8x8x8x8", 8, 8, 8, 8),
+    REGFP(FmtF32, P_F32, 4096, 512, 2, 2, "8x8x8x8", 8, 8, 8, 8),
+    REGFP(FmtF32, P_F32, 4096, 256, 2, 1, "16x16x16", 16, 16, 16),
     // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
     REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
     REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),
@@ -189,11 +212,11 @@ const RegVariant* reg_variant_at(int i) {
 }
 
 std::vector<const RegVariant*> find_reg_variants(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                                 int post) {
+                                                 int post, int fold) {
     std::vector<const RegVariant*> out;
     for (const RegVariant& v : kRegistry)
         if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm && v.stm == stm &&
-            v.post == post)
+            v.post == post && v.fold == fold)
             out.push_back(&v);
     return out;
 }
@@ -205,6 +228,7 @@ std::string reg_variant_name(const RegVariant* v) {
              v->ld_lf, v->st_lf, v->ldm, v->stm, v->post, v->vec, v->radices);
     std::string name = b;
     if (v->persist) name += ":pipe" + std::to_string(v->persist);
+    if (v->fold) name += ":fold";
     return name;
 }
 
@@ -215,10 +239,10 @@ const RegVariant* find_reg_variant_named(const std::string& name) {
 }
 
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                   int post, int vec) {
+                                   int post, int vec, int fold) {
     for (const RegVariant& v : kRegistry) {
         if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm &&
-            v.stm == stm && v.post == post && v.vec == vec)
+            v.stm == stm && v.post == post && v.vec == vec && v.fold == fold)
             return &v;
     }
     return nullptr;

@@ -32,15 +32,18 @@ struct RegVariant {
     // workgroups per CU, each loading its next tile before transforming the
     // current one).
     int persist = 0;
+    // 1: fold variant (two contiguous lines H rows apart per thread, the next
+    // axis's first radix-2 stage on store); looked up only by fold passes.
+    int fold = 0;
 };
 
 // Registry lookup; nullptr when no compiled variant matches.
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                   int post, int vec);
+                                   int post, int vec, int fold = 0);
 // Every registered variant of a line shape (any lines-per-thread width), the
 // default first; stable names for the tuning table.
 std::vector<const RegVariant*> find_reg_variants(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
-                                                 int post);
+                                                 int post, int fold = 0);
 std::string reg_variant_name(const RegVariant* v);
 const RegVariant* find_reg_variant_named(const std::string& name);
 int reg_variant_count();

@@ -286,9 +286,12 @@ int tuning_parse(const std::string& text) {
             fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": expected 'key = choice'");
         const std::string key = trim(line.substr(0, eq)), choice = trim(line.substr(eq + 1));
         const bool split_key = key.rfind("split|", 0) == 0;
-        const bool key_ok = split_key || key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 ||
+        const bool fold_key = key.rfind("fold|", 0) == 0;
+        const bool key_ok = split_key || fold_key || key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 ||
                             key.rfind("df64|n", 0) == 0;
-        const bool choice_ok = split_key ? (choice == "2" || choice == "3") : valid_choice(choice);
+        const bool choice_ok = split_key  ? (choice == "2" || choice == "3")
+                               : fold_key ? (choice == "0" || choice == "1")
+                                          : valid_choice(choice);
         if (!key_ok || !choice_ok)
             fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": unknown key or choice");
         got.emplace_back(key, choice);
@@ -1024,6 +1027,68 @@ struct Builder {
         return PostTable{hi, lo, sh, (uint32_t)(lo_n - 1)};
     }
 
+    // 2D fold: for axes 0 and 1 of lengths n0 and M = 2H (O arrays of n0 x M
+    // points), with y the array after the axis-0 transform,
+    //   X[2k + c] = DFT_H(z_c)[k],  z_0[r] = y[r] + y[r + H],
+    //   z_1[r] = W_M^r (y[r] - y[r + H])   (r < H, rows of axis 1),
+    // the first radix-2 decimation-in-frequency stage of axis 1.  Pass 1
+    // transforms rows r and r + H of axis 0 in one tile and stores z_0, z_1
+    // in their places (into the workspace); pass 2 runs H-point lines over both
+    // halves (2 n0 O lines) and stores X[2k + c] at row 2k + c (into the
+    // output).  Axis 1's tile is then half as deep: 16 columns of H points
+    // fit two workgroups per CU with 128-byte row segments, where 16 columns
+    // of M points would fill a CU alone.  Table key fold|f32|n<n0>x<M>:
+    // 1 on, 0 off; without an entry the fold runs when both kernels are
+    // compiled.
+    bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale) {
+        if (P.batch_mode || P.prec != P_F32 || n1 % 2 != 0 || n1 < 4) return false;
+        const uint64_t H = n1 / 2;
+        const std::vector<const RegVariant*> fv =
+            find_reg_variants(P.prec, (int)n0, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1);
+        if (fv.empty() || !find_reg_variant(P.prec, (int)H, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2)) return false;
+        if (2 * H * O > 0xFFFFFFFFull || n0 * 2 * O > 0xFFFFFFFFull) return false;
+        const std::string onoff = tuning_lookup("fold|f32|n" + std::to_string(n0) + "x" + std::to_string(n1));
+        if (onoff == "0") return false;
+        // pass 1: axis 0 over row pairs (r, r + H), fold on store
+        const LineSet ls1{(uint32_t)(2 * H * O), 1, (uint32_t)H};
+        const Layout lay1{0, (int64_t)n0, (int64_t)(n0 * n1), 1};
+        Op op;
+        op.kind = OP_REG;
+        op.fold = true;
+        op.src = B_IN;
+        op.dst = B_WS;
+        op.prec = P.prec;
+        op.n = (uint32_t)n0;
+        op.tkey = tune_key(P.prec, n0, false, false, LD_COMPLEX, ST_COMPLEX, 0, false, ls1.lines) + "|fold";
+        const RegVariant* rv = find_reg_variant_named(tuning_lookup(op.tkey));
+        if (!rv || !rv->fold || rv->n != (int)n0 || rv->prec != P.prec) rv = fv[0];
+        op.rv = rv;
+        op.a = base_args(ls1, lay1, lay1, inverse, inverse, 1.0);
+        op.a.fold_in = op.a.fold_out = (uint32_t)(H * n0);
+        const PostTable pt = post_table(n1);
+        op.a.post_hi = pt.hi;
+        op.a.post_lo = pt.lo;
+        op.a.post_shift = pt.sh;
+        op.a.post_mask = pt.mask;
+        op.a.tw = stage_twiddles(P.arith, parse_radices(rv->radices));
+        std::ostringstream d;
+        d << "reg N=" << n0 << " " << rv->radices << " lines=" << ls1.lines << " tpl=" << rv->tpl
+          << " vec=2 lpw=" << rv->lines;
+        if (rv->persist) d << " pipe" << rv->persist;
+        d << " fold M=" << n1;
+        op.desc = d.str();
+        P.ops.push_back(op);
+        // pass 2: H-point lines, column i of half c of array o, to rows 2k + c
+        const LineSet ls2{(uint32_t)(n0 * 2 * O), (uint32_t)n0, 2};
+        const Layout in2{1, (int64_t)(H * n0), (int64_t)(n0 * n1), (int64_t)n0};
+        const Layout out2{1, (int64_t)n0, (int64_t)(n0 * n1), (int64_t)(2 * n0)};
+        if (!line_op(H, ls2, in2, out2, B_WS, 0, B_OUT, 0, inverse, inverse, scale, LD_COMPLEX, ST_COMPLEX)) {
+            P.ops.pop_back();
+            return false;
+        }
+        return true;
+    }
+
     // Three-pass split n = n3 * n1 * n2 of a contiguous axis (stride 1, O
     // lines 'n' apart), with q = q1 + n1 q2 + n1 n2 q3 and k = k3 + n3 k2 +
     // n3 n2 k1, since W_n^{qk} = W_n3^{q3 k3} W_n^{(q1 + n1 q2) k3}
@@ -1225,6 +1290,12 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 P->ops.push_back(op);
             } else if (B.small_volume(cfg, logical, inverse, scale)) {
                 P->ws_bytes = 0;
+            } else if (cfg.naxes == 2 && cfg.axes[0] == 0 && cfg.axes[1] == 1 &&
+                       P->axes[0].kind == MIFFT_AXIS_MIXED && P->axes[1].kind == MIFFT_AXIS_MIXED &&
+                       B.fold_2d(cfg.shape[0], cfg.shape[1], total / (cfg.shape[0] * cfg.shape[1]), inverse,
+                                 scale)) {
+                // the fold pair ping-pongs through one workspace copy of the array
+                P->ws_bytes = total * cb;
             } else {
                 // Ping-pong: with several axes, consecutive passes alternate between
                 // the output and a workspace copy of the array (the last pass lands
@@ -1667,9 +1738,9 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
                     if (!ref && o.tkey == key) ref = &o;
             // candidate (name, op template: kind, rv or schedule, tables)
             std::vector<std::pair<std::string, Op>> cands;
-            for (const RegVariant* v :
-                 find_reg_variants(ref->prec, (int)ref->n, ref->ld_lf, ref->st_lf, ref->ldm, ref->stm, ref->post)) {
-                if ((v->vec == 2 && !ref->pairable) || v->lds_bytes > kLdsBudget) continue;
+            for (const RegVariant* v : find_reg_variants(ref->prec, (int)ref->n, ref->ld_lf, ref->st_lf, ref->ldm,
+                                                         ref->stm, ref->post, ref->fold ? 1 : 0)) {
+                if ((v->vec == 2 && !ref->pairable && !v->fold) || v->lds_bytes > kLdsBudget) continue;
                 Op c = *ref;
                 c.kind = OP_REG;
                 c.rv = v;
@@ -1678,7 +1749,7 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
             }
             const uint64_t lstr = lds_line_stride(ref->n);
             const size_t es = elem_bytes(ref->prec == P_F32 ? P_F32 : P_F64);
-            for (uint32_t lpw = 1; lpw <= 32; lpw *= 2) {
+            for (uint32_t lpw = 1; lpw <= 32 && !ref->fold; lpw *= 2) {
                 if (2 * lstr * es * lpw > T.lds_budget || lpw > ref->a.lines) break;
                 Op c = *ref;
                 c.kind = OP_LDS;

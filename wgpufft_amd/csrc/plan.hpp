@@ -66,6 +66,7 @@ struct Op {
     // line ops: lane layout, whether line pairs are allowed, and the
     // tuning-table key of the op's shape
     bool ld_lf = false, st_lf = false, pairable = false;
+    bool fold = false;  // the fold pass of a 2D transform (fold register variants only)
     std::string tkey;
 };
 
