@@ -503,7 +503,8 @@ def profiled_traffic(desc, prec="f32"):
                 continue
             if (f" {prec} N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line
                     and f" {lf} " in line
-                    and (" post" in line) == (" post" in desc) and ("pipe" in line) == ("pipe" in desc)):
+                    and (" post" in line) == (" post" in desc) and ("pipe" in line) == ("pipe" in desc)
+                    and (" fold" in line) == (" fold" in desc)):
                 return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
     return None, None
 
