@@ -29,8 +29,8 @@ def _clean():
     W.clear_tuning()
 
 
-def fold_key(n0, lines):
-    return f"f32|n{n0}|lf00|m0/0|p0|pair0|lines2^{(lines - 1).bit_length()}|fold"
+def fold_key(n0, lines, F=2):
+    return f"f32|n{n0}|lf00|m0/0|p0|pair0|lines2^{(lines - 1).bit_length()}|fold{F}"
 
 
 def _run(cfg, x, in_place):
@@ -57,7 +57,7 @@ def test_fold_matches_oracle_and_unfolded(torch_cuda, shape, batch):
         W.clear_tuning()
         plan, y = _run(cfg, x, in_place=inverse)
         desc = plan.describe()
-        assert "fold M=%d" % n1 in desc, desc
+        assert "fold2 M=%d" % n1 in desc, desc
         e = oracle.rel_l2(y, want)
         W.parse_tuning(f"fold|f32|n{n0}x{n1} = 0\n")
         plain, y0 = _run(cfg, x, in_place=inverse)
@@ -90,3 +90,21 @@ def test_fold_is_off_where_it_does_not_apply(torch_cuda):
     assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_precision(W.FftPrecision.F64)).describe()
     assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([1, 0])).describe()
     assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([0])).describe()
+
+
+@pytest.mark.parametrize("shape,batch", [([4096, 256], 3), ([4096, 2048], 1)])
+def test_fold4_matches_oracle(torch_cuda, shape, batch):
+    """Radix-4 fold (table entry fold|f32|n<n0>x<M> = 4): rows r + b M/4 per
+    thread, quarter-length columns."""
+    n0, n1 = shape
+    x = seeded_complex(900 + n1, n0 * n1 * batch)
+    for inverse in (False, True):
+        W.clear_tuning()
+        W.parse_tuning(f"fold|f32|n{n0}x{n1} = 4\n")
+        cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+        plan, y = _run(cfg, x, in_place=not inverse)
+        desc = plan.describe()
+        assert "fold4 M=%d" % n1 in desc and "N=%d" % (n1 // 4) in desc, desc
+        e = oracle.rel_l2(y, c2c_expected(x, shape, batch=batch, inverse=inverse))
+        print(f"fold4 {shape} b={batch} inverse={inverse} rel_l2={e:.3e} [{desc}]", flush=True)
+        assert e <= 1e-6, (shape, inverse, e)

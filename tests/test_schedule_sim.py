@@ -133,25 +133,26 @@ def test_three_pass_split(n3, n1, n2, O):
     assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n0,n1,O", [(8, 8, 1), (4, 16, 3), (16, 6, 2), (6, 10, 1)])
-def test_fold_2d(n0, n1, O):
-    """plan.cpp fold_2d: pass 1 transforms row pairs (r, r + H) of axis 0
-    (pair q at line_base(q) with S1 = 1, S2 = H, the second row fold_in =
-    H n0 elements on) and stores y[r] + y[r + H] and W_M^r (y[r] - y[r + H]);
-    pass 2 runs H-point lines over both halves (S1 = n0 columns, S2 = 2
-    halves) and stores half c's bin k at row 2k + c."""
-    H = n1 // 2
+@pytest.mark.parametrize("n0,n1,O,F", [(8, 8, 1, 2), (4, 16, 3, 2), (16, 6, 2, 2), (6, 10, 1, 2),
+                                        (8, 16, 2, 4), (4, 8, 1, 4), (6, 12, 3, 4)])
+def test_fold_2d(n0, n1, O, F):
+    """plan.cpp fold_2d: pass 1 transforms row groups r + b Q (b < F, M = F Q)
+    of axis 0 (group q at line_base(q) with S1 = 1, S2 = Q, row b at b Q n0
+    elements on) and stores z_c = W_M^{r c} sum_b y[r + b Q] W_F^{b c} at row
+    r + c Q; pass 2 runs Q-point lines over the F parts (S1 = n0 columns, S2 =
+    F parts) and stores part c's bin k at row F k + c."""
+    Q = n1 // F
     x = crand(n0 * n1 * O)
     ws = np.zeros_like(x)
     out = np.zeros_like(x)
-    fold = H * n0
-    for q in range(H * O):  # the kernel's lines = 2 H O, pair q = g // 2
-        b, r = line_base(q, 1, H, 0, n0, n0 * n1)
-        ya = np.fft.fft(x[b + np.arange(n0)])
-        yb = np.fft.fft(x[b + fold + np.arange(n0)])
-        ws[b + np.arange(n0)] = ya + yb
-        ws[b + fold + np.arange(n0)] = W(n1, r) * (ya - yb)
-    run_lines(ws, out, H, n0 * 2 * O, n0, 2, (1, H * n0, n0 * n1, n0), (1, n0, n0 * n1, 2 * n0))
+    fold = Q * n0
+    for q in range(Q * O):  # the kernel's lines = n1 O, group q = g // F
+        b, r = line_base(q, 1, Q, 0, n0, n0 * n1)
+        y = np.array([np.fft.fft(x[b + k * fold + np.arange(n0)]) for k in range(F)])
+        z = np.fft.fft(y, axis=0)  # radix-F DFT across the rows
+        for c in range(F):
+            ws[b + c * fold + np.arange(n0)] = W(n1, r * c) * z[c]
+    run_lines(ws, out, Q, n0 * F * O, n0, F, (1, Q * n0, n0 * n1, n0), (1, n0, n0 * n1, F * n0))
     ref = np.fft.fft2(x.reshape(O, n1, n0), axes=(1, 2)).reshape(-1)
     assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
 

@@ -42,8 +42,8 @@ struct LineArgs {
     uint32_t post_shift, post_mask;
     double scale;
     uint32_t conj_in, conj_out;
-    // Fold passes: element offsets of the second line of each pair (the
-    // first line at line_base(g / 2)).
+    // Fold passes (F lines per thread): line c of a fold group sits c times
+    // these element offsets after the first, which is at line_base(g / F).
     uint32_t fold_in, fold_out;
 };
 
@@ -258,10 +258,10 @@ __device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uin
 //   the LDS exchange then runs in VEC rounds through one slot per line group).
 //   PF: load every stage's twiddles into registers together with the data, so
 //   their latency overlaps the data loads instead of following each exchange.
-//   FOLD (with VEC 2, contiguous lines): the two lines of a thread are rows r
-//   and r + H of the next axis (M = 2H points, fold_in / fold_out apart),
-//   and the store applies that axis's first radix-2 stage, so that its pass
-//   runs M/2-point lines (half the tile of an M-point line).
+//   FOLD (contiguous lines, F = VEC of 2 or 4): the F lines of a thread are
+//   rows r + c Q (c < F) of the next axis (M = F Q points, fold_in /
+//   fold_out apart), and the store applies that axis's first radix-F stage,
+//   so that its pass runs Q-point lines (1/F of the tile of an M-point line).
 template <class Fmt, int N, int TPL, int LINES, bool LD_LF, bool ST_LF, int LDM, int STM, bool POST,
           class RL, int VEC = 1, bool PF = false, bool FOLD = false>
 struct RegFft {
@@ -282,7 +282,7 @@ struct RegFft {
     static constexpr int NT = TPL * GROUPS;
     static constexpr int LSTR = lds_pad(N) + 1;
     static_assert(VEC == 1 || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
-    static_assert(!FOLD || (VEC == 2 && !LD_LF && !ST_LF && !POST), "fold pairs are contiguous lines");
+    static_assert(!FOLD || ((VEC == 2 || VEC == 4) && !LD_LF && !ST_LF && !POST), "fold rows are contiguous lines");
     // Lines of 16-byte points whose exchange buffer would exceed 80 KiB go
     // through LDS one component at a time: half the space, twice the rounds
     // (four 4096-point f64 lines per workgroup instead of two).
@@ -412,7 +412,7 @@ struct RegFft {
         const Map mp = map_of<0>();
         const uint32_t g = g0 + mp.lg * VEC;
         const uint32_t gl = g < a.lines ? g : (a.lines - VEC);
-        LineBase b = line_base(a, FOLD ? gl >> 1 : gl);
+        LineBase b = line_base(a, FOLD ? gl / VEC : gl);
         // point j + m*NR sits at o0 + (m*NR)*sn: one add per point, the
         // products of constants and the uniform stride stay scalar
         const uint32_t sn = (uint32_t)a.in_sn;
@@ -425,8 +425,9 @@ struct RegFft {
             for (int m = 0; m < R; m++) {
                 const uint32_t idx = o0 + (uint32_t)(m * NR) * sn;
                 if constexpr (FOLD) {
-                    w[0][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx);
-                    w[1][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx + a.fold_in);
+#pragma unroll
+                    for (int v = 0; v < VEC; v++)
+                        w[v][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx + v * a.fold_in);
                 } else if constexpr (VEC == 2)
                     load_pair_at<Fmt, NTL>(a, idx, w[0][q * R + m], w[1][q * R + m]);
                 else if constexpr (LDM == LD_HERMITIAN)
@@ -477,13 +478,16 @@ struct RegFft {
                 }
             }
             if constexpr (LAST) {
-                LineBase b = valid ? line_base(a, FOLD ? g >> 1 : g) : LineBase{0, 0, 0};
+                LineBase b = valid ? line_base(a, FOLD ? g / VEC : g) : LineBase{0, 0, 0};
                 const uint32_t sk = (uint32_t)a.out_sk;
-                // fold: the pair's rows y[r], y[r + H] of the next axis (length
-                // M = 2H) leave as y[r] + y[r + H] and W_M^r (y[r] - y[r + H]),
-                // its first radix-2 decimation-in-frequency stage
-                cpx<T> tf{};
-                if constexpr (FOLD) tf = post_twiddle<T>(a, b.j);
+                // fold: rows y[r + b Q] (b < F) of the next axis (M = F Q)
+                // leave as z_c = W_M^{r c} sum_b y[r + b Q] W_F^{b c}, its first
+                // radix-F decimation-in-frequency stage
+                cpx<T> tf[VEC];
+                if constexpr (FOLD) {
+#pragma unroll
+                    for (int c = 0; c < VEC; c++) tf[c] = post_twiddle<T>(a, (uint64_t)b.j * c);
+                }
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
                     const uint32_t j = mp.t + q * TPL;
@@ -505,10 +509,15 @@ struct RegFft {
                                 v0 = cmul(v0, t0);
                             }
                             if constexpr (FOLD) {
-                                const cpx<T> v1 = w[1][q * R + m];
-                                store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0 + v1);
-                                store_point_at<Fmt, STM, false>(a, idx + a.fold_out, b.j, p0 + m * L, N,
-                                                                cmul(v0 - v1, tf));
+                                cpx<T> z[VEC];
+#pragma unroll
+                                for (int c = 0; c < VEC; c++) z[c] = w[c][q * R + m];
+                                Dft<VEC, T>::run(z);
+                                store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, z[0]);
+#pragma unroll
+                                for (int c = 1; c < VEC; c++)
+                                    store_point_at<Fmt, STM, false>(a, idx + c * a.fold_out, b.j, p0 + m * L, N,
+                                                                    cmul(z[c], tf[c]));
                             } else if constexpr (VEC == 1) {
                                 store_point_at<Fmt, STM, false>(a, idx, b.j, p0 + m * L, N, v0);
                             } else {

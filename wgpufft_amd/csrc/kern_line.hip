@@ -69,22 +69,22 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
      RADSTR,                                                                                    \
      MINB}
 
-// Fold form (two contiguous lines H rows apart per thread, the next axis's
-// first radix-2 stage on store): REGF one tile per workgroup, REGFP the
-// persistent pipelined form with MINB workgroups per CU.
-#define MIFFT_KF(FMT, N, TPL, LINES, PF, ...) \
-    RegFft<FMT, N, TPL, LINES, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), 2, PF, true>
-#define REGF(FMT, PREC, N, TPL, LINES, RADSTR, ...)                                                   \
-    {PREC, N, TPL, LINES, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 2,                                      \
-     MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)::NT,                                         \
-     MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)::lds_bytes(),                                \
-     (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, LINES, false, __VA_ARGS__)>, RADSTR, 0, 1}
-#define REGFP(FMT, PREC, N, TPL, LINES, MINB, RADSTR, ...)                                            \
-    {PREC, N, TPL, LINES, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 2,                                      \
-     MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__)::NT,                                          \
-     MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__)::lds_bytes(),                                 \
-     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, LINES, true, __VA_ARGS__), MINB>,     \
-     RADSTR, MINB, 1}
+// Fold form (F contiguous lines M/F rows apart per thread, the next axis's
+// first radix-F stage on store): REGF one tile of F lines per workgroup, REGFP
+// the persistent pipelined form with MINB workgroups per CU.
+#define MIFFT_KF(FMT, N, TPL, F, PF, ...) \
+    RegFft<FMT, N, TPL, F, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), F, PF, true>
+#define REGF(FMT, PREC, N, TPL, F, RADSTR, ...)                                                       \
+    {PREC, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F,                                          \
+     MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)::NT,                                             \
+     MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)::lds_bytes(),                                    \
+     (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)>, RADSTR, 0, F}
+#define REGFP(FMT, PREC, N, TPL, F, MINB, RADSTR, ...)                                                \
+    {PREC, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F,                                          \
+     MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__)::NT,                                              \
+     MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__)::lds_bytes(),                                     \
+     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__), MINB>,         \
+     RADSTR, MINB, F}
 
 // f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
 #define F32_POW2(LDLF, STLF, LDM, STM, POST, VEC, L4096, L2048, L1024, L512, L256, L128, L64)          \
@@ -191,12 +191,13 @@ static const RegVariant kRegistry[] = {
     REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
     REGP(FmtF32, P_F32, 2048, 128, 1, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
     REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 2, "16x16x8", 16, 16, 8),
-    // fold passes of 2D transforms (rows r and r + H of the second axis per thread)
+    // fold passes of 2D transforms (rows r + c M/F of the second axis per thread)
     REGF(FmtF32, P_F32, 4096, 256, 2, "16x16x16", 16, 16, 16),
     REGF(FmtF32, P_F32, 4096, 512, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFP(FmtF32, P_F32, 4096, 512, 2, 1, "8x8x8x8", 8, 8, 8, 8),
     REGFP(FmtF32, P_F32, 4096, 512, 2, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFP(FmtF32, P_F32, 4096, 256, 2, 1, "16x16x16", 16, 16, 16),
+    REGF(FmtF32, P_F32, 4096, 512, 4, "8x8x8x8", 8, 8, 8, 8),
     // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
     REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
     REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),

@@ -32,8 +32,8 @@ struct RegVariant {
     // workgroups per CU, each loading its next tile before transforming the
     // current one).
     int persist = 0;
-    // 1: fold variant (two contiguous lines H rows apart per thread, the next
-    // axis's first radix-2 stage on store); looked up only by fold passes.
+    // F > 0: fold variant (F contiguous lines M/F rows apart per thread, the
+    // next axis's first radix-F stage on store); looked up only by fold passes.
     int fold = 0;
 };
 

@@ -290,7 +290,7 @@ int tuning_parse(const std::string& text) {
         const bool key_ok = split_key || fold_key || key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 ||
                             key.rfind("df64|n", 0) == 0;
         const bool choice_ok = split_key  ? (choice == "2" || choice == "3")
-                               : fold_key ? (choice == "0" || choice == "1")
+                               : fold_key ? (choice == "0" || choice == "2" || choice == "4")
                                           : valid_choice(choice);
         if (!key_ok || !choice_ok)
             fail(MIFFT_ERR_INVALID_TUNING, "tuning table line " + std::to_string(line_no) + ": unknown key or choice");
@@ -1027,30 +1027,32 @@ struct Builder {
         return PostTable{hi, lo, sh, (uint32_t)(lo_n - 1)};
     }
 
-    // 2D fold: for axes 0 and 1 of lengths n0 and M = 2H (O arrays of n0 x M
+    // 2D fold: for axes 0 and 1 of lengths n0 and M = F Q (O arrays of n0 x M
     // points), with y the array after the axis-0 transform,
-    //   X[2k + c] = DFT_H(z_c)[k],  z_0[r] = y[r] + y[r + H],
-    //   z_1[r] = W_M^r (y[r] - y[r + H])   (r < H, rows of axis 1),
-    // the first radix-2 decimation-in-frequency stage of axis 1.  Pass 1
-    // transforms rows r and r + H of axis 0 in one tile and stores z_0, z_1
-    // in their places (into the workspace); pass 2 runs H-point lines over both
-    // halves (2 n0 O lines) and stores X[2k + c] at row 2k + c (into the
-    // output).  Axis 1's tile is then half as deep: 16 columns of H points
-    // fit two workgroups per CU with 128-byte row segments, where 16 columns
-    // of M points would fill a CU alone.  Table key fold|f32|n<n0>x<M>:
-    // 1 on, 0 off; without an entry the fold runs when both kernels are
-    // compiled.
+    //   X[F k + c] = DFT_Q(z_c)[k],
+    //   z_c[r] = W_M^{r c} sum_b y[r + b Q] W_F^{b c}   (r < Q, rows of axis 1),
+    // the first radix-F decimation-in-frequency stage of axis 1 (F = 2: z_0 =
+    // y[r] + y[r + Q], z_1 = W_M^r (y[r] - y[r + Q])).  Pass 1 transforms rows
+    // r + b Q (b < F) of axis 0 in one tile and stores z_c at row r + c Q
+    // (into the workspace); pass 2 runs Q-point lines over the F parts (F n0 O
+    // lines) and stores X[F k + c] at row F k + c (into the output).  Axis 1's
+    // tile is then 1/F as deep: 8 columns of 1024 points (64 KiB) fit four
+    // workgroups per CU where 16 columns of 2048 points fill a CU alone.
+    // Table key fold|f32|n<n0>x<M>: 0 off, 2 or 4 the factor; without an
+    // entry F = 2 when both kernels are compiled.
     bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale) {
-        if (P.batch_mode || P.prec != P_F32 || n1 % 2 != 0 || n1 < 4) return false;
-        const uint64_t H = n1 / 2;
+        if (P.batch_mode || P.prec != P_F32) return false;
+        const std::string choice = tuning_lookup("fold|f32|n" + std::to_string(n0) + "x" + std::to_string(n1));
+        if (choice == "0") return false;
+        const uint64_t F = choice == "4" ? 4 : 2;
+        if (n1 % F != 0 || n1 / F < 2) return false;
+        const uint64_t Q = n1 / F;
         const std::vector<const RegVariant*> fv =
-            find_reg_variants(P.prec, (int)n0, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1);
-        if (fv.empty() || !find_reg_variant(P.prec, (int)H, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2)) return false;
-        if (2 * H * O > 0xFFFFFFFFull || n0 * 2 * O > 0xFFFFFFFFull) return false;
-        const std::string onoff = tuning_lookup("fold|f32|n" + std::to_string(n0) + "x" + std::to_string(n1));
-        if (onoff == "0") return false;
-        // pass 1: axis 0 over row pairs (r, r + H), fold on store
-        const LineSet ls1{(uint32_t)(2 * H * O), 1, (uint32_t)H};
+            find_reg_variants(P.prec, (int)n0, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, (int)F);
+        if (fv.empty() || !find_reg_variant(P.prec, (int)Q, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2)) return false;
+        if (n1 * O > 0xFFFFFFFFull || n0 * F * O > 0xFFFFFFFFull) return false;
+        // pass 1: axis 0 over row groups (r + b Q, b < F), fold on store
+        const LineSet ls1{(uint32_t)(n1 * O), 1, (uint32_t)Q};
         const Layout lay1{0, (int64_t)n0, (int64_t)(n0 * n1), 1};
         Op op;
         op.kind = OP_REG;
@@ -1059,12 +1061,13 @@ struct Builder {
         op.dst = B_WS;
         op.prec = P.prec;
         op.n = (uint32_t)n0;
-        op.tkey = tune_key(P.prec, n0, false, false, LD_COMPLEX, ST_COMPLEX, 0, false, ls1.lines) + "|fold";
+        op.tkey = tune_key(P.prec, n0, false, false, LD_COMPLEX, ST_COMPLEX, 0, false, ls1.lines) + "|fold" +
+                  std::to_string(F);
         const RegVariant* rv = find_reg_variant_named(tuning_lookup(op.tkey));
-        if (!rv || !rv->fold || rv->n != (int)n0 || rv->prec != P.prec) rv = fv[0];
+        if (!rv || rv->fold != (int)F || rv->n != (int)n0 || rv->prec != P.prec) rv = fv[0];
         op.rv = rv;
         op.a = base_args(ls1, lay1, lay1, inverse, inverse, 1.0);
-        op.a.fold_in = op.a.fold_out = (uint32_t)(H * n0);
+        op.a.fold_in = op.a.fold_out = (uint32_t)(Q * n0);
         const PostTable pt = post_table(n1);
         op.a.post_hi = pt.hi;
         op.a.post_lo = pt.lo;
@@ -1072,17 +1075,17 @@ struct Builder {
         op.a.post_mask = pt.mask;
         op.a.tw = stage_twiddles(P.arith, parse_radices(rv->radices));
         std::ostringstream d;
-        d << "reg N=" << n0 << " " << rv->radices << " lines=" << ls1.lines << " tpl=" << rv->tpl
-          << " vec=2 lpw=" << rv->lines;
+        d << "reg N=" << n0 << " " << rv->radices << " lines=" << ls1.lines << " tpl=" << rv->tpl << " vec=" << F
+          << " lpw=" << rv->lines;
         if (rv->persist) d << " pipe" << rv->persist;
-        d << " fold M=" << n1;
+        d << " fold" << F << " M=" << n1;
         op.desc = d.str();
         P.ops.push_back(op);
-        // pass 2: H-point lines, column i of half c of array o, to rows 2k + c
-        const LineSet ls2{(uint32_t)(n0 * 2 * O), (uint32_t)n0, 2};
-        const Layout in2{1, (int64_t)(H * n0), (int64_t)(n0 * n1), (int64_t)n0};
-        const Layout out2{1, (int64_t)n0, (int64_t)(n0 * n1), (int64_t)(2 * n0)};
-        if (!line_op(H, ls2, in2, out2, B_WS, 0, B_OUT, 0, inverse, inverse, scale, LD_COMPLEX, ST_COMPLEX)) {
+        // pass 2: Q-point lines, column i of part c of array o, to rows F k + c
+        const LineSet ls2{(uint32_t)(n0 * F * O), (uint32_t)n0, (uint32_t)F};
+        const Layout in2{1, (int64_t)(Q * n0), (int64_t)(n0 * n1), (int64_t)n0};
+        const Layout out2{1, (int64_t)n0, (int64_t)(n0 * n1), (int64_t)(F * n0)};
+        if (!line_op(Q, ls2, in2, out2, B_WS, 0, B_OUT, 0, inverse, inverse, scale, LD_COMPLEX, ST_COMPLEX)) {
             P.ops.pop_back();
             return false;
         }
@@ -1739,7 +1742,7 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
             // candidate (name, op template: kind, rv or schedule, tables)
             std::vector<std::pair<std::string, Op>> cands;
             for (const RegVariant* v : find_reg_variants(ref->prec, (int)ref->n, ref->ld_lf, ref->st_lf, ref->ldm,
-                                                         ref->stm, ref->post, ref->fold ? 1 : 0)) {
+                                                         ref->stm, ref->post, ref->fold ? ref->rv->fold : 0)) {
                 if ((v->vec == 2 && !ref->pairable && !v->fold) || v->lds_bytes > kLdsBudget) continue;
                 Op c = *ref;
                 c.kind = OP_REG;
