@@ -108,3 +108,67 @@ def test_fold4_matches_oracle(torch_cuda, shape, batch):
         e = oracle.rel_l2(y, c2c_expected(x, shape, batch=batch, inverse=inverse))
         print(f"fold4 {shape} b={batch} inverse={inverse} rel_l2={e:.3e} [{desc}]", flush=True)
         assert e <= 1e-6, (shape, inverse, e)
+
+
+@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2)])
+def test_r2c_fold_matches_oracle(torch_cuda, shape, batch):
+    """R2C fold (real rows r, r + M/2 in, folded packed rows out, then
+    M/2-point packed columns) against the FP64 oracle, the unfolded plan and a
+    C2R round trip."""
+    import torch
+    n0, n1 = shape
+    xr = oracle.normal_f32(300 + n1 + n0, n0 * n1 * batch)
+    want = oracle.r2c_nd(xr, shape, batch=batch, nthreads=8)
+    outs = []
+    for table in ("", f"fold|f32|r{n0}x{n1} = 0\n"):
+        W.clear_tuning()
+        if table:
+            W.parse_tuning(table)
+        r2c = FftPlan.r2c(FftConfig.new_nd(shape).with_batch(batch))
+        desc = r2c.describe()
+        assert ("fold2" in desc) == (not table), desc
+        pk = empty_dev(r2c.required_output_buffer_size_bytes())
+        r2c.execute(to_dev(xr), pk)
+        torch.cuda.synchronize()
+        outs.append(from_dev(pk, np.complex64))
+        e = oracle.rel_l2(outs[-1], want)
+        print(f"r2c {shape} b={batch} table={table.strip()!r} rel_l2={e:.3e} [{desc}]", flush=True)
+        assert e <= 1e-6
+    assert oracle.rel_l2(outs[0], outs[1]) <= 1e-6
+    W.clear_tuning()
+    c2r = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
+    back = empty_dev(xr.nbytes)
+    c2r.execute(to_dev(outs[0]), back)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(from_dev(back, np.float32), xr) <= 1e-6
+
+
+@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2)])
+def test_c2r_unfold_matches_oracle(torch_cuda, shape, batch):
+    """C2R unfold (half-length inverse columns over the even and odd rows,
+    then the Hermitian-load axis-0 pass forming E_0 +- W_M^{-n} E_1) against
+    the FP64 oracle and the plan without it, from an oracle spectrum whose
+    self-conjugate bins carry imaginary parts (dropped by the C2R)."""
+    import torch
+    n0, n1 = shape
+    P0 = n0 // 2 + 1
+    pk = seeded_complex(400 + n1, P0 * n1 * batch)
+    want = oracle.c2r_nd(pk, shape, batch=batch, nthreads=8)
+    outs = []
+    for table in ("", f"fold|f32|c{n0}x{n1} = 0\n"):
+        W.clear_tuning()
+        if table:
+            W.parse_tuning(table)
+        c2r = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
+        desc = c2r.describe()
+        assert ("unfold2" in desc) == (not table), desc
+        d = to_dev(pk)
+        out = empty_dev(n0 * n1 * batch * 4)
+        c2r.execute(d, out)
+        torch.cuda.synchronize()
+        outs.append(from_dev(out, np.float32))
+        e = oracle.rel_l2(outs[-1], want)
+        print(f"c2r {shape} b={batch} table={table.strip()!r} rel_l2={e:.3e} [{desc}]", flush=True)
+        assert e <= 1e-6
+        assert np.array_equal(from_dev(d, np.complex64), pk)  # the input survives
+    assert oracle.rel_l2(outs[0], outs[1]) <= 1e-6

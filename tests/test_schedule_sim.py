@@ -157,6 +157,54 @@ def test_fold_2d(n0, n1, O, F):
     assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n0,n1,O", [(8, 8, 1), (6, 16, 3), (16, 4, 2)])
+def test_fold_2d_r2c(n0, n1, O):
+    """plan.cpp fold_2d for R2C (real load, packed store): real rows r and
+    r + H (pitch n0, fold_in = H n0) transform to packed rows of P0 = n0/2 + 1
+    bins (pitch P0, fold_out = H P0) folded as in the complex case; pass 2
+    runs H-point lines over the P0 packed columns of both halves."""
+    F, H, P0 = 2, n1 // 2, n0 // 2 + 1
+    x = rng.standard_normal(n0 * n1 * O)
+    ws = np.zeros(P0 * n1 * O, dtype=complex)
+    out = np.zeros_like(ws)
+    for q in range(H * O):
+        bi, r = line_base(q, 1, H, 0, n0, n0 * n1)
+        bo, _ = line_base(q, 1, H, 0, P0, P0 * n1)
+        ya = np.fft.fft(x[bi + np.arange(n0)])[:P0]
+        yb = np.fft.fft(x[bi + H * n0 + np.arange(n0)])[:P0]
+        ws[bo + np.arange(P0)] = ya + yb
+        ws[bo + H * P0 + np.arange(P0)] = W(n1, r) * (ya - yb)
+    run_lines(ws, out, H, P0 * F * O, P0, F, (1, H * P0, P0 * n1, P0), (1, P0, P0 * n1, F * P0))
+    ref = np.fft.fft(np.fft.rfft(x.reshape(O, n1, n0), axis=2), axis=1).reshape(-1)
+    assert np.abs(out - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n0,n1,O", [(8, 8, 1), (6, 16, 3), (16, 4, 2)])
+def test_unfold_2d_c2r(n0, n1, O):
+    """plan.cpp unfold_2d_c2r: pass A runs H-point inverse transforms over
+    the packed columns of the even and odd rows (row 2k + c of part c) into
+    halves (row c H + n); pass B loads rows n and n + H of the halves and
+    forms E_0 + W_M^{-n} E_1 and E_0 - W_M^{-n} E_1 per packed bin (the last
+    radix-2 stage of axis 1), then the Hermitian-extended inverse of axis 0."""
+    H, P0 = n1 // 2, n0 // 2 + 1
+    xr = rng.standard_normal(n0 * n1 * O)
+    X = np.fft.fft(np.fft.rfft(xr.reshape(O, n1, n0), axis=2), axis=1).reshape(-1)
+    ws = np.zeros_like(X)
+    for g in range(P0 * 2 * O):  # pass A (inverse = conj(DFT(conj)))
+        bi, _ = line_base(g, P0, 2, 1, P0, P0 * n1)
+        bo, _ = line_base(g, P0, 2, 1, H * P0, P0 * n1)
+        ws[bo + P0 * np.arange(H)] = np.fft.ifft(X[bi + 2 * P0 * np.arange(H)]) * H
+    out = np.zeros(n0 * n1 * O)
+    for q in range(H * O):  # pass B
+        bi, n = line_base(q, 1, H, 0, P0, P0 * n1)
+        bo, _ = line_base(q, 1, H, 0, n0, n0 * n1)
+        e0, e1 = ws[bi + np.arange(P0)], ws[bi + H * P0 + np.arange(P0)]
+        t = np.conj(W(n1, n))
+        for c, row in enumerate((e0 + t * e1, e0 - t * e1)):
+            out[bo + c * H * n0 + np.arange(n0)] = np.fft.irfft(row, n0) * n0
+    assert np.abs(out / (n0 * n1) - xr).max() <= 1e-9 * np.abs(xr).max()
+
+
 def test_post_twiddle_recurrence():
     """PostSeq: W_P^{j (p0 + m L)} for m < R from W_P^{j p0} and the step
     W_P^{j L} by repeated multiplication, within ~R ulp of the direct values."""

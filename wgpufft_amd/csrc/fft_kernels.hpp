@@ -281,8 +281,17 @@ struct RegFft {
     static constexpr int GROUPS = LINES / VEC;
     static constexpr int NT = TPL * GROUPS;
     static constexpr int LSTR = lds_pad(N) + 1;
-    static_assert(VEC == 1 || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
+    static_assert(VEC == 1 || FOLD || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
+    // Unfold (FOLD with the Hermitian load, C2R): the two rows are n and
+    // n + H of the halves E_0, E_1 left by the previous axis's half-length
+    // inverse transforms; the load forms E_0 + W_M^{-n} E_1 and
+    // E_0 - W_M^{-n} E_1 per packed bin (that axis's last radix-2
+    // decimation-in-time stage) before the Hermitian extension, and the store
+    // writes the two rows as they are.
+    static constexpr bool UNFOLD = FOLD && LDM == LD_HERMITIAN;
     static_assert(!FOLD || ((VEC == 2 || VEC == 4) && !LD_LF && !ST_LF && !POST), "fold rows are contiguous lines");
+    static_assert(!FOLD || (UNFOLD ? (VEC == 2 && STM == ST_REAL) : STM != ST_REAL),
+                  "fold: complex or real in, complex or packed out; unfold: packed in, real out");
     // Lines of 16-byte points whose exchange buffer would exceed 80 KiB go
     // through LDS one component at a time: half the space, twice the rounds
     // (four 4096-point f64 lines per workgroup instead of two).
@@ -416,6 +425,11 @@ struct RegFft {
         // point j + m*NR sits at o0 + (m*NR)*sn: one add per point, the
         // products of constants and the uniform stride stay scalar
         const uint32_t sn = (uint32_t)a.in_sn;
+        cpx<T> tin{};
+        if constexpr (UNFOLD) {
+            const cpx<T> t = post_twiddle<T>(a, b.j);
+            tin = mk<T>(t.x, -t.y);  // W_M^{-n}
+        }
 #pragma unroll
         for (int q = 0; q < NB; q++) {
             uint32_t j = mp.t + q * TPL;
@@ -424,7 +438,21 @@ struct RegFft {
 #pragma unroll
             for (int m = 0; m < R; m++) {
                 const uint32_t idx = o0 + (uint32_t)(m * NR) * sn;
-                if constexpr (FOLD) {
+                if constexpr (UNFOLD) {
+                    // Hermitian extension of the two unfolded packed rows, as load_point
+                    const uint32_t p = j + m * NR, h = N >> 1;
+                    const bool up = p > h;
+                    const uint32_t at = b.in + (up ? N - p : p) * sn;
+                    const cpx<T> e0 = Fmt::ld(a.in, at);
+                    const cpx<T> e1 = cmul(Fmt::ld(a.in, at + a.fold_in), tin);
+                    const bool self = p == 0 || 2 * p == N;
+                    const uint32_t cm = sign_mask(a.conj_in);
+                    cpx<T> u0 = e0 + e1, u1 = e0 - e1;
+                    u0.y = self ? T(0) : (up ? -u0.y : u0.y);
+                    u1.y = self ? T(0) : (up ? -u1.y : u1.y);
+                    w[0][q * R + m] = conj_by(u0, cm);
+                    w[1][q * R + m] = conj_by(u1, cm);
+                } else if constexpr (FOLD) {
 #pragma unroll
                     for (int v = 0; v < VEC; v++)
                         w[v][q * R + m] = load_point_at<Fmt, LDM, NTL>(a, idx + v * a.fold_in);
@@ -484,7 +512,7 @@ struct RegFft {
                 // leave as z_c = W_M^{r c} sum_b y[r + b Q] W_F^{b c}, its first
                 // radix-F decimation-in-frequency stage
                 cpx<T> tf[VEC];
-                if constexpr (FOLD) {
+                if constexpr (FOLD && !UNFOLD) {
 #pragma unroll
                     for (int c = 0; c < VEC; c++) tf[c] = post_twiddle<T>(a, (uint64_t)b.j * c);
                 }
@@ -508,7 +536,12 @@ struct RegFft {
                                 t0 = ps0.template next<T>();
                                 v0 = cmul(v0, t0);
                             }
-                            if constexpr (FOLD) {
+                            if constexpr (UNFOLD) {
+#pragma unroll
+                                for (int c = 0; c < VEC; c++)
+                                    store_point_at<Fmt, STM, false>(a, idx + c * a.fold_out, b.j, p0 + m * L, N,
+                                                                    w[c][q * R + m]);
+                            } else if constexpr (FOLD) {
                                 cpx<T> z[VEC];
 #pragma unroll
                                 for (int c = 0; c < VEC; c++) z[c] = w[c][q * R + m];

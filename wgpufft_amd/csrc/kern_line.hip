@@ -72,18 +72,21 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
 // Fold form (F contiguous lines M/F rows apart per thread, the next axis's
 // first radix-F stage on store): REGF one tile of F lines per workgroup, REGFP
 // the persistent pipelined form with MINB workgroups per CU.
-#define MIFFT_KF(FMT, N, TPL, F, PF, ...) \
-    RegFft<FMT, N, TPL, F, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), F, PF, true>
-#define REGF(FMT, PREC, N, TPL, F, RADSTR, ...)                                                       \
-    {PREC, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F,                                          \
-     MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)::NT,                                             \
-     MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)::lds_bytes(),                                    \
-     (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, F, false, __VA_ARGS__)>, RADSTR, 0, F}
+#define MIFFT_KF(FMT, N, TPL, F, LDM, STM, PF, ...) \
+    RegFft<FMT, N, TPL, F, false, false, LDM, STM, false, MIFFT_RL(__VA_ARGS__), F, PF, true>
+#define REGFM(FMT, PREC, N, TPL, F, LDM, STM, RADSTR, ...)                                            \
+    {PREC, N, TPL, F, 0, 0, LDM, STM, 0, F,                                                        \
+     MIFFT_KF(FMT, N, TPL, F, LDM, STM, false, __VA_ARGS__)::NT,                                   \
+     MIFFT_KF(FMT, N, TPL, F, LDM, STM, false, __VA_ARGS__)::lds_bytes(),                          \
+     (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, F, LDM, STM, false, __VA_ARGS__)>, RADSTR, 0, F}
+#define REGF(FMT, PREC, N, TPL, F, RADSTR, ...) \
+    REGFM(FMT, PREC, N, TPL, F, LD_COMPLEX, ST_COMPLEX, RADSTR, __VA_ARGS__)
 #define REGFP(FMT, PREC, N, TPL, F, MINB, RADSTR, ...)                                                \
     {PREC, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F,                                          \
-     MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__)::NT,                                              \
-     MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__)::lds_bytes(),                                     \
-     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, F, true, __VA_ARGS__), MINB>,         \
+     MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__)::NT,                      \
+     MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__)::lds_bytes(),             \
+     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__), \
+                                       MINB>,                                                      \
      RADSTR, MINB, F}
 
 // f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
@@ -198,6 +201,12 @@ static const RegVariant kRegistry[] = {
     REGFP(FmtF32, P_F32, 4096, 512, 2, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFP(FmtF32, P_F32, 4096, 256, 2, 1, "16x16x16", 16, 16, 16),
     REGF(FmtF32, P_F32, 4096, 512, 4, "8x8x8x8", 8, 8, 8, 8),
+    // R2C fold: real rows r, r + M/2 in, packed rows out
+    REGFM(FmtF32, P_F32, 2048, 128, 2, LD_REAL, ST_PACKED, "16x16x8", 16, 16, 8),
+    REGFM(FmtF32, P_F32, 2048, 256, 2, LD_REAL, ST_PACKED, "8x8x8x4", 8, 8, 8, 4),
+    // C2R unfold: packed rows n, n + M/2 of the halves in, real rows out
+    REGFM(FmtF32, P_F32, 2048, 128, 2, LD_HERMITIAN, ST_REAL, "16x16x8", 16, 16, 8),
+    REGFM(FmtF32, P_F32, 2048, 256, 2, LD_HERMITIAN, ST_REAL, "8x8x8x4", 8, 8, 8, 4),
     // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
     REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
     REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),

@@ -1027,6 +1027,75 @@ struct Builder {
         return PostTable{hi, lo, sh, (uint32_t)(lo_n - 1)};
     }
 
+    // The axis-0 pass of a fold (fold on store) or unfold (Hermitian load,
+    // unfold on load): one tile of F rows fold_in / fold_out elements apart,
+    // W_{n1}^t from FP64 tables; the tuning table's "...|fold<F>" key picks
+    // among the compiled variants fv.
+    void fold_row_op(uint64_t n0, uint64_t n1, uint64_t F, int ldm, int stm, const LineSet& ls,
+                     const Layout& lay_in, const Layout& lay_out, uint64_t fold_in, uint64_t fold_out, int src,
+                     int dst, bool conj, double scale, const std::vector<const RegVariant*>& fv) {
+        Op op;
+        op.kind = OP_REG;
+        op.fold = true;
+        op.src = src;
+        op.dst = dst;
+        op.prec = P.prec;
+        op.n = (uint32_t)n0;
+        op.ldm = ldm;
+        op.stm = stm;
+        op.tkey = tune_key(P.prec, n0, false, false, ldm, stm, 0, false, ls.lines) + "|fold" + std::to_string(F);
+        const RegVariant* rv = find_reg_variant_named(tuning_lookup(op.tkey));
+        if (!rv || rv->fold != (int)F || rv->n != (int)n0 || rv->prec != P.prec || rv->ldm != ldm || rv->stm != stm)
+            rv = fv[0];
+        op.rv = rv;
+        op.a = base_args(ls, lay_in, lay_out, conj, conj, scale);
+        op.a.fold_in = (uint32_t)fold_in;
+        op.a.fold_out = (uint32_t)fold_out;
+        const PostTable pt = post_table(n1);
+        op.a.post_hi = pt.hi;
+        op.a.post_lo = pt.lo;
+        op.a.post_shift = pt.sh;
+        op.a.post_mask = pt.mask;
+        op.a.tw = stage_twiddles(P.arith, parse_radices(rv->radices));
+        std::ostringstream d;
+        d << "reg N=" << n0 << " " << rv->radices << " lines=" << ls.lines << " tpl=" << rv->tpl << " vec=" << F
+          << " lpw=" << rv->lines;
+        if (rv->persist) d << " pipe" << rv->persist;
+        d << (ldm == LD_HERMITIAN ? " unfold" : " fold") << F << " M=" << n1;
+        if (ldm != LD_COMPLEX || stm != ST_COMPLEX) d << " mode=" << ldm << "/" << stm;
+        op.desc = d.str();
+        P.ops.push_back(op);
+    }
+
+    // C2R of [n0, M] (M = 2H): pass A runs H-point inverse transforms over the
+    // packed columns of the even and odd rows of axis 1 (row 2k + c of part
+    // c) into halves (row c H + n, workspace); pass B is the C2R axis-0 pass
+    // over row pairs (n, n + H), whose load forms E_0 + W_M^{-n} E_1 and
+    // E_0 - W_M^{-n} E_1 per packed bin (axis 1's last radix-2
+    // decimation-in-time stage) before the Hermitian extension:
+    //   x[n + c H] = sum_k X[2k] W_H^{-nk} + (-1)^c W_M^{-n} sum_k X[2k+1] W_H^{-nk}.
+    // Table key fold|f32|c<n0>x<M>: 0 off; on (2) by default when both
+    // kernels are compiled.
+    bool unfold_2d_c2r(uint64_t n0, uint64_t n1, uint64_t O, double scale) {
+        if (P.batch_mode || P.prec != P_F32 || n1 % 2 != 0 || n1 < 4) return false;
+        const std::string choice = tuning_lookup("fold|f32|c" + std::to_string(n0) + "x" + std::to_string(n1));
+        if (!choice.empty() && choice != "2") return false;
+        const uint64_t H = n1 / 2, P0 = n0 / 2 + 1;
+        const std::vector<const RegVariant*> fv = find_reg_variants(P.prec, (int)n0, 0, 0, LD_HERMITIAN, ST_REAL, 0, 2);
+        if (fv.empty() || !find_reg_variant(P.prec, (int)H, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2)) return false;
+        if (P0 * 2 * O > 0xFFFFFFFFull || n1 * O > 0xFFFFFFFFull) return false;
+        const LineSet lsA{(uint32_t)(P0 * 2 * O), (uint32_t)P0, 2};
+        const Layout inA{1, (int64_t)P0, (int64_t)(P0 * n1), (int64_t)(2 * P0)};
+        const Layout outA{1, (int64_t)(H * P0), (int64_t)(P0 * n1), (int64_t)P0};
+        if (!line_op(H, lsA, inA, outA, B_IN, 0, B_WS, 0, true, true, 1.0, LD_COMPLEX, ST_COMPLEX)) return false;
+        const LineSet lsB{(uint32_t)(n1 * O), 1, (uint32_t)H};
+        const Layout lay_in{0, (int64_t)P0, (int64_t)(P0 * n1), 1};
+        const Layout lay_out{0, (int64_t)n0, (int64_t)(n0 * n1), 1};
+        fold_row_op(n0, n1, 2, LD_HERMITIAN, ST_REAL, lsB, lay_in, lay_out, H * P0, H * n0, B_WS, B_OUT, true,
+                    scale, fv);
+        return true;
+    }
+
     // 2D fold: for axes 0 and 1 of lengths n0 and M = F Q (O arrays of n0 x M
     // points), with y the array after the axis-0 transform,
     //   X[F k + c] = DFT_Q(z_c)[k],
@@ -1038,53 +1107,34 @@ struct Builder {
     // lines) and stores X[F k + c] at row F k + c (into the output).  Axis 1's
     // tile is then 1/F as deep: 8 columns of 1024 points (64 KiB) fit four
     // workgroups per CU where 16 columns of 2048 points fill a CU alone.
-    // Table key fold|f32|n<n0>x<M>: 0 off, 2 or 4 the factor; without an
-    // entry F = 2 when both kernels are compiled.
-    bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale) {
+    // Table key fold|f32|n<n0>x<M> (r<n0>x<M> for R2C): 0 off, 2 or 4 the
+    // factor; without an entry F = 2 when both kernels are compiled.  R2C
+    // (real load, packed store) folds the packed rows of P0 = n0/2 + 1 bins,
+    // and pass 2 runs over P0 columns.
+    bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale, int ldm = LD_COMPLEX,
+                 int stm = ST_COMPLEX) {
         if (P.batch_mode || P.prec != P_F32) return false;
-        const std::string choice = tuning_lookup("fold|f32|n" + std::to_string(n0) + "x" + std::to_string(n1));
+        const bool real = ldm == LD_REAL;
+        const std::string choice = tuning_lookup(std::string(real ? "fold|f32|r" : "fold|f32|n") +
+                                                 std::to_string(n0) + "x" + std::to_string(n1));
         if (choice == "0") return false;
         const uint64_t F = choice == "4" ? 4 : 2;
         if (n1 % F != 0 || n1 / F < 2) return false;
         const uint64_t Q = n1 / F;
-        const std::vector<const RegVariant*> fv =
-            find_reg_variants(P.prec, (int)n0, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, (int)F);
+        const uint64_t pin = n0, pout = stm == ST_PACKED ? n0 / 2 + 1 : n0;  // row pitches in elements
+        const std::vector<const RegVariant*> fv = find_reg_variants(P.prec, (int)n0, 0, 0, ldm, stm, 0, (int)F);
         if (fv.empty() || !find_reg_variant(P.prec, (int)Q, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2)) return false;
-        if (n1 * O > 0xFFFFFFFFull || n0 * F * O > 0xFFFFFFFFull) return false;
+        if (n1 * O > 0xFFFFFFFFull || pout * F * O > 0xFFFFFFFFull) return false;
         // pass 1: axis 0 over row groups (r + b Q, b < F), fold on store
         const LineSet ls1{(uint32_t)(n1 * O), 1, (uint32_t)Q};
-        const Layout lay1{0, (int64_t)n0, (int64_t)(n0 * n1), 1};
-        Op op;
-        op.kind = OP_REG;
-        op.fold = true;
-        op.src = B_IN;
-        op.dst = B_WS;
-        op.prec = P.prec;
-        op.n = (uint32_t)n0;
-        op.tkey = tune_key(P.prec, n0, false, false, LD_COMPLEX, ST_COMPLEX, 0, false, ls1.lines) + "|fold" +
-                  std::to_string(F);
-        const RegVariant* rv = find_reg_variant_named(tuning_lookup(op.tkey));
-        if (!rv || rv->fold != (int)F || rv->n != (int)n0 || rv->prec != P.prec) rv = fv[0];
-        op.rv = rv;
-        op.a = base_args(ls1, lay1, lay1, inverse, inverse, 1.0);
-        op.a.fold_in = op.a.fold_out = (uint32_t)(Q * n0);
-        const PostTable pt = post_table(n1);
-        op.a.post_hi = pt.hi;
-        op.a.post_lo = pt.lo;
-        op.a.post_shift = pt.sh;
-        op.a.post_mask = pt.mask;
-        op.a.tw = stage_twiddles(P.arith, parse_radices(rv->radices));
-        std::ostringstream d;
-        d << "reg N=" << n0 << " " << rv->radices << " lines=" << ls1.lines << " tpl=" << rv->tpl << " vec=" << F
-          << " lpw=" << rv->lines;
-        if (rv->persist) d << " pipe" << rv->persist;
-        d << " fold" << F << " M=" << n1;
-        op.desc = d.str();
-        P.ops.push_back(op);
+        const Layout lay_in{0, (int64_t)pin, (int64_t)(pin * n1), 1};
+        const Layout lay_out{0, (int64_t)pout, (int64_t)(pout * n1), 1};
+        fold_row_op(n0, n1, F, ldm, stm, ls1, lay_in, lay_out, Q * pin, Q * pout, B_IN, B_WS, inverse, 1.0, fv);
         // pass 2: Q-point lines, column i of part c of array o, to rows F k + c
-        const LineSet ls2{(uint32_t)(n0 * F * O), (uint32_t)n0, (uint32_t)F};
-        const Layout in2{1, (int64_t)(Q * n0), (int64_t)(n0 * n1), (int64_t)n0};
-        const Layout out2{1, (int64_t)n0, (int64_t)(n0 * n1), (int64_t)(F * n0)};
+        const uint64_t C = pout;
+        const LineSet ls2{(uint32_t)(C * F * O), (uint32_t)C, (uint32_t)F};
+        const Layout in2{1, (int64_t)(Q * C), (int64_t)(C * n1), (int64_t)C};
+        const Layout out2{1, (int64_t)C, (int64_t)(C * n1), (int64_t)(F * C)};
         if (!line_op(Q, ls2, in2, out2, B_WS, 0, B_OUT, 0, inverse, inverse, scale, LD_COMPLEX, ST_COMPLEX)) {
             P.ops.pop_back();
             return false;
@@ -1365,7 +1415,11 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 P->ops.push_back(op);
             };
             uint64_t ws = 0;
-            if (kind == MIFFT_R2C) {
+            if (kind == MIFFT_R2C && cfg.rank == 2 && smooth0 && P->axes[1].kind == MIFFT_AXIS_MIXED &&
+                B.fold_2d(N0, cfg.shape[1], cfg.batch, false, scale, LD_REAL, ST_PACKED)) {
+                // 2D fold: real rows -> folded packed rows in the workspace -> output
+                ws = packed * 8;
+            } else if (kind == MIFFT_R2C) {
                 const double s0 = cfg.rank == 1 ? scale : 1.0;
                 if (!(smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false, s0, LD_REAL,
                                            ST_PACKED))) {
@@ -1384,6 +1438,10 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                                                  false, axis + 1 == cfg.rank ? scale : 1.0));
                     S *= n;
                 }
+            } else if (cfg.rank == 2 && smooth0 && P->axes[1].kind == MIFFT_AXIS_MIXED &&
+                       B.unfold_2d_c2r(N0, cfg.shape[1], cfg.batch, scale)) {
+                // 2D unfold: packed spectrum -> halves in the workspace -> real output
+                ws = packed * 8;
             } else {
                 int src = B_IN;
                 uint64_t src_off = 0;
