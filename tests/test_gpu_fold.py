@@ -47,7 +47,8 @@ def _run(cfg, x, in_place):
     return plan, from_dev(out, np.complex64)
 
 
-@pytest.mark.parametrize("shape,batch", [([4096, 128], 3), ([4096, 2048], 1), ([4096, 8192], 1)])
+@pytest.mark.parametrize("shape,batch", [([4096, 128], 3), ([4096, 2048], 1), ([4096, 8192], 1), ([2048, 2048], 1),
+                                         ([2048, 512], 2), ([1024, 1024], 2), ([1024, 4096], 1)])
 def test_fold_matches_oracle_and_unfolded(torch_cuda, shape, batch):
     n0, n1 = shape
     x = seeded_complex(500 + n1, n0 * n1 * batch)

@@ -201,6 +201,10 @@ static const RegVariant kRegistry[] = {
     REGFP(FmtF32, P_F32, 4096, 512, 2, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFP(FmtF32, P_F32, 4096, 256, 2, 1, "16x16x16", 16, 16, 16),
     REGF(FmtF32, P_F32, 4096, 512, 4, "8x8x8x8", 8, 8, 8, 8),
+    REGF(FmtF32, P_F32, 2048, 128, 2, "16x16x8", 16, 16, 8),
+    REGF(FmtF32, P_F32, 2048, 256, 2, "8x8x8x4", 8, 8, 8, 4),
+    REGF(FmtF32, P_F32, 1024, 64, 2, "16x16x4", 16, 16, 4),
+    REGF(FmtF32, P_F32, 1024, 128, 2, "8x8x4x4", 8, 8, 4, 4),
     // R2C fold: real rows r, r + M/2 in, packed rows out
     REGFM(FmtF32, P_F32, 2048, 128, 2, LD_REAL, ST_PACKED, "16x16x8", 16, 16, 8),
     REGFM(FmtF32, P_F32, 2048, 256, 2, LD_REAL, ST_PACKED, "8x8x8x4", 8, 8, 8, 4),
