@@ -173,3 +173,59 @@ def test_c2r_unfold_matches_oracle(torch_cuda, shape, batch):
         assert e <= 1e-6
         assert np.array_equal(from_dev(d, np.complex64), pk)  # the input survives
     assert oracle.rel_l2(outs[0], outs[1]) <= 1e-6
+
+
+@pytest.mark.parametrize("norm", [W.Normalization.NONE, W.Normalization.Forward, W.Normalization.Orthogonal])
+def test_fold_normalizations(torch_cuda, norm):
+    """The scale lands on the fold's second pass (config.rs:262-286)."""
+    shape = [1024, 256]
+    x = seeded_complex(60 + int(norm), 1024 * 256)
+    for inverse in (False, True):
+        cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_normalization(norm)
+        plan, y = _run(cfg, x, in_place=False)
+        assert "fold2" in plan.describe(), plan.describe()
+        e = oracle.rel_l2(y, c2c_expected(x, shape, inverse=inverse, norm=int(norm)))
+        print(f"fold norm={norm.name} inverse={inverse} rel_l2={e:.3e}", flush=True)
+        assert e <= 1e-6
+
+
+def test_fold_recorder_graph_replay(torch_cuda):
+    """Fold plans (C2C pair, R2C, C2R) recorded into one captured graph and
+    replayed several times with eager work in between: every replay equals
+    the separate executions bit for bit."""
+    import torch
+    from wgpufft_amd import FftRecorder
+    shape = [2048, 512]
+    x = seeded_complex(61, 2048 * 512)
+    xr = oracle.normal_f32(62, 2048 * 512)
+    fwd, inv = FftPlan.c2c(FftConfig.new_nd(shape)), FftPlan.c2c(FftConfig.inverse_nd(shape))
+    r2c, c2r = FftPlan.r2c(FftConfig.new_nd(shape)), FftPlan.c2r(FftConfig.inverse_nd(shape))
+    for p in (fwd, inv):
+        assert "fold2" in p.describe(), p.describe()
+    assert "fold2" in r2c.describe() and "unfold2" in c2r.describe(), (r2c.describe(), c2r.describe())
+    a, b = to_dev(x), empty_dev(x.nbytes)
+    ra, pk, back = to_dev(xr), empty_dev(r2c.required_output_buffer_size_bytes()), empty_dev(xr.nbytes)
+    fwd.execute(a, b)
+    inv.execute_in_place(b)
+    r2c.execute(ra, pk)
+    c2r.execute(pk, back)
+    torch.cuda.synchronize()
+    want_c, want_pk, want_r = (from_dev(b, np.complex64).copy(), from_dev(pk, np.complex64).copy(),
+                               from_dev(back, np.float32).copy())
+    s = torch.cuda.Stream()
+    rec = FftRecorder(stream=s, capture_graph=True)
+    fwd.record(rec, a, b)
+    inv.record_in_place(rec, b)
+    r2c.record(rec, ra, pk)
+    c2r.record(rec, pk, back)
+    rec.end()
+    for i in range(3):
+        b.zero_()
+        pk.zero_()
+        back.zero_()
+        torch.cuda.synchronize()
+        rec.launch(stream=s)
+        s.synchronize()
+        assert np.array_equal(from_dev(b, np.complex64).view(np.uint32), want_c.view(np.uint32)), i
+        assert np.array_equal(from_dev(pk, np.complex64).view(np.uint32), want_pk.view(np.uint32)), i
+        assert np.array_equal(from_dev(back, np.float32).view(np.uint32), want_r.view(np.uint32)), i
