@@ -16,8 +16,11 @@ def _sync():
     torch.cuda.synchronize()
 
 
+# the reference's own cases (tests/gpu_real.rs:22-120: 16, 17, 34, [17, 4],
+# batch 5 of 16 and 17) and wider ones
 REAL_SHAPES = [([4], 1), ([5], 3), ([16], 2), ([120], 4), ([1024], 8), ([2048, 2048], 1),
-               ([64, 48], 2), ([30, 7], 1), ([256, 17, 3], 1), ([4096], 2), ([3, 2], 1)]
+               ([64, 48], 2), ([30, 7], 1), ([256, 17, 3], 1), ([4096], 2), ([3, 2], 1),
+               ([17], 1), ([34], 1), ([17, 4], 1), ([16], 5), ([17], 5)]
 
 
 @pytest.mark.parametrize("shape,batch", REAL_SHAPES)
