@@ -344,3 +344,29 @@ def test_long_axes_envelope(torch_cuda, shape, batch):
     e2, em = oracle.rel_l2(y, ref), oracle.max_rel(y, ref)
     print(f"{shape} b={batch} rms_rel={e2:.3e} max_rel={em:.3e} [{plan.describe()}]", flush=True)
     assert e2 < 5e-7 and em < 2e-6, (e2, em)
+
+
+# tests/gpu_in_place.rs:22-245 configurations (length, batch, direction,
+# forced Bluestein) beyond the shapes above
+IN_PLACE_CASES = [([13], 1, False, False), ([17], 1, False, False), ([1523], 1, False, False),
+                  ([47], 8, False, False), ([100, 47], 1, False, False), ([179, 179], 1, False, False),
+                  ([256, 256, 3], 1, False, False), ([3, 8192], 1, False, False), ([31, 31, 31], 1, False, False),
+                  ([47, 47], 1, False, False), ([60, 64], 1, True, False), ([64, 64], 1, False, False),
+                  ([65536, 17], 1, False, False), ([65536, 3, 17], 1, False, False), ([11], 100, True, False),
+                  ([2003], 64, True, False), ([97, 97, 97], 1, True, False), ([101], 1, False, True)]
+
+
+@pytest.mark.parametrize("shape,batch,inverse,bluestein", IN_PLACE_CASES)
+def test_in_place_reference_cases(torch_cuda, shape, batch, inverse, bluestein):
+    cnt = int(np.prod(shape)) * batch
+    x = seeded_complex(19, cnt)
+    cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+    if bluestein:
+        cfg = cfg.with_tuning(FftTuning().with_force_bluestein_axes([0]))
+    plan, y_out = run_c2c(cfg, x)
+    _, y_in = run_c2c(cfg, x, in_place=True)
+    ref = c2c_expected(x, shape, batch=batch, inverse=inverse)
+    e_in = oracle.rel_l2(y_in, ref)
+    print(f"in place {shape} b={batch} inverse={inverse} rel_l2={e_in:.3e} [{plan.describe()}]", flush=True)
+    assert np.array_equal(y_out.view(np.uint32), y_in.view(np.uint32))
+    assert e_in <= 1e-6
