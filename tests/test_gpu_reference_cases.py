@@ -1,15 +1,16 @@
 """The configurations of the reference's own GPU tests (tests/gpu_fused_pow2.rs,
 gpu_fused_prime.rs, gpu_four_step.rs, gpu_small_volume.rs,
-gpu_dispatch_split.rs), run through this engine's plans and checked against
-the FP64 oracle at the reference's acceptance bound for f32 results (relative
-L2 <= 1e-6).  The reference compares its own kernels against each other and a
+gpu_dispatch_split.rs, gpu_c2c.rs, gpu_long_axes.rs, gpu_f64.rs,
+gpu_df64.rs), run through this engine's plans and checked against the FP64
+oracle at the acceptance bounds (relative L2 <= 1e-6 for f32, 1e-13 for f64,
+1e-11 for df64).  The reference compares its own kernels against each other and a
 CPU plan; the values are what those comparisons pin."""
 import numpy as np
 import pytest
 
 import oracle
 from tests.helpers import c2c_expected, empty_dev, from_dev, seeded_complex, to_dev
-from wgpufft_amd import FftConfig, FftPlan, FftTuning, Normalization as Nm
+from wgpufft_amd import FftConfig, FftError, FftPlan, FftPrecision, FftTuning, Normalization as Nm
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +53,28 @@ CASES = [
     ("small_volume", [32, 16, 16], 1, False, N, None, None), ("small_volume", [47, 47], 1, False, I, None, None),
     ("small_volume", [51, 8, 3], 1, False, F, None, None), ("small_volume", [64, 64], 1, False, I, [1], None),
     ("small_volume", [8, 8, 8, 16], 1, False, N, None, None), ("small_volume", [8, 9, 10], 1, False, O, None, None),
+    # gpu_c2c.rs
+    ("c2c", [1], 1, True, I, None, None), ("c2c", [17], 1, True, I, None, None), ("c2c", [34], 1, True, I, None, None),
+    ("c2c", [64], 1, True, I, None, None), ("c2c", [8], 1, True, I, None, None), ("c2c", [17, 4], 2, True, I, None, None),
+    ("c2c", [2, 3], 2, True, I, None, None), ("c2c", [34, 16], 1, True, I, None, None),
+    ("c2c", [4, 3], 1, True, I, [0], None), ("c2c", [1], 1, False, N, None, None),
+    ("c2c", [1024], 1, False, N, None, None), ("c2c", [16], 1, False, N, None, None),
+    ("c2c", [17], 2, False, I, None, None), ("c2c", [34], 1, False, N, None, None),
+    ("c2c", [64], 1, False, F, None, None), ("c2c", [8], 1, False, N, None, None),
+    ("c2c", [16, 16], 1, False, N, None, None), ("c2c", [17, 3, 2], 1, False, N, None, None),
+    ("c2c", [17, 34], 1, False, N, None, None), ("c2c", [2, 64], 1, False, I, None, None),
+    ("c2c", [34, 4], 1, False, I, None, None), ("c2c", [4, 17], 1, False, N, None, None),
+    ("c2c", [4, 3, 2], 1, False, N, None, None), ("c2c", [4, 34], 1, False, N, None, None),
+    ("c2c", [64, 2], 1, False, N, None, None), ("c2c", [64, 64], 1, False, N, None, None),
+    # gpu_long_axes.rs
+    ("long_axes", [12288], 1, True, I, None, None), ("long_axes", [32768], 1, True, I, None, None),
+    ("long_axes", [3, 65536], 1, True, I, [1], None), ("long_axes", [5, 4096], 1, True, I, [1], None),
+    ("long_axes", [8192, 4], 1, True, I, None, None), ("long_axes", [1 << 21], 1, False, N, None, None),
+    ("long_axes", [16384], 1, False, I, None, None), ("long_axes", [65536], 1, False, N, None, None),
+    ("long_axes", [8192], 1, False, N, None, None), ("long_axes", [1024, 1024], 1, False, I, None, None),
+    ("long_axes", [16, 2048], 1, False, I, None, None), ("long_axes", [3, 10000, 2], 1, False, I, None, None),
+    ("long_axes", [3, 8192], 1, False, I, None, None), ("long_axes", [65536, 3], 1, False, I, None, None),
+    ("long_axes", [8192, 2], 1, False, I, None, None),
     # gpu_dispatch_split.rs: a Bluestein line count above 65535 workgroups
     ("dispatch_split", [34], 62601, False, I, None, None), ("dispatch_split", [34], 62601, True, I, None, None),
 ]
@@ -81,3 +104,49 @@ def test_reference_configuration(torch_cuda, case):
     print(f"{src} {shape} b={batch} inverse={inverse} norm={norm.name} axes={axes} rel_l2={e:.3e} "
           f"[{plan.describe()}]", flush=True)
     assert e <= 1e-6
+
+
+# gpu_f64.rs and gpu_df64.rs: (shape, batch, inverse, normalization)
+HIGH = [([1], 1, True, I), ([16], 1, True, I), ([4096], 1, True, I), ([45], 1, True, I), ([60], 1, True, I),
+        ([8, 15], 1, True, I), ([1], 1, False, I), ([16], 1, False, I), ([2048], 1, False, I),
+        ([3000], 1, False, I), ([45], 1, False, I), ([60], 1, False, I), ([64], 1, False, N), ([8], 1, False, I),
+        ([8, 15], 1, False, I), ([2], 1, True, I), ([34], 1, True, I), ([67], 1, True, I), ([3, 17], 1, True, I),
+        ([9, 10], 1, True, I), ([17], 1, False, I), ([68], 1, False, I)]
+
+
+def _f64_values(seed, cnt):
+    a = oracle.uniform_f32(seed, 4 * cnt).astype(np.float64).reshape(4, cnt)
+    return (a[0] + a[1] * 2.0 ** -24) + 1j * (a[2] + a[3] * 2.0 ** -24)
+
+
+@pytest.mark.parametrize("prec", ["f64", "df64"])
+@pytest.mark.parametrize("case", range(len(HIGH)), ids=[f"{c[0]}{'-inv' if c[2] else ''}" for c in HIGH])
+def test_reference_high_precision(torch_cuda, prec, case):
+    import torch
+    shape, batch, inverse, norm = HIGH[case]
+    cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+    cfg = cfg.with_normalization(norm).with_precision(FftPrecision.F64 if prec == "f64" else FftPrecision.Df64)
+    cnt = int(np.prod(shape)) * batch
+    x = _f64_values(2000 + case, cnt).astype(np.complex128)
+    plan = FftPlan.c2c(cfg)
+    if prec == "f64":
+        d = to_dev(x)
+    else:
+        hilo = oracle.df64_split(x.view(np.float64))
+        x = oracle.df64_join(hilo).view(np.complex128)
+        d = to_dev(hilo)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    torch.cuda.synchronize()
+    y = from_dev(out, np.complex128) if prec == "f64" else oracle.df64_join(from_dev(out, np.float32)).view(np.complex128)
+    ref = c2c_expected(x, shape, batch=batch, inverse=inverse, norm=int(norm))
+    e = oracle.rel_l2(y, ref)
+    print(f"{prec} {shape} inverse={inverse} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= (1e-13 if prec == "f64" else 1e-11)
+
+
+def test_reference_invalid_axis(torch_cuda):
+    """gpu_c2c.rs: FftConfig::new(8).with_axes([1]) is InvalidAxis."""
+    with pytest.raises(FftError) as e:
+        FftPlan.c2c(FftConfig.new(8).with_axes([1]))
+    assert e.value.kind == "InvalidAxis"
