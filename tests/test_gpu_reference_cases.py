@@ -150,3 +150,30 @@ def test_reference_invalid_axis(torch_cuda):
     with pytest.raises(FftError) as e:
         FftPlan.c2c(FftConfig.new(8).with_axes([1]))
     assert e.value.kind == "InvalidAxis"
+
+
+@pytest.mark.parametrize("shape,batch", [([30, 12], 1), ([64, 32], 1), ([15, 14], 1), ([8, 8, 32], 3)])
+def test_reference_recorder_shapes(torch_cuda, shape, batch):
+    """gpu_recorder.rs shapes: forward (no normalization) and inverse recorded
+    on one stream equal separate executions bit for bit and the oracle."""
+    import torch
+    from wgpufft_amd import FftRecorder
+    cnt = int(np.prod(shape)) * batch
+    x = seeded_complex(3000 + cnt, cnt)
+    fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch).with_normalization(N))
+    inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch))
+    a, b = to_dev(x), empty_dev(x.nbytes)
+    fwd.execute(a, b)
+    torch.cuda.synchronize()
+    mid = from_dev(b, np.complex64).copy()
+    inv.execute_in_place(b)
+    torch.cuda.synchronize()
+    sep = from_dev(b, np.complex64).copy()
+    assert oracle.rel_l2(mid, c2c_expected(x, shape, batch=batch, norm=int(N))) <= 1e-6
+    assert oracle.rel_l2(sep, c2c_expected(mid, shape, batch=batch, inverse=True)) <= 1e-6
+    a2, b2 = to_dev(x), empty_dev(x.nbytes)
+    with FftRecorder() as rec:
+        fwd.record(rec, a2, b2)
+        inv.record_in_place(rec, b2)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(b2, np.complex64).view(np.uint32), sep.view(np.uint32))
