@@ -111,7 +111,8 @@ def test_fold4_matches_oracle(torch_cuda, shape, batch):
         assert e <= 1e-6, (shape, inverse, e)
 
 
-@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2)])
+@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2), ([1024, 1024], 1),
+                                         ([1024, 256], 2)])
 def test_r2c_fold_matches_oracle(torch_cuda, shape, batch):
     """R2C fold (real rows r, r + M/2 in, folded packed rows out, then
     M/2-point packed columns) against the FP64 oracle, the unfolded plan and a
@@ -144,7 +145,8 @@ def test_r2c_fold_matches_oracle(torch_cuda, shape, batch):
     assert oracle.rel_l2(from_dev(back, np.float32), xr) <= 1e-6
 
 
-@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2)])
+@pytest.mark.parametrize("shape,batch", [([2048, 2048], 1), ([2048, 256], 3), ([2048, 128], 2), ([1024, 1024], 1),
+                                         ([1024, 256], 2)])
 def test_c2r_unfold_matches_oracle(torch_cuda, shape, batch):
     """C2R unfold (half-length inverse columns over the even and odd rows,
     then the Hermitian-load axis-0 pass forming E_0 +- W_M^{-n} E_1) against

@@ -211,6 +211,8 @@ static const RegVariant kRegistry[] = {
     // C2R unfold: packed rows n, n + M/2 of the halves in, real rows out
     REGFM(FmtF32, P_F32, 2048, 128, 2, LD_HERMITIAN, ST_REAL, "16x16x8", 16, 16, 8),
     REGFM(FmtF32, P_F32, 2048, 256, 2, LD_HERMITIAN, ST_REAL, "8x8x8x4", 8, 8, 8, 4),
+    REGFM(FmtF32, P_F32, 1024, 64, 2, LD_REAL, ST_PACKED, "16x16x4", 16, 16, 4),
+    REGFM(FmtF32, P_F32, 1024, 64, 2, LD_HERMITIAN, ST_REAL, "16x16x4", 16, 16, 4),
     // smaller tiles: more tiles per persistent workgroup, a shorter pipeline tail
     REGP(FmtF32, P_F32, 1024, 64, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
     REGP(FmtF32, P_F32, 1024, 128, 2, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "8x8x4x4", 8, 8, 4, 4),
