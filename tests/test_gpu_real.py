@@ -102,7 +102,47 @@ def test_real_precision_unsupported(torch_cuda):
 # (four-step) run as widened complex lines (R2C) or Hermitian-extended lines
 # (C2R) in the workspace.
 GENERAL_SHAPES = [([17], 3), ([1009], 2), ([10007], 1), ([2 * 1709], 1), ([16384], 2), ([65536], 1),
-                  ([17, 12], 1), ([1009, 6], 2), ([16384, 4], 1), ([23, 5, 3], 1)]
+                  ([17, 12], 1), ([1009, 6], 2), ([16384, 4], 1), ([23, 5, 3], 1), ([1 << 20], 1), ([34, 6], 2),
+                  ([2 * 1009], 3)]
+
+
+# Even axis 0 that is not fused runs at half length (N0/2-point complex lines
+# and one twiddle pass); the table key fold|f32|h<N0> = 0 restores the
+# widened N0-point lines.  Both against the oracle and each other.
+@pytest.mark.parametrize("shape,batch", [([65536], 2), ([2 * 1709], 1), ([34, 6], 2), ([1 << 20], 1)])
+def test_half_length_real_axis0(torch_cuda, shape, batch):
+    import wgpufft_amd as W
+    n0 = shape[0]
+    cnt = int(np.prod(shape)) * batch
+    x = oracle.normal_f32(15, cnt)
+    pk = int(np.prod(oracle.packed_shape(shape))) * batch
+    p = oracle.uniform_f32(16, 2 * pk).view(np.complex64)
+    want_f = oracle.r2c_nd(x.astype(np.float64), shape, batch, nthreads=16)
+    want_i = oracle.c2r_nd(p.astype(np.complex128), shape, batch, nthreads=16)
+    got = {}
+    try:
+        for table in ("", f"fold|f32|h{n0} = 0\n"):
+            W.clear_tuning()
+            if table:
+                W.parse_tuning(table)
+            r2c = FftPlan.r2c(FftConfig.new_nd(shape).with_batch(batch))
+            c2r = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
+            assert ("r2c post" in r2c.describe()) == (not table), r2c.describe()
+            assert ("c2r pre" in c2r.describe()) == (not table), c2r.describe()
+            out = empty_dev(r2c.required_output_buffer_size_bytes())
+            r2c.execute(to_dev(x), out)
+            back = empty_dev(c2r.required_output_buffer_size_bytes())
+            c2r.execute(to_dev(p), back)
+            _sync()
+            yf, yi = from_dev(out, np.complex64), from_dev(back, np.float32)
+            ef, ei = oracle.rel_l2(yf, want_f), oracle.rel_l2(yi, want_i)
+            print(f"half={not table} {shape} b={batch} r2c {ef:.3e} c2r {ei:.3e} [{r2c.describe()}]", flush=True)
+            assert ef <= 1e-6 and ei <= 1e-6
+            got[bool(table)] = (yf, yi)
+    finally:
+        W.clear_tuning()
+    assert oracle.rel_l2(got[False][0], got[True][0]) <= 1e-6
+    assert oracle.rel_l2(got[False][1], got[True][1]) <= 1e-6
 
 
 @pytest.mark.parametrize("shape,batch", GENERAL_SHAPES)

@@ -205,6 +205,34 @@ def test_unfold_2d_c2r(n0, n1, O):
     assert np.abs(out / (n0 * n1) - xr).max() <= 1e-9 * np.abs(xr).max()
 
 
+@pytest.mark.parametrize("n", [4, 6, 34, 64, 3418])
+def test_half_length_real(n):
+    """kern_util.hip k_r2c_post / k_c2r_pre: R2C of N0 = 2h real points as an
+    h-point complex DFT of z[n] = x[2n] + i x[2n+1] plus one twiddle pass; the
+    C2R from a packed spectrum whose bins 0 and h carry imaginary parts (they
+    are dropped) through the h-point inverse, equal to the unnormalized
+    full-length C2R."""
+    h = n // 2
+    x = rng.standard_normal(n)
+    Z = np.fft.fft(x[0::2] + 1j * x[1::2])
+    k = np.arange(h + 1)
+    a, b = Z[k % h], np.conj(Z[(h - k) % h])
+    X = (a + b) / 2 + W(n, k) * (a - b) / 2j
+    assert np.abs(X - np.fft.rfft(x)).max() <= 1e-9 * np.abs(X).max()
+    P = rng.standard_normal(h + 1) + 1j * rng.standard_normal(h + 1)
+    Q = P.copy()
+    Q[0] = Q[0].real
+    Q[h] = Q[h].real
+    k = np.arange(h)
+    xa, xb = Q[k], np.conj(Q[h - k])
+    Zi = (xa + xb) + 1j * (xa - xb) * np.conj(W(n, k))
+    z = np.fft.ifft(Zi) * h
+    got = np.empty(n)
+    got[0::2], got[1::2] = z.real, z.imag
+    want = np.fft.irfft(P, n) * n
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+
+
 def test_post_twiddle_recurrence():
     """PostSeq: W_P^{j (p0 + m L)} for m < R from W_P^{j p0} and the step
     W_P^{j L} by repeated multiplication, within ~R ulp of the direct values."""

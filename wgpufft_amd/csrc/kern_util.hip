@@ -205,6 +205,60 @@ __global__ void k_real_part(const float2* __restrict__ full, float* __restrict__
     GRID_STRIDE(i, count) { out[i] = full[i].x * sc; }
 }
 
+// Half-length real transforms (N0 = 2h): a real line read as h complex points
+// z[n] = x[2n] + i x[2n+1], with Z = DFT_h(z) and Z[h] = Z[0], gives
+//   X[k] = A[k] + W_N0^k B[k],  A = (Z[k] + conj Z[h-k]) / 2,
+//   B = (Z[k] - conj Z[h-k]) / (2i)            (k <= h: even / odd samples);
+// the inverse builds, from the packed bins (imaginary parts of bins 0 and h
+// dropped), Z[k] = E + i O with E = X[k] + conj X[h-k] and
+// O = (X[k] - conj X[h-k]) W_N0^{-k} (k < h), whose unnormalized half-length
+// inverse is x[2n] + i x[2n+1] of the unnormalized full-length one.
+// W_N0^k = hi[k >> sh] * lo[k & mask] in FP64, rounded once.
+__device__ __forceinline__ float2 half_twiddle(const double2* hi, const double2* lo, uint32_t sh, uint32_t mask,
+                                               uint32_t k) {
+    const double2 a = hi[k >> sh], b = lo[k & mask];
+    return make_float2((float)(a.x * b.x - a.y * b.y), (float)(a.x * b.y + a.y * b.x));
+}
+
+__global__ void k_r2c_post(const float2* __restrict__ z, float2* __restrict__ x, uint32_t h, uint64_t lines, float sc,
+                           const double2* __restrict__ hi, const double2* __restrict__ lo, uint32_t sh,
+                           uint32_t mask) {
+    const uint64_t total = lines * (h + 1);
+    GRID_STRIDE(e, total) {
+        const uint64_t l = e / (h + 1);
+        const uint32_t k = (uint32_t)(e - l * (h + 1));
+        const float2 a = z[l * h + (k == h ? 0 : k)];
+        const float2 b = z[l * h + (k == 0 ? 0 : h - k)];  // conj(b) = Z[h-k]^*
+        const float2 A = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+        const float2 d = make_float2(a.x - b.x, a.y + b.y);  // Z[k] - conj Z[h-k]
+        const float2 B = make_float2(0.5f * d.y, -0.5f * d.x);  // d / (2i)
+        const float2 w = half_twiddle(hi, lo, sh, mask, k);
+        const float2 r = make_float2(A.x + w.x * B.x - w.y * B.y, A.y + w.x * B.y + w.y * B.x);
+        x[e] = make_float2(r.x * sc, r.y * sc);
+    }
+}
+
+__global__ void k_c2r_pre(const float2* __restrict__ x, float2* __restrict__ z, uint32_t h, uint64_t lines,
+                          const double2* __restrict__ hi, const double2* __restrict__ lo, uint32_t sh,
+                          uint32_t mask) {
+    const uint64_t total = lines * h;
+    GRID_STRIDE(e, total) {
+        const uint64_t l = e / h;
+        const uint32_t k = (uint32_t)(e - l * h);
+        float2 a = x[l * (h + 1) + k];
+        float2 b = x[l * (h + 1) + h - k];
+        if (k == 0) {  // bins 0 and h are self-conjugate
+            a.y = 0.0f;
+            b.y = 0.0f;
+        }
+        const float2 E = make_float2(a.x + b.x, a.y - b.y);   // X[k] + conj X[h-k]
+        const float2 d = make_float2(a.x - b.x, a.y + b.y);   // X[k] - conj X[h-k]
+        const float2 w = half_twiddle(hi, lo, sh, mask, k);  // W^k; O = d conj(W^k)
+        const float2 O = make_float2(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+        z[e] = make_float2(E.x - O.y, E.y + O.x);
+    }
+}
+
 // ------------------------------------------------------------ synthetic data
 __device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
     // State after k+1 steps of SplitMix64 (Vigna), mixed.
@@ -356,6 +410,19 @@ hipError_t launch_pack(const void* full, void* packed, uint32_t n0, uint64_t lin
 hipError_t launch_herm_extend(const void* packed, void* full, uint32_t n0, uint64_t lines, hipStream_t s) {
     k_herm_extend<<<grid_for(lines * n0), 256, 0, s>>>((const float2*)packed, (float2*)full, n0, n0 / 2 + 1,
                                                        lines);
+    return hipGetLastError();
+}
+
+hipError_t launch_r2c_post(const void* z, void* x, uint32_t h, uint64_t lines, double sc, const double2* hi,
+                           const double2* lo, uint32_t sh, uint32_t mask, hipStream_t s) {
+    k_r2c_post<<<grid_for(lines * (h + 1)), 256, 0, s>>>((const float2*)z, (float2*)x, h, lines, (float)sc, hi, lo,
+                                                         sh, mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_c2r_pre(const void* x, void* z, uint32_t h, uint64_t lines, const double2* hi, const double2* lo,
+                          uint32_t sh, uint32_t mask, hipStream_t s) {
+    k_c2r_pre<<<grid_for(lines * h), 256, 0, s>>>((const float2*)x, (float2*)z, h, lines, hi, lo, sh, mask);
     return hipGetLastError();
 }
 

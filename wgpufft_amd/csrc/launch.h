@@ -137,6 +137,12 @@ hipError_t launch_strided_copy(uint32_t elem_bytes, const void* src, void* dst, 
 hipError_t launch_pack(const void* full, void* packed, uint32_t n0, uint64_t lines, double scale, hipStream_t s);
 hipError_t launch_herm_extend(const void* packed, void* full, uint32_t n0, uint64_t lines, hipStream_t s);
 hipError_t launch_real_part(const void* full, void* out, uint64_t count, double scale, hipStream_t s);
+// Half-length real transforms (N0 = 2h): the bins of h-point complex lines ->
+// packed N0-point spectra, and the packed spectra -> h-point complex lines.
+hipError_t launch_r2c_post(const void* z, void* x, uint32_t h, uint64_t lines, double scale, const double2* hi,
+                           const double2* lo, uint32_t sh, uint32_t mask, hipStream_t s);
+hipError_t launch_c2r_pre(const void* x, void* z, uint32_t h, uint64_t lines, const double2* hi, const double2* lo,
+                          uint32_t sh, uint32_t mask, hipStream_t s);
 hipError_t launch_scale_copy(int prec, const void* in, void* out, uint64_t count, double scale,
                              hipStream_t s);
 

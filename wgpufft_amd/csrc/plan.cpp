@@ -1263,13 +1263,29 @@ struct Builder {
     }
 };
 
+// axis_kind_for_len_with_rader_max (src/runtime/axis_policy.rs:72-80)
+static int kind_for_len(const mifft_tuning& t, uint64_t n) {
+    bool ok = false;
+    factor_supported(n, &ok);
+    if (ok) return MIFFT_AXIS_MIXED;
+    if (is_prime_u64(n) && n <= t.rader_max_prime) return MIFFT_AXIS_RADER;
+    return MIFFT_AXIS_BLUESTEIN;
+}
+
+static bool axis_forced(const mifft_tuning& t, int axis) {
+    for (int i = 0; i < t.n_force_rader_axes; i++)
+        if (t.force_rader_axes[i] == axis) return true;
+    for (int i = 0; i < t.n_force_bluestein_axes; i++)
+        if (t.force_bluestein_axes[i] == axis) return true;
+    return false;
+}
+
 static int axis_kind_for(const mifft_config& c, int axis) {
     const mifft_tuning& t = c.tuning;
     for (int i = 0; i < t.n_force_rader_axes; i++)
         if (t.force_rader_axes[i] == axis) return MIFFT_AXIS_RADER;
     for (int i = 0; i < t.n_force_bluestein_axes; i++)
         if (t.force_bluestein_axes[i] == axis) return MIFFT_AXIS_BLUESTEIN;
-    // axis_kind_for_len_with_rader_max (src/runtime/axis_policy.rs:72-80)
     const uint64_t n = c.shape[axis];
     bool ok = false;
     factor_supported(n, &ok);
@@ -1414,6 +1430,34 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 op.desc = std::string(what) + " N0=" + std::to_string(N0) + " lines=" + std::to_string(outer);
                 P->ops.push_back(op);
             };
+            // Axis 0 of even length that is not fused runs as N0/2-point complex
+            // lines (the real line read as complex pairs) plus one twiddle pass,
+            // instead of widening to N0-point complex lines: half the points and
+            // two passes fewer (kern_util.hip k_r2c_post / k_c2r_pre).  Not for a
+            // forced axis 0; table key fold|f32|h<N0> = 0 turns it off.
+            const bool half_real = N0 % 2 == 0 && N0 >= 4 && !axis_forced(cfg.tuning, 0) &&
+                                   tuning_lookup("fold|f32|h" + std::to_string(N0)) != "0";
+            Builder::PostTable half_tw{};
+            auto half_op = [&](OpKind k, int src, uint64_t src_off, int dst, uint64_t dst_off, double sc,
+                               const char* what) {
+                if (!half_tw.hi) half_tw = B.post_table(N0);
+                Op op;
+                op.kind = k;
+                op.src = src;
+                op.src_off = src_off;
+                op.dst = dst;
+                op.dst_off = dst_off;
+                op.prec = P_F32;
+                op.n = (uint32_t)(N0 / 2);
+                op.count = outer;
+                op.scale = sc;
+                op.a.post_hi = half_tw.hi;
+                op.a.post_lo = half_tw.lo;
+                op.a.post_shift = half_tw.sh;
+                op.a.post_mask = half_tw.mask;
+                op.desc = std::string(what) + " N0=" + std::to_string(N0) + " lines=" + std::to_string(outer);
+                P->ops.push_back(op);
+            };
             uint64_t ws = 0;
             if (kind == MIFFT_R2C && cfg.rank == 2 && smooth0 && P->axes[1].kind == MIFFT_AXIS_MIXED &&
                 B.fold_2d(N0, cfg.shape[1], cfg.batch, false, scale, LD_REAL, ST_PACKED)) {
@@ -1421,8 +1465,15 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                 ws = packed * 8;
             } else if (kind == MIFFT_R2C) {
                 const double s0 = cfg.rank == 1 ? scale : 1.0;
-                if (!(smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false, s0, LD_REAL,
-                                           ST_PACKED))) {
+                const bool fused0 = smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false,
+                                                         s0, LD_REAL, ST_PACKED);
+                if (!fused0 && half_real) {
+                    // the real lines as N0/2-point complex lines, then their bins to packed spectra
+                    B.cur_axis = -1;
+                    ws = total * 4 + B.axis_ops(N0 / 2, kind_for_len(cfg.tuning, N0 / 2), 1, outer, B_IN, 0, B_WS, 0,
+                                                total * 4, false, 1.0);
+                    half_op(OP_R2C_POST, B_WS, 0, B_OUT, 0, s0, "r2c post");
+                } else if (!fused0) {
                     elem_op(OP_REAL_WIDEN, B_IN, 0, B_WS, 0, total, 1.0, "real widen");
                     B.cur_axis = 0;
                     ws = full_bytes + B.axis_ops(N0, P->axes[0].kind, 1, outer, B_WS, 0, B_WS, 0, full_bytes,
@@ -1460,8 +1511,18 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
                     }
                     ws += pk_bytes;
                 }
-                if (!(smooth0 && B.line_op(N0, l0, pack_l, real_l, src, src_off, B_OUT, 0, true, true, scale,
-                                           LD_HERMITIAN, ST_REAL))) {
+                const bool fused0 = smooth0 && B.line_op(N0, l0, pack_l, real_l, src, src_off, B_OUT, 0, true, true,
+                                                         scale, LD_HERMITIAN, ST_REAL);
+                if (!fused0 && half_real) {
+                    // packed spectra -> N0/2-point complex lines after the packed copy of
+                    // the outer passes, then their inverse straight into the real output
+                    const uint64_t X = cfg.rank > 1 ? pk_bytes : 0;
+                    half_op(OP_C2R_PRE, src, src_off, B_WS, X, 1.0, "c2r pre");
+                    B.cur_axis = -1;
+                    ws = std::max(ws, X + total * 4 +
+                                          B.axis_ops(N0 / 2, kind_for_len(cfg.tuning, N0 / 2), 1, outer, B_WS, X,
+                                                     B_OUT, 0, X + total * 4, true, scale));
+                } else if (!fused0) {
                     // full complex lines after the packed copy of the outer passes
                     const uint64_t X = cfg.rank > 1 ? pk_bytes : 0;
                     elem_op(OP_HERM_EXTEND, src, src_off, B_WS, X, outer, 1.0, "hermitian extend");
@@ -1581,6 +1642,16 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             hip_check(launch_real_part(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.count, op.scale, s),
                       "real part");
             break;
+        case OP_R2C_POST:
+            hip_check(launch_r2c_post(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.n, op.count, op.scale,
+                                      op.a.post_hi, op.a.post_lo, op.a.post_shift, op.a.post_mask, s),
+                      "r2c post");
+            break;
+        case OP_C2R_PRE:
+            hip_check(launch_c2r_pre(ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), op.n, op.count, op.a.post_hi,
+                                     op.a.post_lo, op.a.post_shift, op.a.post_mask, s),
+                      "c2r pre");
+            break;
     }
 }
 
@@ -1649,6 +1720,8 @@ static const char* op_kind_name(OpKind k) {
         case OP_PACK: return "real-pack";
         case OP_HERM_EXTEND: return "hermitian-extend";
         case OP_REAL_PART: return "real-part";
+        case OP_R2C_POST: return "r2c-post";
+        case OP_C2R_PRE: return "c2r-pre";
         case OP_SMALL_VOLUME: return "small-volume";
         case OP_DIRECT: return "direct-prime";
         case OP_STOCKHAM: return "stockham-pass";

@@ -36,6 +36,8 @@ enum OpKind {
     OP_PACK,        // general R2C: complex lines -> packed bins k <= N0/2
     OP_HERM_EXTEND, // general C2R: packed lines -> Hermitian-extended complex lines
     OP_REAL_PART,   // general C2R: complex lines -> real part
+    OP_R2C_POST,    // half-length R2C: h-point complex bins -> packed spectrum
+    OP_C2R_PRE,     // half-length C2R: packed spectrum -> h-point complex lines
     OP_SMALL_VOLUME, // every axis of small f32 volumes, one workgroup per volume
     OP_DIRECT,       // direct DFT of short prime lines (n = p, m = lines per workgroup, filt = W_p table)
     OP_STOCKHAM      // one radix stage of a long axis over HBM (st)
