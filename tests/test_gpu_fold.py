@@ -56,9 +56,12 @@ def test_fold_matches_oracle_and_unfolded(torch_cuda, shape, batch):
         want = c2c_expected(x, shape, batch=batch, inverse=inverse)
         cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
         W.clear_tuning()
+        if n1 < 512:  # below M = 512 the fold runs only by table entry
+            W.parse_tuning(f"fold|f32|n{n0}x{n1} = 2\n")
         plan, y = _run(cfg, x, in_place=inverse)
         desc = plan.describe()
         assert "fold2 M=%d" % n1 in desc, desc
+        W.clear_tuning()
         e = oracle.rel_l2(y, want)
         W.parse_tuning(f"fold|f32|n{n0}x{n1} = 0\n")
         plain, y0 = _run(cfg, x, in_place=inverse)
@@ -74,7 +77,7 @@ def test_fold_matches_oracle_and_unfolded(torch_cuda, shape, batch):
 def test_forced_fold_variants(torch_cuda, variant):
     shape, batch = [4096, 256], 5  # 640 row pairs: persistent forms loop over several tiles
     x = seeded_complex(77, 4096 * 256 * batch)
-    W.parse_tuning(f"{fold_key(4096, 2 * 128 * batch)} = {variant}\n")
+    W.parse_tuning(f"{fold_key(4096, 2 * 128 * batch)} = {variant}\nfold|f32|n4096x256 = 2\n")
     for inverse in (False, True):
         cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
         plan, y = _run(cfg, x, in_place=False)
@@ -122,13 +125,12 @@ def test_r2c_fold_matches_oracle(torch_cuda, shape, batch):
     xr = oracle.normal_f32(300 + n1 + n0, n0 * n1 * batch)
     want = oracle.r2c_nd(xr, shape, batch=batch, nthreads=8)
     outs = []
-    for table in ("", f"fold|f32|r{n0}x{n1} = 0\n"):
+    for table in (f"fold|f32|r{n0}x{n1} = 2\n", f"fold|f32|r{n0}x{n1} = 0\n"):
         W.clear_tuning()
-        if table:
-            W.parse_tuning(table)
+        W.parse_tuning(table)
         r2c = FftPlan.r2c(FftConfig.new_nd(shape).with_batch(batch))
         desc = r2c.describe()
-        assert ("fold2" in desc) == (not table), desc
+        assert ("fold2" in desc) == table.endswith("2\n"), desc
         pk = empty_dev(r2c.required_output_buffer_size_bytes())
         r2c.execute(to_dev(xr), pk)
         torch.cuda.synchronize()
@@ -158,13 +160,12 @@ def test_c2r_unfold_matches_oracle(torch_cuda, shape, batch):
     pk = seeded_complex(400 + n1, P0 * n1 * batch)
     want = oracle.c2r_nd(pk, shape, batch=batch, nthreads=8)
     outs = []
-    for table in ("", f"fold|f32|c{n0}x{n1} = 0\n"):
+    for table in (f"fold|f32|c{n0}x{n1} = 2\n", f"fold|f32|c{n0}x{n1} = 0\n"):
         W.clear_tuning()
-        if table:
-            W.parse_tuning(table)
+        W.parse_tuning(table)
         c2r = FftPlan.c2r(FftConfig.inverse_nd(shape).with_batch(batch))
         desc = c2r.describe()
-        assert ("unfold2" in desc) == (not table), desc
+        assert ("unfold2" in desc) == table.endswith("2\n"), desc
         d = to_dev(pk)
         out = empty_dev(n0 * n1 * batch * 4)
         c2r.execute(d, out)
@@ -180,8 +181,8 @@ def test_c2r_unfold_matches_oracle(torch_cuda, shape, batch):
 @pytest.mark.parametrize("norm", [W.Normalization.NONE, W.Normalization.Forward, W.Normalization.Orthogonal])
 def test_fold_normalizations(torch_cuda, norm):
     """The scale lands on the fold's second pass (config.rs:262-286)."""
-    shape = [1024, 256]
-    x = seeded_complex(60 + int(norm), 1024 * 256)
+    shape = [1024, 512]
+    x = seeded_complex(60 + int(norm), 1024 * 512)
     for inverse in (False, True):
         cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_normalization(norm)
         plan, y = _run(cfg, x, in_place=False)
@@ -231,3 +232,13 @@ def test_fold_recorder_graph_replay(torch_cuda):
         assert np.array_equal(from_dev(b, np.complex64).view(np.uint32), want_c.view(np.uint32)), i
         assert np.array_equal(from_dev(pk, np.complex64).view(np.uint32), want_pk.view(np.uint32)), i
         assert np.array_equal(from_dev(back, np.float32).view(np.uint32), want_r.view(np.uint32)), i
+
+
+def test_fold_default_threshold(torch_cuda):
+    """Without a table entry the fold runs from a second axis of 512 up."""
+    assert "fold2" in FftPlan.c2c(FftConfig.new_nd([1024, 512])).describe()
+    assert "fold2" not in FftPlan.c2c(FftConfig.new_nd([1024, 256])).describe()
+    assert "fold2" in FftPlan.r2c(FftConfig.new_nd([2048, 512])).describe()
+    assert "fold2" not in FftPlan.r2c(FftConfig.new_nd([2048, 256])).describe()
+    assert "unfold2" in FftPlan.c2r(FftConfig.inverse_nd([2048, 512])).describe()
+    assert "unfold2" not in FftPlan.c2r(FftConfig.inverse_nd([2048, 256])).describe()

@@ -46,8 +46,8 @@ C2C_CASES = [
     ([64, 48], 2, L(1, 1), L(0, 1)),              # offset in (staged), contiguous out
     ([120, 9], 2, L(0, 4), L(5, 2, 2500)),        # Rader/mixed ND, both strided
     ([65536], 2, L(0, 2), L(0, 1, 65536 + 40)),   # four-step, strided in, batch-strided out
-    ([1024, 256], 2, L(0, 2), L(0, 1, 1024 * 256 + 64)),  # 2D fold, strided in, batch-strided out
-    ([2048, 128], 1, L(1, 1), L(3, 1)),           # 2D fold, offset in and out (staged)
+    ([1024, 512], 2, L(0, 2), L(0, 1, 1024 * 512 + 64)),  # 2D fold, strided in, batch-strided out
+    ([2048, 512], 1, L(1, 1), L(3, 1)),           # 2D fold, offset in and out (staged)
 ]
 
 
