@@ -51,8 +51,8 @@ def parse():
                          "made by python -m wgpufft_amd.tune), a fresh autotune of this workload, or the "
                          "planner's defaults")
     ap.add_argument("--calibrate", action="store_true",
-                    help="profiling driver: 5 plain 64 MiB device copies (torch), then 5 launches "
-                         "of every kernel of the workload; prints nothing else")
+                    help="counter driver: 5 plain 64 MiB device copies (torch), then 2 x nbuf steps "
+                         "of the timed sequence; prints the launches of one step")
     return ap.parse_args()
 
 
@@ -213,78 +213,45 @@ def main():
         fill_uniform_f32(src, 99)
         for _ in range(5):
             dst.copy_(src)
-        for li in range(plans[0].launch_count()):
-            for r in range(5):
-                k = r % nbuf
-                if spec["kind"] == "r2c_c2r":
-                    plans[0].execute_launch(li, bufs[k], aux[k], None, stream)
-                else:
-                    plans[0].execute_launch(li, bufs[k], bufs[k], None, stream)
+        # then 2 * nbuf steps of the timed sequence itself (rotating buffers)
+        for i in range(2 * nbuf):
+            step(i)
         torch.cuda.synchronize()
-        print(json.dumps({"calibrate": True, "launches": [plans[0].launch_describe(i)
-                                                         for i in range(plans[0].launch_count())]}))
+        print(json.dumps({"calibrate": True, "copies": 5, "steps": 2 * nbuf,
+                          "launches_per_step": [p.launch_describe(i) for p in plans
+                                                for i in range(p.launch_count())]}))
         return
 
-    # one normalised forward+inverse step must return its input (GPU-only check)
-    rt_err = None
-    if spec["kind"] in ("c2c_pair", "r2c_c2r"):
-        ref0 = bufs[0].clone()
-        step(0)
-        torch.cuda.synchronize()
-        def as_f64(t):
-            if spec.get("precision", "f32") == "f64":
-                return t.view(torch.float64)
-            if spec.get("precision", "f32") == "df64":
-                hl = t.view(torch.float32).view(-1, 2).double()
-                return hl[:, 0] + hl[:, 1]
-            return t.view(torch.float32).double()
-        a = as_f64(ref0)
-        bcur = as_f64(bufs[0])
-        rt_err = float(torch.linalg.norm(bcur - a) / torch.linalg.norm(a))
-        bufs[0].copy_(ref0)
-        del ref0
+    # Correctness of what is timed (outside the timed region): the forward
+    # transform of buffer 0 against torch.fft in complex128 on the device (an
+    # FP64 checker independent of this library), and for pairs the round trip
+    # against the input.
+    fwd_err, rt_err = forward_check(spec, plans, bufs, aux, stream)
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
-    ms = timed(args.steps, single=False)
-    ms_single = timed(args.steps, single=True)
+    # median of 5 timed runs of K steps (rotating buffers), then the
+    # cache-resident single-buffer rate the same way
+    runs = sorted(timed(args.steps, single=False) for _ in range(5))
+    ms = runs[2]
+    ms_single = sorted(timed(args.steps, single=True) for _ in range(5))[2]
     value = spec["bytes_per_step"] / (ms * 1e-3) / 1e9
     value_single = spec["bytes_per_step"] / (ms_single * 1e-3) / 1e9
     gflops = spec["flops_per_step"] / (ms * 1e-3) / 1e9
 
-    # live per-kernel timing of every launch of the first plan (rotating buffers)
-    kernels = []
-    reps = max(20, min(200, args.steps))
-    p0 = plans[0]
-    for li in range(p0.launch_count()):
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for r in range(reps):
-            k = r % nbuf
-            if spec["kind"] == "r2c_c2r":
-                p0.execute_launch(li, bufs[k], aux[k], None, stream)
-            else:
-                p0.execute_launch(li, bufs[k], bufs[k], None, stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        kernels.append(dict(launch=li, desc=p0.launch_describe(li), us=e0.elapsed_time(e1) / reps * 1e3))
-    # algorithmic bytes of one axis pass: read + write of the array once
-    if spec["kind"] == "r2c_c2r":
-        n = shape[0]
-        pass_bytes = [4 * n * n + 8 * (n // 2 + 1) * n, 2 * 8 * (n // 2 + 1) * n]
-    else:
-        pass_bytes = [2 * spec.get("elem_bytes", 8) * spec["elems"]] * len(kernels)
-    for kinfo, pb in zip(kernels, pass_bytes):
-        kinfo["alg_bytes"] = pb
-        kinfo["gbs"] = pb / (kinfo["us"] * 1e-6) / 1e9
-    dom = max(kernels, key=lambda k: k["us"])
+    # per-launch durations inside the timed sequence: every launch of one step
+    # (all plans, in order, rotating buffers) bracketed by HIP events on the
+    # launch stream, averaged over `reps` steps
+    kernels = in_sequence_kernel_times(spec, plans, bufs, aux, stream, nbuf, max(20, min(200, args.steps)))
+    dom = max((k for k in kernels if k.get("alg_bytes")), key=lambda k: k["us"])
     traffic, traffic_src = profiled_traffic(dom["desc"], spec.get("precision", "f32"))
+    pair_bytes, pair_src = profiled_step_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom["desc"], "kernel_us": round(dom["us"], 2),
-                "alg_bytes_per_launch": dom["alg_bytes"]}
+                "alg_bytes_per_launch": dom["alg_bytes"],
+                "timing": "HIP events around each launch inside the timed step sequence",
+                "step_counter_bytes": pair_bytes, "step_counter_source": pair_src}
 
     cpu = None
     if not args.no_cpu:
@@ -316,6 +283,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 5),
+        "ms_per_step_runs": [round(r, 5) for r in runs],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -327,6 +295,7 @@ def main():
         "value_single_buffer_cache_resident": round(value_single, 1),
         "ms_per_step_single_buffer": round(ms_single, 5),
         "gflops": round(gflops, 1),
+        "forward_rel_l2_vs_fp64": fwd_err,
         "round_trip_rel_l2": rt_err,
         "kernels": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "roofline": roofline,
@@ -485,6 +454,115 @@ def run_serve(args):
     }
 
 
+def _c128(t, prec):
+    """A device byte buffer of interleaved complex values as complex128."""
+    import torch
+    if prec == "f64":
+        v = t.view(torch.float64)
+    elif prec == "df64":
+        hl = t.view(torch.float32).view(-1, 2).double()
+        v = hl[:, 0] + hl[:, 1]
+    else:
+        v = t.view(torch.float32).double()
+    return torch.view_as_complex(v.view(-1, 2))
+
+
+def _rel(y, want):
+    import torch
+    return float(torch.linalg.norm((y - want).flatten()) / torch.linalg.norm(want.flatten()))
+
+
+def forward_check(spec, plans, bufs, aux, stream):
+    """Relative L2 of the forward transform of buffer 0 against torch.fft in
+    complex128 on the device, and of the forward+inverse round trip against
+    the input; buffer 0 is restored afterwards."""
+    import torch
+    prec = spec.get("precision", "f32")
+    shape, batch = spec["shape"], spec["batch"]
+    dims = list(reversed(shape))  # axis 0 is contiguous: the last torch dim
+    if batch > 1:
+        dims = [batch] + dims
+    fft_dims = tuple(range(len(dims) - len(shape), len(dims)))
+    torch.cuda.synchronize()
+    ref0 = bufs[0].clone()
+    if spec["kind"] == "r2c_c2r":
+        plans[0].execute(bufs[0], aux[0], stream)
+        torch.cuda.synchronize()
+        x = ref0.view(torch.float32).double().view(*dims)
+        want = torch.fft.rfftn(x, dim=fft_dims)  # halves the last dim = axis 0
+        fwd = _rel(_c128(aux[0], "f32").view(want.shape), want)
+        plans[1].execute(aux[0], bufs[0], stream)
+        torch.cuda.synchronize()
+        rt = _rel(bufs[0].view(torch.float32).double(), ref0.view(torch.float32).double())
+    else:
+        x = _c128(ref0, prec).view(*dims)
+        plans[0].execute_in_place(bufs[0], stream)
+        torch.cuda.synchronize()
+        want = torch.fft.fftn(x, dim=fft_dims)
+        fwd = _rel(_c128(bufs[0], prec).view(*dims), want)
+        rt = None
+        if spec["kind"] == "c2c_pair":
+            plans[1].execute_in_place(bufs[0], stream)
+            torch.cuda.synchronize()
+            rt = _rel(_c128(bufs[0], prec), _c128(ref0, prec))
+        del want
+    bufs[0].copy_(ref0)
+    torch.cuda.synchronize()
+    del ref0
+    return fwd, rt
+
+
+def in_sequence_kernel_times(spec, plans, bufs, aux, stream, nbuf, reps):
+    """Average duration of every launch of one step as the timed region runs
+    it (all plans in order, rotating buffers), each launch bracketed by HIP
+    events on the launch stream; with its algorithmic bytes (one read and one
+    write of what the pass moves)."""
+    import torch
+    launches = [(pi, li) for pi, p in enumerate(plans) for li in range(p.launch_count())]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(launches) + 1)] for _ in range(reps)]
+    torch.cuda.synchronize()
+    for r in range(reps):
+        k = r % nbuf
+        evs[r][0].record(stream)
+        for j, (pi, li) in enumerate(launches):
+            if spec["kind"] == "r2c_c2r":
+                src, dst = (bufs[k], aux[k]) if pi == 0 else (aux[k], bufs[k])
+            else:
+                src = dst = bufs[k]
+            plans[pi].execute_launch(li, src, dst, None, stream)
+            evs[r][j + 1].record(stream)
+    torch.cuda.synchronize()
+    es = spec.get("elem_bytes", 8)
+    if spec["kind"] == "r2c_c2r":
+        n = spec["shape"][0]
+        s_r, s_p = 4 * spec["elems"], 8 * (n // 2 + 1) * (spec["elems"] // n)
+        per = {(0, 0): s_r + s_p, (0, 1): 2 * s_p, (1, 0): 2 * s_p, (1, 1): s_p + s_r}
+        if any(p.launch_count() != 2 for p in plans):
+            per = {}
+    else:
+        per = {lp: 2 * es * spec["elems"] for lp in launches}
+    out = []
+    for j, (pi, li) in enumerate(launches):
+        us = sum(evs[r][j].elapsed_time(evs[r][j + 1]) for r in range(reps)) / reps * 1e3
+        k = dict(plan=pi, launch=li, desc=plans[pi].launch_describe(li), us=us, alg_bytes=per.get((pi, li)))
+        if k["alg_bytes"]:
+            k["gbs"] = k["alg_bytes"] / (us * 1e-6) / 1e9
+        out.append(k)
+    return out
+
+
+def profiled_step_traffic(workload):
+    """HBM-side bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of one whole
+    timed step, from the committed counter summary of the timed region, or
+    (None, None)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if line.startswith(f"step_traffic_bytes: workload={workload} "):
+                return int(line.rsplit("=", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def profiled_traffic(desc, prec="f32"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on precision,
@@ -510,34 +588,56 @@ def profiled_traffic(desc, prec="f32"):
 
 
 def cpu_baseline(spec):
-    """FP64 oracle (oracle/oracle.c) on the host: one bounded sample of the workload."""
+    """The host-memory plans (CpuFftPlan restated in C++, mifft_cpu_*) on the
+    host cores: the same workload in the same precision, one warm-up, then the
+    median of 5 runs (benches/cpu_fft_bench.rs:64-84), bounded by a sample of
+    at most about 20 s of CPU work."""
     import numpy as np
 
-    import oracle
-    ncores = min(os.cpu_count() or 1, 16)
+    from wgpufft_amd import (CpuFftPlan, FftConfig, FftPrecision, cpu_isa, cpu_threads, fill_uniform_f32)
     shape, batch = spec["shape"], spec["batch"]
-    t0 = time.perf_counter()
-    if spec["kind"] == "c2c_pair":
-        x = oracle.uniform_f32(2, 2 * spec["elems"]).view(np.complex64).astype(np.complex128)
-        t0 = time.perf_counter()
-        y = oracle.c2c_nd(x, shape, batch=batch, nthreads=ncores)
-        oracle.c2c_nd(y, shape, batch=batch, inverse=True, nthreads=ncores)
-        sample = f"one forward+inverse pair of the same {shape} workload in FP64"
-    elif spec["kind"] == "c2c_fwd":
-        x = oracle.uniform_f32(1, 2 * spec["elems"]).view(np.complex64).astype(np.complex128)
-        t0 = time.perf_counter()
-        oracle.c2c_nd(x, shape, batch=batch, nthreads=ncores)
-        sample = "one forward FFT of the same N=1024 x 4096 workload in FP64"
+    prec = spec.get("precision", "f32")
+    p = {"f32": FftPrecision.F32, "f64": FftPrecision.F64, "df64": FftPrecision.Df64}[prec]
+    rng = np.random.default_rng(2)
+    if spec["kind"] == "r2c_c2r":
+        r2c = CpuFftPlan.r2c(FftConfig.new_nd(shape))
+        c2r = CpuFftPlan.c2r(FftConfig.inverse_nd(shape))
+        x = rng.standard_normal(r2c.required_input_len()).astype(np.float32)
+        pk = np.zeros(r2c.required_output_len(), np.float32)
+
+        def one():
+            r2c.execute(x, pk)
+            c2r.execute(pk, x)
+        what = "R2C+C2R pair"
     else:
-        x = oracle.normal_f32(3, spec["elems"]).astype(np.float64)
+        plans = [CpuFftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch).with_precision(p))]
+        if spec["kind"] == "c2c_pair":
+            plans.append(CpuFftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch).with_precision(p)))
+        dt = np.float64 if prec == "f64" else np.float32
+        x = rng.uniform(-1, 1, plans[0].required_input_len()).astype(dt)
+
+        def one():
+            for pl in plans:
+                if prec == "f64":
+                    pl.execute_in_place_f64(x)
+                else:
+                    pl.execute_in_place(x)
+        what = "forward+inverse pair" if len(plans) == 2 else "forward FFT"
+    t0 = time.perf_counter()
+    one()  # warm-up
+    first = time.perf_counter() - t0
+    runs = max(3, min(25, int(10.0 / max(first, 1e-6))))
+    times = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        p = oracle.r2c_nd(x, shape, nthreads=ncores)
-        oracle.c2r_nd(p, shape, nthreads=ncores)
-        sample = "one R2C+C2R pair of the same [2048,2048] workload in FP64"
-    dt = time.perf_counter() - t0
-    return {"value": round(spec["bytes_per_step"] / dt / 1e9, 3), "unit": "GB/s",
-            "ms_per_step": round(dt * 1e3, 1), "cores": ncores, "kind": "port",
-            "sample": sample + f" (oracle/oracle.c, {ncores} threads)"}
+        one()
+        times.append(time.perf_counter() - t0)
+    dt_s = sorted(times)[len(times) // 2]
+    return {"value": round(spec["bytes_per_step"] / dt_s / 1e9, 3), "unit": "GB/s",
+            "ms_per_step": round(dt_s * 1e3, 2), "cores": cpu_threads(), "kind": "port",
+            "sample": f"one {what} of the same {shape} x{batch} {prec} workload per run, median of {runs} runs "
+                      f"after one warm-up, through the host plans (mifft_cpu_*, CpuFftPlan restated in C++, "
+                      f"{cpu_isa()}, {cpu_threads()} threads)"}
 
 
 if __name__ == "__main__":

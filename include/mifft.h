@@ -169,12 +169,29 @@ mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap);
 mifft_status mifft_plan_diagnostics(const mifft_plan* p, char* buf, size_t cap, size_t* needed);
 /* Text of launch `index` (0 <= index < mifft_plan_launch_count). */
 mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char* buf, size_t cap);
+/* The tuning-table key of launch `index` ("" when the launch takes no table
+ * entry) and the name of the registered line-kernel variant it runs ("" for
+ * runtime-schedule and non-line kernels).  No single counterpart in the
+ * reference, whose pipeline-cache keys are internal (pipeline_cache.rs:2048-2840);
+ * this names the autotuner's choice per launch so that a test can force and
+ * check every variant.  Either buffer may be NULL. */
+mifft_status mifft_plan_launch_tuning(const mifft_plan* p, int32_t index, char* key, size_t key_cap, char* variant,
+                                      size_t variant_cap);
+/* The compiled register-kernel registry (the autotuner's search space: radix
+ * split, threads per line, lines per workgroup, pipelined and fold forms):
+ * its size and the name of entry `index`, the form a tuning-table choice uses. */
+int32_t mifft_registry_count(void);
+mifft_status mifft_registry_name(int32_t index, char* buf, size_t cap);
 /* Enqueues only launch `index` of one execution (no counterpart in the
  * reference: per-kernel timing for roofline measurement).  The buffers are
- * those the full execution would use; `workspace` may be NULL to use the
- * plan's own. */
+ * those the full execution would use, validated as mifft_execute validates
+ * them (sizes, 16-byte alignment); in == out only for plans that support in
+ * place (MIFFT_ERR_IN_PLACE_UNSUPPORTED otherwise: a real plan's single launch
+ * cannot take the input copy a whole in-place execution takes).  `workspace`
+ * may be NULL to use the plan's own. */
 mifft_status mifft_execute_launch(const mifft_plan* p, hipStream_t stream, int32_t index, const void* in,
-                                  void* out, void* workspace);
+                                  uint64_t in_bytes, void* out, uint64_t out_bytes, void* workspace,
+                                  uint64_t workspace_bytes);
 
 /* ------------------------------------------------------------ execution */
 /* FftPlan::execute_checked (plan.rs:510-518): input -> output on `stream`.
@@ -367,6 +384,49 @@ mifft_status mifft_tuning_table_load(const char* path);
 mifft_status mifft_tuning_table_parse(const char* text, int32_t* entries);
 void mifft_tuning_table_clear(void);
 int32_t mifft_tuning_table_size(void);
+
+/* ------------------------------------------------------ host (CPU) plans */
+/* CpuFftPlan (src/cpu.rs:52-361): the same FftConfig semantics and buffer
+ * layouts on host memory, for machines or callers without the GPU.  Buffers
+ * are scalar words: f32 for F32 and Df64 plans ([re_hi, re_lo, im_hi, im_lo]
+ * per Df64 element), f64 for F64 plans; R2C output / C2R input is the packed
+ * spectrum [floor(N0/2)+1, N1, ...].  Work is split over host threads by
+ * whole lines and tiles, so results do not depend on the thread count; long
+ * contiguous lines in batches of fewer than 16 lines run as a four-step split
+ * (cpu.rs:683-796), strided axes through tiles of neighbouring columns
+ * (cpu.rs:526-681).  Executions of one plan may run concurrently. */
+typedef struct mifft_cpu_plan mifft_cpu_plan;
+/* CpuFftPlan::c2c / r2c / c2r (cpu.rs:88-210): r2c needs the forward and c2r
+ * the inverse direction (MIFFT_ERR_INVALID_REAL_DIRECTION), every axis in
+ * order (MIFFT_ERR_UNSUPPORTED_REAL_AXES) and F32 or F64
+ * (MIFFT_ERR_PRECISION_UNSUPPORTED for Df64). */
+mifft_status mifft_cpu_plan_create(const mifft_config* c, mifft_kind kind, mifft_cpu_plan** out);
+void mifft_cpu_plan_destroy(mifft_cpu_plan* p);
+/* CpuFftPlan::kind / config / packed_shape (cpu.rs:212-225). */
+mifft_kind mifft_cpu_plan_kind(const mifft_cpu_plan* p);
+mifft_status mifft_cpu_plan_config(const mifft_cpu_plan* p, mifft_config* out);
+int32_t mifft_cpu_plan_packed_shape(const mifft_cpu_plan* p, uint64_t* out, int32_t cap);
+/* CpuFftPlan::required_input_len / required_output_len (cpu.rs:227-243), in scalar words. */
+uint64_t mifft_cpu_plan_required_input_len(const mifft_cpu_plan* p);
+uint64_t mifft_cpu_plan_required_output_len(const mifft_cpu_plan* p);
+/* CpuFftPlan::execute (cpu.rs:245-268): F32 and Df64 plans on f32 words;
+ * lengths must equal the required lengths (MIFFT_ERR_HOST_BUFFER_LENGTH);
+ * an F64 plan gives MIFFT_ERR_PRECISION_UNSUPPORTED. */
+mifft_status mifft_cpu_execute(const mifft_cpu_plan* p, const float* in, uint64_t in_len, float* out,
+                               uint64_t out_len);
+/* CpuFftPlan::execute_f64 (cpu.rs:270-283): F64 plans on f64 words. */
+mifft_status mifft_cpu_execute_f64(const mifft_cpu_plan* p, const double* in, uint64_t in_len, double* out,
+                                   uint64_t out_len);
+/* CpuFftPlan::execute_in_place / execute_in_place_f64 (cpu.rs:285-334): C2C
+ * only (MIFFT_ERR_IN_PLACE_UNSUPPORTED for real plans). */
+mifft_status mifft_cpu_execute_in_place(const mifft_cpu_plan* p, float* data, uint64_t len);
+mifft_status mifft_cpu_execute_in_place_f64(const mifft_cpu_plan* p, double* data, uint64_t len);
+/* Host threads the plans may use (0: every hardware thread, the reference's
+ * available_parallelism, cpu.rs:829-832), the count in effect, and the
+ * instruction set of the engine in use ("avx2+fma" or "x86-64"). */
+void mifft_cpu_set_max_threads(int32_t n);
+int32_t mifft_cpu_max_threads(void);
+const char* mifft_cpu_isa(void);
 
 /* --------------------------------------------------------------- errors */
 const char* mifft_status_string(mifft_status s);

@@ -112,3 +112,18 @@ def test_status_strings_and_no_throw_on_null():
     assert lib.mifft_plan_create(None, 0, None) == 17
     assert lib.mifft_execute(None, None, None, 0, None, 0) != 0
     assert b"mifft" in lib.mifft_version()
+
+
+def test_registry_names_parse_and_are_unique():
+    """mifft_registry_count / mifft_registry_name read the compiled variant table (no GPU call)."""
+    from tests.test_gpu_registry import parse_name
+    names = W.registry_names()
+    assert len(names) == len(set(names)) and len(names) > 50
+    for nm in names:
+        parse_name(nm)
+    # the committed MI355X table names only registered variants
+    table = os.path.join(ROOT, "wgpufft_amd", "tuning", "mi355x.txt")
+    for line in open(table):
+        line = line.split("#")[0].strip()
+        if "= reg:" in line:
+            assert line.split("=", 1)[1].strip() in names, line

@@ -24,7 +24,7 @@ __all__ = [
     "FftPrecision", "FftDirection", "Normalization", "FftTransformKind", "AxisKind", "C2cRoute",
     "FftTuning", "FftConfig", "FftPlan", "FftRecorder", "FftBatch", "FftError", "LibraryMissing",
     "create_plan", "create_r2c_plan", "create_c2r_plan", "fill_uniform_f32", "fill_normal_f32",
-    "FftLogicalLayout",
+    "FftLogicalLayout", "CpuFftPlan", "set_cpu_threads", "cpu_threads", "cpu_isa",
 ]
 
 
@@ -407,6 +407,14 @@ class FftPlan:
         _check(self._lib.mifft_plan_launch_describe(self._h, int(index), buf, 1024))
         return buf.value.decode()
 
+    def launch_tuning(self, index: int) -> tuple:
+        """(tuning-table key, registered variant name) of launch `index`;
+        either is "" where it does not apply."""
+        k = ctypes.create_string_buffer(256)
+        v = ctypes.create_string_buffer(256)
+        _check(self._lib.mifft_plan_launch_tuning(self._h, int(index), k, 256, v, 256))
+        return k.value.decode(), v.value.decode()
+
     def diagnostics(self) -> dict:
         """FftPlan::diagnostics (plan.rs:938-1004): route, stages, blockers,
         buffer requirements, active tuning and device limits."""
@@ -419,10 +427,10 @@ class FftPlan:
 
     def execute_launch(self, index: int, input, output, workspace=None, stream=None) -> None:
         """Enqueue only launch `index` of one execution (per-kernel timing)."""
-        ip, _ = _buf(input)
-        op, _ = _buf(output)
-        wp = _buf(workspace)[0] if workspace is not None else None
-        _check(self._lib.mifft_execute_launch(self._h, _stream(stream), int(index), ip, op, wp))
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        wp, wb = _buf(workspace) if workspace is not None else (None, 0)
+        _check(self._lib.mifft_execute_launch(self._h, _stream(stream), int(index), ip, ib, op, ob, wp, wb))
 
     # execution (plan.rs:494-836)
     def execute_checked(self, input, output, stream=None) -> None:
@@ -480,6 +488,96 @@ class FftPlan:
 
     def record_in_place(self, recorder: "FftRecorder", buffer) -> None:
         recorder._record(self, buffer, buffer)
+
+
+class CpuFftPlan:
+    """CpuFftPlan (src/cpu.rs:52-361): the same configuration and layouts on
+    host memory (numpy arrays of scalar words: float32 for F32 and Df64 plans,
+    float64 for F64 plans)."""
+
+    def __init__(self, config: FftConfig, kind: FftTransformKind):
+        lib = _lib.load()
+        self._config = config
+        self._kind = FftTransformKind(kind)
+        c = config._to_c()
+        h = ctypes.c_void_p()
+        _check(lib.mifft_cpu_plan_create(ctypes.byref(c), int(kind), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mifft_cpu_plan_destroy(h)
+            self._h = None
+
+    @classmethod
+    def c2c(cls, config: FftConfig) -> "CpuFftPlan":
+        return cls(config, FftTransformKind.C2c)
+
+    @classmethod
+    def r2c(cls, config: FftConfig) -> "CpuFftPlan":
+        return cls(config, FftTransformKind.R2c)
+
+    @classmethod
+    def c2r(cls, config: FftConfig) -> "CpuFftPlan":
+        return cls(config, FftTransformKind.C2r)
+
+    def kind(self) -> FftTransformKind: return self._kind
+    def config(self) -> FftConfig: return self._config
+
+    def packed_shape(self) -> Optional[list[int]]:
+        buf = (ctypes.c_uint64 * _lib.MAX_RANK)()
+        n = self._lib.mifft_cpu_plan_packed_shape(self._h, buf, _lib.MAX_RANK)
+        return [int(buf[i]) for i in range(n)] if n else None
+
+    def required_input_len(self) -> int: return int(self._lib.mifft_cpu_plan_required_input_len(self._h))
+    def required_output_len(self) -> int: return int(self._lib.mifft_cpu_plan_required_output_len(self._h))
+
+    @staticmethod
+    def _words(a, dtype, writable=False):
+        import numpy as np
+        if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags["C_CONTIGUOUS"] or \
+                (writable and not a.flags["WRITEABLE"]):
+            raise TypeError(f"expected a contiguous{' writable' if writable else ''} {np.dtype(dtype).name} array")
+        return a.ctypes.data_as(ctypes.c_void_p), a.size
+
+    def execute(self, input, output) -> None:
+        """F32 and Df64 plans on float32 words (cpu.rs:245-268)."""
+        import numpy as np
+        ip, il = self._words(input, np.float32)
+        op, ol = self._words(output, np.float32, True)
+        _check(self._lib.mifft_cpu_execute(self._h, ip, il, op, ol))
+
+    def execute_f64(self, input, output) -> None:
+        """F64 plans on float64 words (cpu.rs:270-283)."""
+        import numpy as np
+        ip, il = self._words(input, np.float64)
+        op, ol = self._words(output, np.float64, True)
+        _check(self._lib.mifft_cpu_execute_f64(self._h, ip, il, op, ol))
+
+    def execute_in_place(self, data) -> None:
+        import numpy as np
+        p, n = self._words(data, np.float32, True)
+        _check(self._lib.mifft_cpu_execute_in_place(self._h, p, n))
+
+    def execute_in_place_f64(self, data) -> None:
+        import numpy as np
+        p, n = self._words(data, np.float64, True)
+        _check(self._lib.mifft_cpu_execute_in_place_f64(self._h, p, n))
+
+
+def set_cpu_threads(n: int) -> None:
+    """Host threads CpuFftPlan executions may use (0: every hardware thread)."""
+    _lib.load().mifft_cpu_set_max_threads(int(n))
+
+
+def cpu_threads() -> int:
+    return int(_lib.load().mifft_cpu_max_threads())
+
+
+def cpu_isa() -> str:
+    return _lib.load().mifft_cpu_isa().decode()
 
 
 class FftRecorder:
@@ -619,6 +717,17 @@ def autotune_sequence(jobs, reps: int = 20, stream=None) -> tuple:
     _check(lib.mifft_autotune_sequence(cs, ks, len(jobs), int(reps), _stream(stream), ctypes.byref(n), buf,
                                        len(buf)))
     return n.value, buf.value.decode()
+
+
+def registry_names() -> list:
+    """Names of every compiled register-kernel variant (tuning-table choices)."""
+    lib = _lib.load()
+    out = []
+    buf = ctypes.create_string_buffer(256)
+    for i in range(int(lib.mifft_registry_count())):
+        _check(lib.mifft_registry_name(i, buf, 256))
+        out.append(buf.value.decode())
+    return out
 
 
 def save_tuning(path: str) -> None:

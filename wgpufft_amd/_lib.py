@@ -113,7 +113,10 @@ SIGNATURES = [
     ("mifft_plan_describe", i32, [vp, ctypes.c_char_p, ctypes.c_size_t]),
     ("mifft_plan_diagnostics", i32, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("mifft_plan_launch_describe", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t]),
-    ("mifft_execute_launch", i32, [vp, vp, i32, vp, vp, vp]),
+    ("mifft_execute_launch", i32, [vp, vp, i32, vp, u64, vp, u64, vp, u64]),
+    ("mifft_plan_launch_tuning", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]),
+    ("mifft_registry_count", i32, []),
+    ("mifft_registry_name", i32, [i32, ctypes.c_char_p, ctypes.c_size_t]),
     ("mifft_execute", i32, [vp, vp, vp, u64, vp, u64]),
     ("mifft_execute_with_workspace", i32, [vp, vp, vp, u64, vp, u64, vp, u64]),
     ("mifft_execute_in_place", i32, [vp, vp, vp, u64]),
@@ -158,6 +161,20 @@ SIGNATURES = [
     ("mifft_tuning_table_parse", i32, [ctypes.c_char_p, ctypes.POINTER(i32)]),
     ("mifft_tuning_table_clear", None, []),
     ("mifft_tuning_table_size", i32, []),
+    ("mifft_cpu_plan_create", i32, [ctypes.POINTER(Config), i32, ctypes.POINTER(vp)]),
+    ("mifft_cpu_plan_destroy", None, [vp]),
+    ("mifft_cpu_plan_kind", i32, [vp]),
+    ("mifft_cpu_plan_config", i32, [vp, ctypes.POINTER(Config)]),
+    ("mifft_cpu_plan_packed_shape", i32, [vp, ctypes.POINTER(u64), i32]),
+    ("mifft_cpu_plan_required_input_len", u64, [vp]),
+    ("mifft_cpu_plan_required_output_len", u64, [vp]),
+    ("mifft_cpu_execute", i32, [vp, vp, u64, vp, u64]),
+    ("mifft_cpu_execute_f64", i32, [vp, vp, u64, vp, u64]),
+    ("mifft_cpu_execute_in_place", i32, [vp, vp, u64]),
+    ("mifft_cpu_execute_in_place_f64", i32, [vp, vp, u64]),
+    ("mifft_cpu_set_max_threads", None, [i32]),
+    ("mifft_cpu_max_threads", i32, []),
+    ("mifft_cpu_isa", cp, []),
     ("mifft_fill_uniform_f32", i32, [vp, u64, u64, vp]),
     ("mifft_fill_normal_f32", i32, [vp, u64, u64, vp]),
 ]
@@ -178,6 +195,14 @@ def load() -> ctypes.CDLL:
         raise LibraryMissing(
             f"{LIB_PATH} is missing: build it with `python -m wgpufft_amd.build` "
             "(there is no CPU fallback)")
+    # torch ships its own libamdhip64.so.7 (and HSA runtime); libmifft.so
+    # needs the same soname.  Loading torch first makes both use torch's copy:
+    # loaded the other way round, torch would run on a mismatched HIP runtime
+    # and see no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -18,10 +18,15 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmifft.so")
 SOURCES = ["kern_line.hip", "kern_conv.hip", "kern_util.hip", "kern_batch.hip", "kern_small.hip", "plan.cpp",
-           "api.cpp", "batch.cpp", "serve.cpp"]
+           "api.cpp", "batch.cpp", "serve.cpp", "cpu_plan.cpp", "cpu_fft_avx2.cpp", "cpu_fft_base.cpp"]
+# Host-only translation units of the CPU FFT engine: plain C++ (no device
+# code), one built for AVX2 + FMA and one for the x86-64 baseline; the library
+# picks one at run time from what the CPU reports.
+HOST_ONLY = {"cpu_fft_avx2.cpp": ["-mavx2", "-mfma"], "cpu_fft_base.cpp": []}
 ARCH = os.environ.get("MIFFT_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 # Packed-FP32 instructions are disabled for every HIP translation unit.
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -30,6 +35,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if src in HOST_ONLY:
+        cmd = [CXX, "-O3", "-std=c++17", "-fPIC", "-Wall", *HOST_ONLY[src], "-c", os.path.join(CSRC, src),
+               "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return obj
     cmd = [HIPCC, *COMMON]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]

@@ -172,12 +172,50 @@ mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char
     return MIFFT_SUCCESS;
 }
 
+static void copy_out(const std::string& v, char* buf, size_t cap) {
+    if (!buf || cap == 0) return;
+    const size_t n = std::min(cap - 1, v.size());
+    std::memcpy(buf, v.data(), n);
+    buf[n] = '\0';
+}
+
+mifft_status mifft_plan_launch_tuning(const mifft_plan* p, int32_t index, char* key, size_t key_cap, char* variant,
+                                      size_t variant_cap) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    const Op* op = plan_launch_op(*p->p, index);
+    if (!op) return set_err(MIFFT_ERR_INVALID_ENUM, "launch index out of range");
+    copy_out(op->tkey, key, key_cap);
+    copy_out(op->kind == OP_REG && op->rv ? reg_variant_name(op->rv) : std::string(), variant, variant_cap);
+    return MIFFT_SUCCESS;
+}
+
+int32_t mifft_registry_count(void) { return reg_variant_count(); }
+
+mifft_status mifft_registry_name(int32_t index, char* buf, size_t cap) {
+    if (!buf || cap == 0) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
+    const RegVariant* v = reg_variant_at(index);
+    if (!v) return set_err(MIFFT_ERR_INVALID_ENUM, "registry index out of range");
+    copy_out(reg_variant_name(v), buf, cap);
+    return MIFFT_SUCCESS;
+}
+
+static void check_sizes(const Plan& P, const void* in, uint64_t in_bytes, const void* out, uint64_t out_bytes);
+
 mifft_status mifft_execute_launch(const mifft_plan* p, hipStream_t stream, int32_t index, const void* in,
-                                  void* out, void* workspace) {
+                                  uint64_t in_bytes, void* out, uint64_t out_bytes, void* workspace,
+                                  uint64_t workspace_bytes) {
     if (!p || !in || !out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null argument");
     return guard([&] {
-        if (!run_plan_launch(*p->p, stream, true, index, in, out, workspace ? workspace : p->p->own_ws.p,
-                             nullptr))
+        const Plan& P = *p->p;
+        check_sizes(P, in, in_bytes, out, out_bytes);
+        if (in == out && !P.in_place_kernels)
+            fail(MIFFT_ERR_IN_PLACE_UNSUPPORTED, "a single launch of this plan cannot run in place");
+        if (workspace && ((uintptr_t)workspace & 15u))
+            fail(MIFFT_ERR_UNALIGNED, "device buffers must be 16-byte aligned");
+        if (P.ws_bytes && workspace && workspace_bytes < P.ws_bytes)
+            fail(MIFFT_ERR_WORKSPACE_TOO_SMALL,
+                 "workspace needs " + std::to_string(P.ws_bytes) + " bytes, got " + std::to_string(workspace_bytes));
+        if (!run_plan_launch(P, stream, true, index, in, out, workspace ? workspace : P.own_ws.p, nullptr))
             fail(MIFFT_ERR_INVALID_ENUM, "launch index out of range");
     });
 }

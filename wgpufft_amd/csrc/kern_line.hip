@@ -181,9 +181,8 @@ static const RegVariant kRegistry[] = {
     // strided line pairs in 256-thread workgroups (the batched kernel's shape)
     REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
     REGV(FmtF32, P_F32, 512, 32, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8", 8, 8, 8),
-    REGV(FmtF32, P_F32, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16", 16, 16),
-    REGV(FmtF32, P_F32, 128, 8, 64, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x8", 16, 8),
-    REGV(FmtF32, P_F32, 64, 4, 128, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x4", 16, 4),
+    // (256, 128 and 64 points at 32, 64 and 128 line pairs are the strided
+    // defaults above)
     // persistent pipelined forms of the 4096x2048 passes
     REGP(FmtF32, P_F32, 4096, 256, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x16", 16, 16, 16),
     REGP(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, "8x8x8x8", 8, 8, 8, 8),

@@ -1693,6 +1693,24 @@ bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const vo
     return launch_at(p, s, run, &i, in, out, ws, desc);
 }
 
+static const Op* op_at(const Plan& p, int* index) {
+    for (const Op& op : p.ops) {
+        if (op.kind == OP_SUBPLAN) {
+            if (const Op* f = op_at(*op.sub, index)) return f;
+            continue;
+        }
+        if (*index == 0) return &op;
+        (*index)--;
+    }
+    return nullptr;
+}
+
+const Op* plan_launch_op(const Plan& p, int index) {
+    if (index < 0) return nullptr;
+    int i = index;
+    return op_at(p, &i);
+}
+
 std::string describe_plan(const Plan& p) {
     std::ostringstream d;
     bool first = true;

@@ -124,6 +124,8 @@ void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws)
 bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const void* in, void* out, void* ws,
                      std::string* desc);
 int plan_launch_count(const Plan& p);
+// The op behind launch `index` of the flattened sequence, or nullptr.
+const Op* plan_launch_op(const Plan& p, int index);
 
 // Process-wide tuning table: measured line-kernel choices keyed by op shape
 // (precision, length, lane layout, load/store modes, twiddle, pairability,
