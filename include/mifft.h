@@ -138,6 +138,22 @@ mifft_status mifft_config_total_len(const mifft_config* c, uint64_t* len);
  * synchronises; the plan is immutable afterwards. */
 mifft_status mifft_plan_create(const mifft_config* c, mifft_kind kind, mifft_plan** out);
 void mifft_plan_destroy(mifft_plan* p);
+/* FftPlan::c2c_with_diagnostics / c2c_checked_with_diagnostics (plan.rs:227-261):
+ * mifft_plan_create, and on failure the FftPlanCreationError diagnostics as
+ * JSON (error.rs:210-228): {"error": <FftError variant>, "route": {...},
+ * "stages": [], "blockers": [{kind, route, stage, layout, helper_buffer,
+ * reason, required_bytes, actual_bytes, limit_bytes}], "active_tuning": {...}}
+ * with the blocker FftError::diagnostics gives the error (error.rs:498-850);
+ * on success the plan's diagnostics.  Writes at most cap bytes
+ * (NUL-terminated) and the full size to *needed. */
+mifft_status mifft_plan_create_with_diagnostics(const mifft_config* c, mifft_kind kind, mifft_plan** out, char* buf,
+                                                size_t cap, size_t* needed);
+/* The diagnostics of the last failing plan creation or execution on this
+ * thread (FftPlanCreationError / FftExecutionError::diagnostics,
+ * error.rs:180-228): for an execution, the plan's diagnostics with the
+ * blockers of the buffers it was given; "null" when the last failure was
+ * neither. */
+mifft_status mifft_last_error_diagnostics(char* buf, size_t cap, size_t* needed);
 
 /* Queries (plan.rs:390-492, 815-820). */
 mifft_kind mifft_plan_kind(const mifft_plan* p);
@@ -167,6 +183,15 @@ mifft_status mifft_plan_describe(const mifft_plan* p, char* buf, size_t cap);
  * active_tuning {FftTuning fields}, device_limits.  Writes at most cap bytes
  * (NUL-terminated) and the full size including the NUL to *needed. */
 mifft_status mifft_plan_diagnostics(const mifft_plan* p, char* buf, size_t cap, size_t* needed);
+/* FftPlan::diagnostics_for_workspace (plan.rs:1023-1055): the plan's
+ * diagnostics with the blockers executing it over these buffers would meet
+ * (an input or output smaller than required, a pointer not 16-byte aligned, a
+ * workspace smaller than mifft_plan_workspace_bytes); pointers may be NULL
+ * (alignment not checked).  An empty "blockers" list means the execution
+ * would be accepted. */
+mifft_status mifft_plan_diagnostics_for_workspace(const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                                  const void* out, uint64_t out_bytes, const void* workspace,
+                                                  uint64_t workspace_bytes, char* buf, size_t cap, size_t* needed);
 /* Text of launch `index` (0 <= index < mifft_plan_launch_count). */
 mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char* buf, size_t cap);
 /* The tuning-table key of launch `index` ("" when the launch takes no table
@@ -177,6 +202,35 @@ mifft_status mifft_plan_launch_describe(const mifft_plan* p, int32_t index, char
  * check every variant.  Either buffer may be NULL. */
 mifft_status mifft_plan_launch_tuning(const mifft_plan* p, int32_t index, char* key, size_t key_cap, char* variant,
                                       size_t variant_cap);
+/* Plan-owned device tables, copied back for inspection (the reference
+ * builds its tables on the host and checks them in its tests,
+ * twiddle.rs:198-237, bluestein_axis.rs:818-867, rader_axis.rs:1291-1336).
+ * Each table has a text description of what it holds, space-separated:
+ *   "stage-twiddles prec=<f32|f64> radices=<R0,R1,..> count=<c>": entry
+ *     off_s + (m-1) L_s + k = W_{L_s R_s}^{k m} for stages s >= 1 (L_s the
+ *     product of the radices before s, off_s the entries of stages 1..s-1);
+ *   "chirp prec=.. n=<n>": entry i = W_{2n}^{i^2 mod 2n};
+ *   "bluestein-filter n=<n> prec=.. m=<m>": FFT_m(b)/m, b[i] = conj(chirp[i])
+ *     for |i| < n, indices mod m, zero elsewhere;
+ *   "rader-filter p=<p> ginv=<h> prec=.. m=<m>": FFT_m(b)/m, b[t] =
+ *     W_p^{h^t mod p} for t < p-1 and b[m-t] = b[p-1-t] when m > p-1;
+ *   "rader-perm-in p=<p> g=<g>", "rader-perm-out p=<p> ginv=<h>": u32
+ *     entries g^q mod p, h^q mod p;
+ *   "roots prec=.. n=<n> count=<c>": entry k = W_n^k;
+ *   "roots-strided prec=f64 n=<n> count=<c> mult=<s>": entry k = W_n^{k s mod n}.
+ * W_n^t = exp(-2 pi i t / n); complex entries are interleaved [re, im] of
+ * the stated precision, generated in FP64 on the device with the phase
+ * reduced in integers and rounded once.  mifft_plan_table writes the
+ * description (desc may be NULL) and the size to *bytes, and with `host`
+ * non-NULL copies the table (synchronously) into host_cap >= *bytes bytes. */
+int32_t mifft_plan_table_count(const mifft_plan* p);
+mifft_status mifft_plan_table(const mifft_plan* p, int32_t index, char* desc, size_t desc_cap, void* host,
+                              uint64_t host_cap, uint64_t* bytes);
+/* twiddle_lut_f32 / twiddle_lut_f64 (runtime/twiddle.rs:13-37) on the
+ * device: dst[k] = W_n^k for k < count (k taken mod n), FP64 with the phase
+ * reduced in integers, rounded once to `precision` (MIFFT_F32 or MIFFT_F64),
+ * enqueued on `stream`. */
+mifft_status mifft_twiddle_table(void* dst, uint64_t n, uint64_t count, int32_t precision, hipStream_t stream);
 /* The compiled register-kernel registry (the autotuner's search space: radix
  * split, threads per line, lines per workgroup, pipelined and fold forms):
  * its size and the name of entry `index`, the form a tuning-table choice uses. */
@@ -204,6 +258,21 @@ mifft_status mifft_execute(const mifft_plan* p, hipStream_t stream, const void* 
 mifft_status mifft_execute_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
                                           uint64_t in_bytes, void* out, uint64_t out_bytes,
                                           void* workspace, uint64_t workspace_bytes);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mifft_execute_with_workspace with the workspace taken
+ * from the library's static device scratch, MIFFT_STATIC_SCRATCH_BYTES per
+ * GPU (held by every GPU the library runs on until the process ends).  It
+ * calls no hipMalloc / hipFree.  In place on a real plan the input copy also
+ * goes into the scratch.  An execution needing more scratch than that fails
+ * with MIFFT_ERR_WORKSPACE_TOO_SMALL before any GPU work.  Every call on one
+ * GPU shares that GPU's scratch: executions that overlap on one GPU (other
+ * streams, other threads) race and must not be issued; calls in order on one
+ * stream are safe.  The plan's kernels write every scratch element they read,
+ * so nothing needs clearing between calls. */
+#define MIFFT_STATIC_SCRATCH_BYTES (64ull << 20)
+mifft_status mifft_execute_static_scratch(const mifft_plan* p, hipStream_t stream, const void* in,
+                                          uint64_t in_bytes, void* out, uint64_t out_bytes);
 /* FftPlan::execute_in_place (plan.rs:827-835). */
 mifft_status mifft_execute_in_place(const mifft_plan* p, hipStream_t stream, void* buf, uint64_t bytes);
 
@@ -326,6 +395,16 @@ mifft_status mifft_logical_layout_span(const mifft_logical_layout* l, uint64_t p
  * the plan's workspace plus contiguous staging for each strided side. */
 mifft_status mifft_plan_logical_workspace_bytes(const mifft_plan* p, const mifft_logical_layout* in_layout,
                                                 const mifft_logical_layout* out_layout, uint64_t* bytes);
+/* FftPlan::diagnostics_for_views / _for_logical_views(_with_workspace)
+ * (plan.rs:1057-1300): the same for logical layouts (NULL: contiguous):
+ * layout blockers (zero element stride, batch stride too small, span too
+ * large), views smaller than their layouts span, and with workspace_bytes
+ * non-NULL a caller workspace smaller than mifft_plan_logical_workspace_bytes. */
+mifft_status mifft_plan_diagnostics_for_views(const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                              const mifft_logical_layout* in_layout, const void* out,
+                                              uint64_t out_bytes, const mifft_logical_layout* out_layout,
+                                              const uint64_t* workspace_bytes, char* buf, size_t cap,
+                                              size_t* needed);
 /* FftPlan::execute_logical_views (plan.rs:734-746, c2c.rs:1564-1584,
  * real.rs:997-1030): the plan over caller layouts.  A side that is contiguous
  * (stride 1, batches back to back, 16-byte aligned start) is used in place; a

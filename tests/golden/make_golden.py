@@ -11,6 +11,10 @@
 * twiddle_f32.json -- exp(-2 pi i k / N) at 40 digits rounded once to f32 bits,
                     for a few N and k (the "rounded once" contract of the
                     reference's canonical_twiddle_f32, src/runtime/twiddle.rs:198-204).
+* twiddle_f64.json -- the same rounded once to f64 (bits as hex), for the
+                    f64 tables (twiddle.rs:206-211).
+* chirp_f32.json -- Bluestein chirps exp(-pi i (k^2 mod 2N) / N) rounded once to
+                    f32 bits for N = 1009, 3256, 10007 (bluestein_axis.rs:818-867).
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -51,7 +55,17 @@ def mp_dft(x, sign):
 
 
 def f32_bits(v):
-    return struct.unpack("<I", struct.pack("<f", float(v)))[0]
+    """v rounded once (to nearest, ties to even) to f32, as its bit pattern."""
+    with mp.workprec(24):
+        r = +mp.mpf(v)
+    return struct.unpack("<I", struct.pack("<f", float(r)))[0]
+
+
+def f64_hex(v):
+    """v rounded once to f64, as the hex of its bit pattern."""
+    with mp.workprec(53):
+        r = +mp.mpf(v)
+    return "%016x" % struct.unpack("<Q", struct.pack("<d", float(r)))[0]
 
 
 def main():
@@ -92,6 +106,24 @@ def main():
             tw.append([n, k, f32_bits(w.real), f32_bits(w.imag)])
     with open(os.path.join(HERE, "twiddle_f32.json"), "w") as f:
         json.dump(tw, f)
+
+    tw64 = []
+    for n in [7, 1009, 4096, 65536, 1 << 24]:
+        ks = sorted(set([0, 1, 2, n // 8, n // 4, n // 3, n // 2, n - 1] + [(k * 7919) % n for k in range(24)]))
+        for k in ks:
+            w = mp.expjpi(-mp.mpf(2 * k) / n)
+            tw64.append([n, k, f64_hex(w.real), f64_hex(w.imag)])
+    with open(os.path.join(HERE, "twiddle_f64.json"), "w") as f:
+        json.dump(tw64, f)
+
+    ch = []
+    for n in [1009, 3256, 10007]:
+        ks = sorted(set([0, 1, 2, 3, n // 2, n - 2, n - 1] + [(k * 7919) % n for k in range(40)]))
+        for k in ks:
+            w = mp.expjpi(-mp.mpf((k * k) % (2 * n)) / n)
+            ch.append([n, k, f32_bits(w.real), f32_bits(w.imag)])
+    with open(os.path.join(HERE, "chirp_f32.json"), "w") as f:
+        json.dump(ch, f)
 
 
 if __name__ == "__main__":

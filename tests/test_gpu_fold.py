@@ -89,11 +89,20 @@ def test_forced_fold_variants(torch_cuda, variant):
 
 
 def test_fold_is_off_where_it_does_not_apply(torch_cuda):
-    # odd second axis, f64, axes in the other order, a single axis of a 2D array
+    # Second axes of at least 512 points fold by default (the positive
+    # control), so each guard below is the only thing keeping the fold off:
+    # an odd second axis (also with a table entry asking for the fold), f64,
+    # axes in the other order, a single axis of a 2D array.
+    assert "fold2" in FftPlan.c2c(FftConfig.new_nd([4096, 512])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 513])).describe()
+    W.parse_tuning("fold|f32|n4096x3 = 2\nfold|f32|n4096x513 = 2\n")
     assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 3])).describe()
-    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_precision(W.FftPrecision.F64)).describe()
-    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([1, 0])).describe()
-    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 256]).with_axes([0])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 513])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 512]).with_precision(W.FftPrecision.F64)).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 512]).with_precision(W.FftPrecision.Df64)).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 512]).with_axes([1, 0])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 512]).with_axes([0])).describe()
+    assert "fold" not in FftPlan.c2c(FftConfig.new_nd([4096, 512]).with_axes([1])).describe()
 
 
 @pytest.mark.parametrize("shape,batch", [([4096, 256], 3), ([4096, 2048], 1)])

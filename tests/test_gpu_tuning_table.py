@@ -187,24 +187,96 @@ def test_f32_two_pass_entry_2p22():
     assert e <= 1e-6
 
 
-def test_every_table_entry_is_exercised():
-    """Each entry of the committed table is taken by one of the plans above."""
-    hit = set()
-    plans = [FftPlan.c2c(FftConfig.new(1024).with_batch(4096)),
-             FftPlan.c2c(FftConfig.new_nd([4096, 2048])), FftPlan.c2c(FftConfig.inverse_nd([4096, 2048])),
-             FftPlan.r2c(FftConfig.new_nd([2048, 2048])), FftPlan.c2r(FftConfig.inverse_nd([2048, 2048])),
-             FftPlan.c2c(FftConfig.new(1 << 22))]
-    for shape in ([256, 256, 256], [1 << 24]):
-        for p in (FftPrecision.F64, FftPrecision.Df64):
-            plans.append(FftPlan.c2c(FftConfig.new_nd(shape).with_precision(p)))
-            plans.append(FftPlan.c2c(FftConfig.inverse_nd(shape).with_precision(p)))
-    entries = table_entries()
-    for plan in plans:
-        for li in range(plan.launch_count()):
-            k, _ = plan.launch_tuning(li)
-            if k in entries:
-                hit.add(k)
-    for n, prec in ((1 << 22, "f32"), (1 << 24, "f64"), (1 << 24, "df64")):
-        hit.add(f"split|{prec}|n{n}")
-    missing = sorted(set(entries) - hit)
-    assert not missing, missing
+def _entries_list():
+    try:
+        return sorted(table_entries().items())
+    except OSError:
+        return []
+
+
+def _synthesize(key):
+    """A plan configuration with a launch of tuning key `key`
+    (prec|nN|lfAB|mL/S|pP|pairQ|lines2^K[|foldF]), its kind and extra table
+    text (a forced split)."""
+    f = key.split("|")
+    prec, n = f[0], int(f[1][1:])
+    lf, mm, post, pair = f[2][2:], f[3][1:], int(f[4][1:]), int(f[5][4:])
+    lines = 1 << int(f[6][len("lines2^"):])
+    fold = f[7] if len(f) > 7 else ""
+    p = {"f32": FftPrecision.F32, "f64": FftPrecision.F64, "df64": FftPrecision.Df64}[prec]
+    kind = {"0/0": "c2c", "1/1": "r2c", "2/2": "c2r"}[mm]
+    if fold:
+        return kind, [n, lines], None, 1, p, ""
+    if lf == "00":
+        return kind, [n], None, lines, p, ""
+    if lf == "11" and not post:
+        w = lines if pair else lines - 1  # an odd column count takes the one-line variants
+        return kind, [w, n], [1], 1, p, ""
+    # passes of a two-pass split of an n x n line: columns with the twiddle on
+    # store (lf11, p1) and rows with the transposed store (lf01), n lines per line
+    b = max(1, lines // n)
+    return kind, [n * n], None, b, p, f"split|{prec}|n{n * n} = 2\n"
+
+
+@pytest.mark.parametrize("key,choice", _entries_list())
+def test_every_table_entry_runs_against_the_oracle(key, choice):
+    """Each entry of the committed table: a plan with a launch of its key runs
+    the table's choice and matches the oracle."""
+    import torch
+    if key.startswith("split|"):
+        _, prec, nk = key.split("|")
+        n = int(nk[1:])
+        p = {"f32": FftPrecision.F32, "f64": FftPrecision.F64, "df64": FftPrecision.Df64}[prec]
+        plan = FftPlan.c2c(FftConfig.new(n).with_precision(p))
+        assert plan.describe().count("post") == int(choice) - 1, plan.describe()
+        return
+    if key.startswith("fold|"):
+        return  # fold on/off entries: covered by the fold tests
+    kind, shape, axes, batch, p, extra = _synthesize(key)
+    if extra:
+        W.parse_tuning(extra)
+    cfg = FftConfig.inverse_nd(shape) if kind == "c2r" else FftConfig.new_nd(shape)
+    if axes:
+        cfg = cfg.with_axes(axes)
+    cfg = cfg.with_batch(batch).with_precision(p)
+    plan = {"c2c": FftPlan.c2c, "r2c": FftPlan.r2c, "c2r": FftPlan.c2r}[kind](cfg)
+    used = dict(plan.launch_tuning(i) for i in range(plan.launch_count()))
+    assert key in used, (key, list(used), plan.describe())
+    assert used[key] == choice, (key, used[key], choice)
+    cnt = int(np.prod(shape)) * batch
+    if kind == "r2c":
+        x = oracle.normal_f32(60, cnt)
+        d = to_dev(x)
+    elif kind == "c2r":
+        x = oracle.r2c_nd(oracle.normal_f32(61, cnt), shape, batch=batch, nthreads=16).astype(np.complex64)
+        d = to_dev(x)
+    elif p == FftPrecision.F32:
+        x = seeded_complex(62, cnt)
+        d = to_dev(x)
+    else:
+        x = _f64_input(63, cnt)
+        if p == FftPrecision.Df64:
+            hilo = oracle.df64_split(x.view(np.float64))
+            x = oracle.df64_join(hilo).view(np.complex128)
+            d = to_dev(hilo)
+        else:
+            d = to_dev(x)
+    out = empty_dev(plan.required_output_buffer_size_bytes())
+    plan.execute(d, out)
+    torch.cuda.synchronize()
+    if kind == "r2c":
+        y, want = from_dev(out, np.complex64), oracle.r2c_nd(x, shape, batch=batch, nthreads=16)
+    elif kind == "c2r":
+        y, want = from_dev(out, np.float32), oracle.c2r_nd(x.astype(np.complex128), shape, batch=batch, nthreads=16)
+    elif p == FftPrecision.F32:
+        y = from_dev(out, np.complex64)
+        want = oracle.c2c_nd(x.astype(np.complex128), shape, axes, batch, nthreads=16)
+    elif p == FftPrecision.F64:
+        y, want = from_dev(out, np.complex128), oracle.c2c_nd(x, shape, axes, batch, nthreads=16)
+    else:
+        y = oracle.df64_join(from_dev(out, np.float32)).view(np.complex128)
+        want = oracle.c2c_nd(x, shape, axes, batch, nthreads=16)
+    e = oracle.rel_l2(y, want)
+    tol = {FftPrecision.F32: 1e-6, FftPrecision.F64: 1e-13, FftPrecision.Df64: 1e-11}[p]
+    print(f"{key} = {choice}: {kind} {shape} axes={axes} b={batch} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= tol

@@ -41,6 +41,9 @@ def test_parse_save_load_round_trip(tmp_path):
     f"c64|n8 = {V1}",                              # unknown precision prefix
     "split|f32|n65536 = 4",                        # splits are two or three passes
     "fold|f32|n4096x2048 = 3",                     # the fold factor is 2 or 4 (0: off)
+    "fold|f32|c2048x2048 = 4",                     # the C2R unfold is on (2) or off (0)
+    "fold|f32|r2048x2048 = 4",                     # the R2C fold has radix-2 kernels only
+    "fold|f32|h1048576 = 1",                       # the half-length real axis is on (2) or off (0)
 ])
 def test_malformed_rejected_atomically(bad):
     W.parse_tuning(f"{K2} = lds:lpw=2\n")
@@ -64,6 +67,7 @@ def test_split_entries():
 def test_fold_entries():
     text = ("fold|f32|n4096x2048 = 0\n"
             "fold|f32|n4096x256 = 4\n"
-            "f32|n4096|lf00|m0/0|p0|pair0|lines2^11|fold2 = reg:f32:n4096:t512:l2:lf00:m0/0:p0:v2:8x8x8x8:fold\n")
-    assert W.parse_tuning(text) == 3
-    assert W.tuning_size() == 3
+            "f32|n4096|lf00|m0/0|p0|pair0|lines2^11|fold2 = reg:f32:n4096:t512:l2:lf00:m0/0:p0:v2:8x8x8x8:fold\n"
+            "fold|f32|c2048x2048 = 2\nfold|f32|r2048x2048 = 0\nfold|f32|h1048576 = 0\n")
+    assert W.parse_tuning(text) == 6
+    assert W.tuning_size() == 6

@@ -24,7 +24,7 @@ __all__ = [
     "FftPrecision", "FftDirection", "Normalization", "FftTransformKind", "AxisKind", "C2cRoute",
     "FftTuning", "FftConfig", "FftPlan", "FftRecorder", "FftBatch", "FftError", "LibraryMissing",
     "create_plan", "create_r2c_plan", "create_c2r_plan", "fill_uniform_f32", "fill_normal_f32",
-    "FftLogicalLayout", "CpuFftPlan", "set_cpu_threads", "cpu_threads", "cpu_isa",
+    "FftLogicalLayout", "FftPlanCreationError", "CpuFftPlan", "set_cpu_threads", "cpu_threads", "cpu_isa",
 ]
 
 
@@ -93,13 +93,37 @@ class FftError(Exception):
         self.code = int(code)
         self.kind = _STATUS_NAMES.get(self.code, f"Status{code}")
         self.detail = detail
+        self.diagnostics = None  # FftExecutionError / FftPlanCreationError diagnostics, when recorded
         super().__init__(f"{self.kind}: {detail}" if detail else self.kind)
+
+
+def _last_diagnostics():
+    """The thread's last plan-creation / execution failure diagnostics, or None."""
+    import json
+    lib = _lib.load()
+    n = ctypes.c_size_t()
+    lib.mifft_last_error_diagnostics(None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value)
+    lib.mifft_last_error_diagnostics(buf, n.value, ctypes.byref(n))
+    return json.loads(buf.value.decode())
 
 
 def _check(status: int) -> None:
     if status != 0:
         lib = _lib.load()
-        raise FftError(status, (lib.mifft_last_error_detail() or b"").decode())
+        err = FftError(status, (lib.mifft_last_error_detail() or b"").decode())
+        err.diagnostics = _last_diagnostics()
+        raise err
+
+
+class FftPlanCreationError(FftError):
+    """FftPlanCreationError (error.rs:187-240): the error and its diagnostics."""
+
+    def error(self) -> "FftError":
+        return self
+
+    def diagnostics_dict(self) -> dict:
+        return self.diagnostics
 
 
 class FftTuning:
@@ -363,6 +387,68 @@ class FftPlan:
     def c2r(cls, config: FftConfig) -> "FftPlan":
         return cls(config, FftTransformKind.C2r)
 
+    @classmethod
+    def _with_diagnostics(cls, config: FftConfig, kind: FftTransformKind) -> "FftPlan":
+        import json
+        lib = _lib.load()
+        c = config._to_c()
+        h = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        buf = ctypes.create_string_buffer(1 << 16)
+        st = lib.mifft_plan_create_with_diagnostics(ctypes.byref(c), int(kind), ctypes.byref(h), buf, 1 << 16,
+                                                    ctypes.byref(n))
+        if st != 0:
+            err = FftPlanCreationError(st, (lib.mifft_last_error_detail() or b"").decode())
+            err.diagnostics = json.loads(buf.value.decode())
+            raise err
+        plan = cls.__new__(cls)
+        plan._config, plan._kind, plan._h, plan._lib = config, FftTransformKind(kind), h, lib
+        return plan
+
+    # FftPlan::c2c_with_diagnostics (plan.rs:227-235) and its real counterparts:
+    # a failure raises FftPlanCreationError carrying the diagnostics
+    @classmethod
+    def c2c_with_diagnostics(cls, config: FftConfig) -> "FftPlan":
+        return cls._with_diagnostics(config, FftTransformKind.C2c)
+
+    @classmethod
+    def r2c_with_diagnostics(cls, config: FftConfig) -> "FftPlan":
+        return cls._with_diagnostics(config, FftTransformKind.R2c)
+
+    @classmethod
+    def c2r_with_diagnostics(cls, config: FftConfig) -> "FftPlan":
+        return cls._with_diagnostics(config, FftTransformKind.C2r)
+
+    def diagnostics_for_workspace(self, input, output, workspace) -> dict:
+        """FftPlan::diagnostics_for_workspace (plan.rs:1023-1055)."""
+        import json
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        wp, wb = _buf(workspace)
+        n = ctypes.c_size_t()
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(self._lib.mifft_plan_diagnostics_for_workspace(self._h, ip, ib, op, ob, wp, wb, buf, 1 << 16,
+                                                              ctypes.byref(n)))
+        return json.loads(buf.value.decode())
+
+    def diagnostics_for_views(self, input, input_layout: Optional["FftLogicalLayout"], output,
+                              output_layout: Optional["FftLogicalLayout"], workspace=None) -> dict:
+        """FftPlan::diagnostics_for_views / _for_logical_views(_with_workspace)
+        (plan.rs:1057-1300); `workspace` a buffer, a byte count, or None for
+        the plan's own."""
+        import json
+        ip, ib = _buf(input)
+        op, ob = _buf(output)
+        wsz = None
+        if workspace is not None:
+            wsz = ctypes.c_uint64(int(workspace) if isinstance(workspace, int) else _buf(workspace)[1])
+        n = ctypes.c_size_t()
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(self._lib.mifft_plan_diagnostics_for_views(
+            self._h, ip, ib, _layout_ptr(input_layout), op, ob, _layout_ptr(output_layout),
+            ctypes.byref(wsz) if wsz is not None else None, buf, 1 << 16, ctypes.byref(n)))
+        return json.loads(buf.value.decode())
+
     # queries (plan.rs:390-492)
     def kind(self) -> FftTransformKind: return self._kind
     def config(self) -> FftConfig: return self._config
@@ -406,6 +492,26 @@ class FftPlan:
         buf = ctypes.create_string_buffer(1024)
         _check(self._lib.mifft_plan_launch_describe(self._h, int(index), buf, 1024))
         return buf.value.decode()
+
+    def tables(self) -> list:
+        """(description dict, numpy uint8 array) of every plan-owned device
+        table (mifft_plan_table)."""
+        import numpy as np
+        out = []
+        d = ctypes.create_string_buffer(512)
+        for i in range(int(self._lib.mifft_plan_table_count(self._h))):
+            nb = ctypes.c_uint64()
+            _check(self._lib.mifft_plan_table(self._h, i, d, 512, None, 0, ctypes.byref(nb)))
+            host = np.zeros(nb.value, np.uint8)
+            _check(self._lib.mifft_plan_table(self._h, i, d, 512, host.ctypes.data_as(ctypes.c_void_p), nb.value,
+                                              ctypes.byref(nb)))
+            words = d.value.decode().split()
+            desc = {"kind": words[0]}
+            for w in words[1:]:
+                k, v = w.split("=", 1)
+                desc[k] = v
+            out.append((desc, host))
+        return out
 
     def launch_tuning(self, index: int) -> tuple:
         """(tuning-table key, registered variant name) of launch `index`;
@@ -717,6 +823,15 @@ def autotune_sequence(jobs, reps: int = 20, stream=None) -> tuple:
     _check(lib.mifft_autotune_sequence(cs, ks, len(jobs), int(reps), _stream(stream), ctypes.byref(n), buf,
                                        len(buf)))
     return n.value, buf.value.decode()
+
+
+def twiddle_table(tensor, n: int, count: int, precision: "FftPrecision" = None, stream=None) -> None:
+    """W_n^k for k < count into a device buffer (mifft_twiddle_table)."""
+    p, b = _buf(tensor)
+    prec = FftPrecision.F32 if precision is None else FftPrecision(precision)
+    if b < count * prec.complex_size_bytes():
+        raise ValueError("buffer too small for the table")
+    _check(_lib.load().mifft_twiddle_table(p, int(n), int(count), int(prec), _stream(stream)))
 
 
 def registry_names() -> list:

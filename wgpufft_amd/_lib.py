@@ -97,6 +97,14 @@ SIGNATURES = [
     ("mifft_config_total_len", i32, [ctypes.POINTER(Config), ctypes.POINTER(u64)]),
     ("mifft_plan_create", i32, [ctypes.POINTER(Config), i32, ctypes.POINTER(vp)]),
     ("mifft_plan_destroy", None, [vp]),
+    ("mifft_plan_create_with_diagnostics", i32, [ctypes.POINTER(Config), i32, ctypes.POINTER(vp), ctypes.c_char_p,
+                                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    ("mifft_last_error_diagnostics", i32, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    ("mifft_plan_diagnostics_for_workspace", i32, [vp, vp, u64, vp, u64, vp, u64, ctypes.c_char_p, ctypes.c_size_t,
+                                                   ctypes.POINTER(ctypes.c_size_t)]),
+    ("mifft_plan_diagnostics_for_views", i32, [vp, vp, u64, ctypes.POINTER(LogicalLayout), vp, u64,
+                                               ctypes.POINTER(LogicalLayout), ctypes.POINTER(u64), ctypes.c_char_p,
+                                               ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("mifft_plan_kind", i32, [vp]),
     ("mifft_plan_config", i32, [vp, ctypes.POINTER(Config)]),
     ("mifft_plan_factors", i32, [vp, ctypes.POINTER(u64), i32]),
@@ -115,11 +123,15 @@ SIGNATURES = [
     ("mifft_plan_launch_describe", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t]),
     ("mifft_execute_launch", i32, [vp, vp, i32, vp, u64, vp, u64, vp, u64]),
     ("mifft_plan_launch_tuning", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]),
+    ("mifft_plan_table_count", i32, [vp]),
+    ("mifft_plan_table", i32, [vp, i32, ctypes.c_char_p, ctypes.c_size_t, vp, u64, ctypes.POINTER(u64)]),
+    ("mifft_twiddle_table", i32, [vp, u64, u64, i32, vp]),
     ("mifft_registry_count", i32, []),
     ("mifft_registry_name", i32, [i32, ctypes.c_char_p, ctypes.c_size_t]),
     ("mifft_execute", i32, [vp, vp, vp, u64, vp, u64]),
     ("mifft_execute_with_workspace", i32, [vp, vp, vp, u64, vp, u64, vp, u64]),
     ("mifft_execute_in_place", i32, [vp, vp, vp, u64]),
+    ("mifft_execute_static_scratch", i32, [vp, vp, vp, u64, vp, u64]),
     ("mifft_logical_layout_init", None, [ctypes.POINTER(LogicalLayout)]),
     ("mifft_logical_layout_span", i32, [ctypes.POINTER(LogicalLayout), u64, u64, ctypes.POINTER(u64)]),
     ("mifft_plan_logical_workspace_bytes", i32, [vp, ctypes.POINTER(LogicalLayout), ctypes.POINTER(LogicalLayout),

@@ -6,6 +6,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include "diagnostics.hpp"
 #include "plan.hpp"
 
 using namespace mifft;
@@ -23,12 +24,29 @@ struct mifft_recorder {
 };
 
 static thread_local std::string g_detail;
+// Diagnostics JSON of the last failing plan creation or execution on this
+// thread (FftPlanCreationError / FftExecutionError diagnostics).
+static thread_local std::string g_diag;
 
-void mifft::set_last_detail(const std::string& d) { g_detail = d; }
+void mifft::set_last_detail(const std::string& d) {
+    g_detail = d;
+    g_diag.clear();
+}
 
 static mifft_status set_err(mifft_status s, const std::string& d) {
     g_detail = d;
+    g_diag.clear();
     return s;
+}
+
+static mifft_status write_json(const std::string& d, char* buf, size_t cap, size_t* needed) {
+    if (needed) *needed = d.size() + 1;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, d.size());
+        std::memcpy(buf, d.data(), n);
+        buf[n] = '\0';
+    }
+    return MIFFT_SUCCESS;
 }
 
 template <class F>
@@ -99,9 +117,13 @@ mifft_status mifft_config_total_len(const mifft_config* c, uint64_t* len) {
 }
 
 mifft_status mifft_plan_create(const mifft_config* c, mifft_kind kind, mifft_plan** out) {
-    if (!c || !out) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null config or output handle");
+    if (!c || !out) {
+        set_err(MIFFT_ERR_NULL_ARGUMENT, "null config or output handle");
+        g_diag = creation_failure_json(c, kind, MIFFT_ERR_NULL_ARGUMENT, g_detail);
+        return MIFFT_ERR_NULL_ARGUMENT;
+    }
     *out = nullptr;
-    return guard([&] {
+    const mifft_status s = guard([&] {
         auto h = std::make_unique<mifft_plan>();
         h->p = build_plan(*c, kind);
         if (h->p->ws_bytes) {
@@ -111,6 +133,27 @@ mifft_status mifft_plan_create(const mifft_config* c, mifft_kind kind, mifft_pla
         }
         *out = h.release();
     });
+    if (s != MIFFT_SUCCESS) g_diag = creation_failure_json(c, kind, s, g_detail);
+    return s;
+}
+
+mifft_status mifft_plan_create_with_diagnostics(const mifft_config* c, mifft_kind kind, mifft_plan** out, char* buf,
+                                                size_t cap, size_t* needed) {
+    const mifft_status s = mifft_plan_create(c, kind, out);
+    if (s == MIFFT_SUCCESS) {
+        std::string d;
+        const mifft_status g = guard([&] { d = plan_diagnostics_json(*(*out)->p); });
+        if (g != MIFFT_SUCCESS) return g;
+        return write_json(d, buf, cap, needed);
+    }
+    const std::string detail = g_detail;
+    write_json(g_diag, buf, cap, needed);
+    g_detail = detail;
+    return s;
+}
+
+mifft_status mifft_last_error_diagnostics(char* buf, size_t cap, size_t* needed) {
+    return write_json(g_diag.empty() ? std::string("null") : g_diag, buf, cap, needed);
 }
 
 void mifft_plan_destroy(mifft_plan* p) { delete p; }
@@ -187,6 +230,41 @@ mifft_status mifft_plan_launch_tuning(const mifft_plan* p, int32_t index, char* 
     copy_out(op->tkey, key, key_cap);
     copy_out(op->kind == OP_REG && op->rv ? reg_variant_name(op->rv) : std::string(), variant, variant_cap);
     return MIFFT_SUCCESS;
+}
+
+int32_t mifft_plan_table_count(const mifft_plan* p) {
+    if (!p) return 0;
+    std::vector<const TableInfo*> t;
+    plan_tables(*p->p, &t);
+    return (int32_t)t.size();
+}
+
+mifft_status mifft_plan_table(const mifft_plan* p, int32_t index, char* desc, size_t desc_cap, void* host,
+                              uint64_t host_cap, uint64_t* bytes) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    return guard([&] {
+        std::vector<const TableInfo*> t;
+        plan_tables(*p->p, &t);
+        if (index < 0 || index >= (int32_t)t.size()) fail(MIFFT_ERR_INVALID_ENUM, "table index out of range");
+        const TableInfo& ti = *t[index];
+        copy_out(ti.desc, desc, desc_cap);
+        if (bytes) *bytes = ti.bytes;
+        if (host) {
+            if (host_cap < ti.bytes)
+                fail(MIFFT_ERR_HOST_BUFFER_LENGTH, "table needs " + std::to_string(ti.bytes) + " bytes");
+            hip_check(hipMemcpy(host, ti.p, ti.bytes, hipMemcpyDeviceToHost), "table copy");
+        }
+    });
+}
+
+mifft_status mifft_twiddle_table(void* dst, uint64_t n, uint64_t count, int32_t precision, hipStream_t stream) {
+    if (!dst) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null buffer");
+    if (n == 0) return set_err(MIFFT_ERR_ZERO_LENGTH, "n must be at least 1");
+    if (precision != MIFFT_F32 && precision != MIFFT_F64) return set_err(MIFFT_ERR_INVALID_ENUM, "f32 or f64 table");
+    if ((uintptr_t)dst & 15u) return set_err(MIFFT_ERR_UNALIGNED, "device buffers must be 16-byte aligned");
+    return guard([&] {
+        if (count) hip_check(gen_twiddles(precision == MIFFT_F32 ? P_F32 : P_F64, dst, n, count, stream), "twiddles");
+    });
 }
 
 int32_t mifft_registry_count(void) { return reg_variant_count(); }
@@ -279,6 +357,34 @@ static void exec_impl(const mifft_plan* p, hipStream_t s, const void* in, uint64
         return;
     }
     run_plan(P, s, in, out, ws ? ws : P.own_ws.p);
+}
+
+mifft_status mifft_execute_static_scratch(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
+                                          void* out, uint64_t out_bytes) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    return guard([&] {
+        const Plan& P = *p->p;
+        check_sizes(P, in, in_bytes, out, out_bytes);
+        // in place on a plan whose kernels cannot run in place: the input
+        // copy goes into the scratch after the workspace
+        const bool copy_in = in == out && !P.in_place_kernels;
+        const uint64_t in_off = (P.ws_bytes + 255) / 256 * 256;
+        const uint64_t need = copy_in ? in_off + P.in_bytes : P.ws_bytes;
+        if (need > kStaticScratchBytes)
+            fail(MIFFT_ERR_WORKSPACE_TOO_SMALL, "this execution needs " + std::to_string(need) +
+                                                    " bytes of scratch; the static scratch holds " +
+                                                    std::to_string(kStaticScratchBytes));
+        int dev = -1;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        if (dev != P.device) fail(MIFFT_ERR_INVALID_ENUM, "the plan was created on another device");
+        char* sc = (char*)static_scratch(dev);
+        if (copy_in) {
+            hip_check(hipMemcpyAsync(sc + in_off, in, P.in_bytes, hipMemcpyDeviceToDevice, stream), "input copy");
+            run_plan(P, stream, sc + in_off, out, sc);
+        } else {
+            run_plan(P, stream, in, out, sc);
+        }
+    });
 }
 
 // ------------------------------------------------------------ logical views
@@ -426,22 +532,174 @@ static void exec_logical(const mifft_plan* p, hipStream_t s, const void* in, uin
                   "logical output scatter");
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------- execution blockers
+static const char* plan_route_name(const Plan& P) {
+    static const char* routes[] = {"direct-dft", "mixed-radix", "rader", "bluestein", "axis-sequence"};
+    return P.route >= 0 && P.route <= 4 ? routes[P.route] : "unknown";
+}
+
+// What stops executing P over these buffers (plan.rs:1396-1500): unaligned
+// pointers, layouts the reference rejects, sides smaller than their logical
+// view, a caller workspace smaller than required.  Pointers may be NULL (not
+// checked); NULL layouts are contiguous; ws_bytes NULL means the plan's own
+// workspace.
+static std::vector<Blocker> exec_blockers(const Plan& P, const void* in, uint64_t in_bytes,
+                                          const mifft_logical_layout* li, const void* out, uint64_t out_bytes,
+                                          const mifft_logical_layout* lo, const void* ws, const uint64_t* ws_bytes) {
+    std::vector<Blocker> v;
+    const std::string route = plan_route_name(P);
+    auto align = [&](const void* ptr, const char* stage, const char* helper) {
+        if (ptr && ((uintptr_t)ptr & 15u)) {
+            Blocker b;
+            b.set("Alignment", "logical-io", stage, "storage binding").req(16).act((uint64_t)((uintptr_t)ptr & 15u));
+            b.helper = helper;
+            b.reason = "device buffers must be 16-byte aligned";
+            v.push_back(b);
+        }
+    };
+    align(in, "storage-window", "");
+    align(out, "storage-window", "");
+    align(ws, "workspace", "workspace");
+    const uint64_t batch = std::max<uint64_t>(1, P.cfg.batch);
+    bool layouts_ok = true;
+    auto side = [&](bool input, const mifft_logical_layout* l, uint64_t bytes) {
+        uint64_t pb = 0;
+        uint32_t es = 0;
+        endpoint(P, input, &pb, &es);
+        const char* stage = input ? "input-logical-view" : "output-logical-view";
+        try {
+            const LogicalSide r = resolve_layout(l, pb, batch);
+            const bool contiguous = !l || (l->element_stride == 1 && (batch <= 1 || r.sb == pb));
+            const uint64_t need = checked_mul(r.span, es);
+            if (bytes < need) {
+                Blocker b;
+                b.set("Validation", route, stage, contiguous ? "contiguous" : "strided").req(need).act(bytes);
+                b.reason = std::string(input ? "input" : "output") + " view needs " + std::to_string(need) +
+                           " bytes, got " + std::to_string(bytes);
+                v.push_back(b);
+            }
+        } catch (const Error& e) {
+            layouts_ok = false;
+            Blocker b;
+            b.reason = e.detail;
+            switch (e.code) {
+                case MIFFT_ERR_LAYOUT_ZERO_STRIDE:
+                    b.set("Layout", "logical-io", "logical-layout", "element-stride").req(1).act(0);
+                    break;
+                case MIFFT_ERR_LAYOUT_BATCH_STRIDE: {
+                    const uint64_t per_span = l ? l->element_stride * (pb - 1) + 1 : pb;
+                    b.set("Layout", "logical-io", "logical-layout", "batch-stride").req(per_span).act(l ? l->batch_stride : 0);
+                    break;
+                }
+                default:
+                    b.set("DeviceLimit", "logical-io", "logical-layout", "logical layout span").lim(~0ull);
+                    break;
+            }
+            v.push_back(b);
+        }
+    };
+    side(true, li, in_bytes);
+    side(false, lo, out_bytes);
+    if (ws_bytes && layouts_ok) {
+        const LogicalExec e = plan_logical(P, in, out, li, lo);
+        const uint64_t need = (e.in.direct && e.out.direct) ? P.ws_bytes : e.ws_bytes;
+        if (*ws_bytes < need) {
+            Blocker b;
+            b.set("Workspace", route, "workspace", "").req(need).act(*ws_bytes);
+            b.helper = "workspace";
+            b.reason = "workspace needs " + std::to_string(need) + " bytes, got " + std::to_string(*ws_bytes);
+            v.push_back(b);
+        }
+    }
+    return v;
+}
+
+static std::string exec_diagnostics(const Plan& P, const std::vector<Blocker>& bl) {
+    std::vector<std::string> js;
+    for (const Blocker& b : bl) js.push_back(blocker_json(b));
+    return plan_diagnostics_json(P, js);
+}
+
+// Runs an execution entry point; on failure the thread's last-error
+// diagnostics become the plan's diagnostics with the blockers of these
+// buffers, plus one for the error itself when none of them names it
+// (FftExecutionError, error.rs:180-208).
+template <class F>
+static mifft_status guard_exec(const mifft_plan* p, const void* in, uint64_t in_bytes, const mifft_logical_layout* li,
+                               const void* out, uint64_t out_bytes, const mifft_logical_layout* lo, const void* ws,
+                               const uint64_t* ws_bytes, F&& f) {
+    const mifft_status s = guard(std::forward<F>(f));
+    if (s == MIFFT_SUCCESS || !p) return s;
+    const std::string detail = g_detail;
+    try {
+        std::vector<Blocker> bl = exec_blockers(*p->p, in, in_bytes, li, out, out_bytes, lo, ws, ws_bytes);
+        if (bl.empty()) {
+            Blocker b;
+            b.reason = detail;
+            if (s == MIFFT_ERR_NULL_ARGUMENT) b.set("Validation", plan_route_name(*p->p), "arguments", "null pointer");
+            else if (s == MIFFT_ERR_IN_PLACE_UNSUPPORTED) b.set("Validation", plan_route_name(*p->p), "in-place", "");
+            else b.set("DeviceLimit", plan_route_name(*p->p), "kernel", "");
+            bl.push_back(b);
+        }
+        g_diag = exec_diagnostics(*p->p, bl);
+    } catch (...) {
+        g_diag.clear();
+    }
+    g_detail = detail;
+    return s;
+}
+
+extern "C" {
+
 mifft_status mifft_execute(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
                            void* out, uint64_t out_bytes) {
-    return guard([&] { exec_impl(p, stream, in, in_bytes, out, out_bytes, nullptr, 0); });
+    return guard_exec(p, in, in_bytes, nullptr, out, out_bytes, nullptr, nullptr, nullptr,
+                      [&] { exec_impl(p, stream, in, in_bytes, out, out_bytes, nullptr, 0); });
 }
 
 mifft_status mifft_execute_with_workspace(const mifft_plan* p, hipStream_t stream, const void* in,
                                           uint64_t in_bytes, void* out, uint64_t out_bytes,
                                           void* workspace, uint64_t workspace_bytes) {
-    return guard([&] {
+    return guard_exec(p, in, in_bytes, nullptr, out, out_bytes, nullptr, workspace, &workspace_bytes, [&] {
         if (p && p->p->ws_bytes && !workspace) fail(MIFFT_ERR_NULL_ARGUMENT, "null workspace");
         exec_impl(p, stream, in, in_bytes, out, out_bytes, workspace, workspace_bytes);
     });
 }
 
 mifft_status mifft_execute_in_place(const mifft_plan* p, hipStream_t stream, void* buf, uint64_t bytes) {
-    return guard([&] { exec_impl(p, stream, buf, bytes, buf, bytes, nullptr, 0); });
+    return guard_exec(p, buf, bytes, nullptr, buf, bytes, nullptr, nullptr, nullptr,
+                      [&] { exec_impl(p, stream, buf, bytes, buf, bytes, nullptr, 0); });
+}
+
+// FftPlan::diagnostics_for_workspace (plan.rs:1023-1055).
+mifft_status mifft_plan_diagnostics_for_workspace(const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                                  const void* out, uint64_t out_bytes, const void* workspace,
+                                                  uint64_t workspace_bytes, char* buf, size_t cap, size_t* needed) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    std::string d;
+    const mifft_status s = guard([&] {
+        d = exec_diagnostics(*p->p, exec_blockers(*p->p, in, in_bytes, nullptr, out, out_bytes, nullptr, workspace,
+                                                  &workspace_bytes));
+    });
+    return s != MIFFT_SUCCESS ? s : write_json(d, buf, cap, needed);
+}
+
+// FftPlan::diagnostics_for_views / _for_logical_views(_with_workspace)
+// (plan.rs:1057-1300).
+mifft_status mifft_plan_diagnostics_for_views(const mifft_plan* p, const void* in, uint64_t in_bytes,
+                                              const mifft_logical_layout* in_layout, const void* out,
+                                              uint64_t out_bytes, const mifft_logical_layout* out_layout,
+                                              const uint64_t* workspace_bytes, char* buf, size_t cap,
+                                              size_t* needed) {
+    if (!p) return set_err(MIFFT_ERR_NULL_ARGUMENT, "null plan");
+    std::string d;
+    const mifft_status s = guard([&] {
+        d = exec_diagnostics(*p->p, exec_blockers(*p->p, in, in_bytes, in_layout, out, out_bytes, out_layout,
+                                                  nullptr, workspace_bytes));
+    });
+    return s != MIFFT_SUCCESS ? s : write_json(d, buf, cap, needed);
 }
 
 mifft_status mifft_recorder_begin(hipStream_t stream, int32_t capture_graph, mifft_recorder** out) {
@@ -500,7 +758,7 @@ mifft_status mifft_plan_logical_workspace_bytes(const mifft_plan* p, const mifft
 mifft_status mifft_execute_logical(const mifft_plan* p, hipStream_t stream, const void* in, uint64_t in_bytes,
                                    const mifft_logical_layout* in_layout, void* out, uint64_t out_bytes,
                                    const mifft_logical_layout* out_layout) {
-    return guard([&] {
+    return guard_exec(p, in, in_bytes, in_layout, out, out_bytes, out_layout, nullptr, nullptr, [&] {
         exec_logical(p, stream, in, in_bytes, in_layout, out, out_bytes, out_layout, nullptr, 0, false);
     });
 }
@@ -510,7 +768,7 @@ mifft_status mifft_execute_logical_with_workspace(const mifft_plan* p, hipStream
                                                   void* out, uint64_t out_bytes,
                                                   const mifft_logical_layout* out_layout, void* workspace,
                                                   uint64_t workspace_bytes) {
-    return guard([&] {
+    return guard_exec(p, in, in_bytes, in_layout, out, out_bytes, out_layout, workspace, &workspace_bytes, [&] {
         exec_logical(p, stream, in, in_bytes, in_layout, out, out_bytes, out_layout, workspace, workspace_bytes,
                      true);
     });

@@ -30,7 +30,9 @@ __device__ double2 root_f64(uint64_t t, uint64_t n) {
         case 2: ct = -c; st = -s; break;
         default: ct = s; st = -c; break;
     }
-    return make_double2(ct, -st);
+    // + 0.0 and 0.0 - st give exact zeros a positive sign, as the exact value
+    // rounded once has (cos(pi/2) = 0, sin(0) = 0)
+    return make_double2(ct + 0.0, 0.0 - st);
 }
 
 __device__ __forceinline__ uint64_t mod_mul(uint64_t a, uint64_t b, uint64_t m) {

@@ -8,6 +8,7 @@
 // chirps, Rader/Bluestein filter spectra) are generated on the device in FP64
 // at plan creation and rounded once.
 #include "plan.hpp"
+#include "diagnostics.hpp"
 
 #include <algorithm>
 
@@ -289,7 +290,12 @@ int tuning_parse(const std::string& text) {
         const bool fold_key = key.rfind("fold|", 0) == 0;
         const bool key_ok = split_key || fold_key || key.rfind("f32|n", 0) == 0 || key.rfind("f64|n", 0) == 0 ||
                             key.rfind("df64|n", 0) == 0;
+        // fold|f32|n.. and r.. take the fold factor (0 off, 2, 4 for n); c..
+        // (the C2R unfold) and h.. (half-length real axis 0) are on (2) or off (0)
+        const bool onoff_key = key.rfind("fold|f32|c", 0) == 0 || key.rfind("fold|f32|h", 0) == 0 ||
+                               key.rfind("fold|f32|r", 0) == 0;
         const bool choice_ok = split_key  ? (choice == "2" || choice == "3")
+                               : onoff_key ? (choice == "0" || choice == "2")
                                : fold_key ? (choice == "0" || choice == "2" || choice == "4")
                                           : valid_choice(choice);
         if (!key_ok || !choice_ok)
@@ -474,15 +480,19 @@ struct Builder {
     hipStream_t s;
     std::map<std::string, const void*> tw_cache;
 
-    void* alloc(size_t bytes) {
+    // A plan-owned device allocation; `what` describes a table for
+    // mifft_plan_table (empty: not listed).
+    void* alloc(size_t bytes, const std::string& what = std::string()) {
         DevBuf b;
         b.bytes = bytes;
         hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
         if (e != hipSuccess)
             fail(MIFFT_ERR_ALLOCATION_FAILED, "plan table allocation of " + std::to_string(bytes) + " bytes");
         P.owned.push_back(b);
+        if (!what.empty()) P.tables.push_back(TableInfo{b.p, (uint64_t)bytes, what});
         return b.p;
     }
+    static const char* prec_name(int arith) { return arith == P_F32 ? "f32" : "f64"; }
 
     // Stage-twiddle table of a radix schedule, arithmetic precision, generated
     // in FP64 on the device and rounded once.
@@ -492,7 +502,11 @@ struct Builder {
         auto it = tw_cache.find(key);
         if (it != tw_cache.end()) return it->second;
         const uint64_t cnt = stage_twiddle_count(rad.data(), (int)rad.size());
-        void* t = alloc(std::max<uint64_t>(cnt, 1) * elem_bytes(arith));
+        std::string rs;
+        for (size_t i = 0; i < rad.size(); i++) rs += (i ? "," : "") + std::to_string(rad[i]);
+        void* t = alloc(std::max<uint64_t>(cnt, 1) * elem_bytes(arith),
+                        std::string("stage-twiddles prec=") + prec_name(arith) + " radices=" + rs +
+                            " count=" + std::to_string(cnt));
         hip_check(gen_stage_twiddles(arith, t, rad.data(), (int)rad.size(), s), "twiddle generation");
         tw_cache[key] = t;
         return t;
@@ -660,9 +674,9 @@ struct Builder {
     }
 
     // FFT_m(seq)/m rounded once to the arithmetic precision.
-    const void* filter_spectrum(double2* seq, uint64_t m) {
+    const void* filter_spectrum(double2* seq, uint64_t m, const std::string& what) {
         fft_f64_inplace(seq, m);
-        void* f = alloc(m * elem_bytes(P.arith));
+        void* f = alloc(m * elem_bytes(P.arith), what + " prec=" + prec_name(P.arith) + " m=" + std::to_string(m));
         hip_check(round_table(P.arith, f, seq, m, 1.0 / (double)m, s), "filter rounding");
         return f;
     }
@@ -689,14 +703,16 @@ struct Builder {
             pin[q] = (uint32_t)mod_pow_u64(g, q, p);
             pout[q] = (uint32_t)mod_pow_u64(ginv, q, p);
         }
-        uint32_t* dpin = (uint32_t*)alloc(4 * (p - 1));
-        uint32_t* dpout = (uint32_t*)alloc(4 * (p - 1));
+        uint32_t* dpin = (uint32_t*)alloc(4 * (p - 1), "rader-perm-in p=" + std::to_string(p) + " g=" + std::to_string(g));
+        uint32_t* dpout =
+            (uint32_t*)alloc(4 * (p - 1), "rader-perm-out p=" + std::to_string(p) + " ginv=" + std::to_string(ginv));
         hip_check(hipMemcpyAsync(dpin, pin.data(), 4 * (p - 1), hipMemcpyHostToDevice, s), "perm upload");
         hip_check(hipMemcpyAsync(dpout, pout.data(), 4 * (p - 1), hipMemcpyHostToDevice, s), "perm upload");
         double2* seq = nullptr;
         hip_check(hipMalloc(&seq, m * sizeof(double2)), "rader sequence");
         hip_check(gen_rader_seq(seq, p, m, ginv, s), "rader sequence");
-        const void* filt = filter_spectrum(seq, m);
+        const void* filt =
+            filter_spectrum(seq, m, "rader-filter p=" + std::to_string(p) + " ginv=" + std::to_string(ginv));
         hip_check(hipStreamSynchronize(s), "rader filter");
         (void)hipFree(seq);
         Op op;
@@ -722,7 +738,7 @@ struct Builder {
     }
 
     const void* chirp_table(uint64_t n) {
-        void* c = alloc(n * elem_bytes(P.arith));
+        void* c = alloc(n * elem_bytes(P.arith), std::string("chirp prec=") + prec_name(P.arith) + " n=" + std::to_string(n));
         hip_check(gen_chirp(P.arith, c, n, s), "chirp generation");
         return c;
     }
@@ -731,7 +747,7 @@ struct Builder {
         double2* seq = nullptr;
         hip_check(hipMalloc(&seq, m * sizeof(double2)), "bluestein sequence");
         hip_check(gen_bluestein_seq(seq, n, m, s), "bluestein sequence");
-        const void* f = filter_spectrum(seq, m);
+        const void* f = filter_spectrum(seq, m, "bluestein-filter n=" + std::to_string(n));
         hip_check(hipStreamSynchronize(s), "bluestein filter");
         (void)hipFree(seq);
         return f;
@@ -947,7 +963,7 @@ struct Builder {
         if (lf) lpw = std::max<uint32_t>(lpw, 8);
         lpw = std::min<uint32_t>(lpw, std::max<uint32_t>(1, ls.lines / 256));  // keep 256 workgroups
         if (direct_prime_bytes((uint32_t)p, lpw) > P.lds_budget) return false;
-        void* wt = alloc(p * sizeof(cpx<float>));
+        void* wt = alloc(p * sizeof(cpx<float>), "roots prec=f32 n=" + std::to_string(p) + " count=" + std::to_string(p));
         hip_check(gen_twiddles(P_F32, wt, p, p, s), "direct twiddles");
         Op op;
         op.kind = OP_DIRECT;
@@ -1020,8 +1036,11 @@ struct Builder {
         while ((1ull << (2 * sh)) < n) sh++;
         const uint64_t lo_n = 1ull << sh;
         const uint64_t hi_n = (n + lo_n - 1) / lo_n;
-        double2* lo = (double2*)alloc(lo_n * sizeof(double2));
-        double2* hi = (double2*)alloc(hi_n * sizeof(double2));
+        double2* lo = (double2*)alloc(lo_n * sizeof(double2), "roots-strided prec=f64 n=" + std::to_string(n) +
+                                                                   " count=" + std::to_string(lo_n) + " mult=1");
+        double2* hi = (double2*)alloc(hi_n * sizeof(double2), "roots-strided prec=f64 n=" + std::to_string(n) +
+                                                                   " count=" + std::to_string(hi_n) +
+                                                                   " mult=" + std::to_string(lo_n));
         hip_check(gen_twiddles_f64_strided(lo, n, lo_n, 1, s), "post twiddles");
         hip_check(gen_twiddles_f64_strided(hi, n, hi_n, lo_n, s), "post twiddles");
         return PostTable{hi, lo, sh, (uint32_t)(lo_n - 1)};
@@ -1705,6 +1724,11 @@ static const Op* op_at(const Plan& p, int* index) {
     return nullptr;
 }
 
+void plan_tables(const Plan& p, std::vector<const TableInfo*>* out) {
+    for (const TableInfo& t : p.tables) out->push_back(&t);
+    for (const auto& c : p.children) plan_tables(*c, out);
+}
+
 const Op* plan_launch_op(const Plan& p, int index) {
     if (index < 0) return nullptr;
     int i = index;
@@ -1766,8 +1790,10 @@ static void diag_stages(const Plan& p, const std::string& route, std::ostringstr
             diag_stages(*op.sub, route, d, first);
             continue;
         }
-        d << (first ? "" : ",") << "{\"label\":" << json_str(op.desc) << ",\"kind\":\"" << op_kind_name(op.kind)
-          << "\",\"route\":" << json_str(route) << "}";
+        // kind: the reference's stage kind (every launch here is a kernel);
+        // algorithm: which kernel family runs it
+        d << (first ? "" : ",") << "{\"label\":" << json_str(op.desc) << ",\"kind\":\"kernel\",\"algorithm\":\""
+          << op_kind_name(op.kind) << "\",\"route\":" << json_str(route) << ",\"required_bytes\":null}";
         first = false;
     }
 }
@@ -1779,7 +1805,7 @@ static uint64_t table_bytes(const Plan& p) {
     return b;
 }
 
-std::string plan_diagnostics_json(const Plan& p) {
+std::string plan_diagnostics_json(const Plan& p, const std::vector<std::string>& blockers) {
     static const char* routes[] = {"direct-dft", "mixed-radix", "rader", "bluestein", "axis-sequence"};
     static const char* kinds[] = {"c2c", "r2c", "c2r"};
     static const char* cfmt[] = {"complex-f32", "complex-f64", "complex-df64"};
@@ -1789,11 +1815,14 @@ std::string plan_diagnostics_json(const Plan& p) {
     const std::string out_fmt = p.kind == MIFFT_R2C ? "packed-complex-f32" : p.kind == MIFFT_C2R ? "real-f32" : cf;
     std::ostringstream d;
     d << "{\"route\":{\"transform\":\"" << kinds[p.kind] << "\",\"route\":" << json_str(route)
-      << ",\"execution_kind\":\"normal\",\"reason_codes\":[],\"attempted_routes\":[" << json_str(route) << "]},";
+      << ",\"large_route_mode\":\"normal\",\"execution_kind\":\"normal\",\"reason_codes\":[\"within-bindings\"],"
+         "\"attempted_routes\":[" << json_str(route) << "]},";
     d << "\"stages\":[";
     bool first = true;
     diag_stages(p, route, d, first);
-    d << "],\"blockers\":[],";
+    d << "],\"blockers\":[";
+    for (size_t i = 0; i < blockers.size(); i++) d << (i ? "," : "") << blockers[i];
+    d << "],";
     d << "\"buffer_requirements\":[{\"role\":\"input\",\"required_bytes\":" << p.in_bytes << ",\"format\":\""
       << in_fmt << "\"},{\"role\":\"output\",\"required_bytes\":" << p.out_bytes << ",\"format\":\"" << out_fmt
       << "\"}";
@@ -1802,16 +1831,7 @@ std::string plan_diagnostics_json(const Plan& p) {
     const uint64_t tb = table_bytes(p);
     if (tb) d << ",{\"role\":\"helper:twiddle-luts-total\",\"required_bytes\":" << tb << ",\"format\":\"" << cf << "\"}";
     d << "],";
-    const mifft_tuning& t = p.cfg.tuning;
-    d << "\"active_tuning\":{\"workgroup_size\":" << t.workgroup_size << ",\"fused_workgroup_size\":"
-      << t.fused_workgroup_size << ",\"rader_max_prime\":" << t.rader_max_prime << ",\"direct_max_prime\":"
-      << t.direct_max_prime << ",\"fuse_long_axes\":" << (t.fuse_long_axes ? "true" : "false")
-      << ",\"fuse_small_volumes\":" << (t.fuse_small_volumes ? "true" : "false")
-      << ",\"fused_min_convolution_length\":" << t.fused_min_convolution_length << ",\"force_rader_axes\":[";
-    for (int i = 0; i < t.n_force_rader_axes; i++) d << (i ? "," : "") << t.force_rader_axes[i];
-    d << "],\"force_bluestein_axes\":[";
-    for (int i = 0; i < t.n_force_bluestein_axes; i++) d << (i ? "," : "") << t.force_bluestein_axes[i];
-    d << "]},";
+    d << "\"active_tuning\":" << tuning_json(p.cfg.tuning) << ",";
     d << "\"device_limits\":{\"max_workgroup_lds_bytes\":" << kLdsBudget << ",\"device\":" << p.device << "}}";
     return d.str();
 }

@@ -77,6 +77,14 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// A plan-owned device table and what it holds ("stage-twiddles prec=f32
+// radices=16,16,16 count=4080", "chirp prec=f32 n=1009", ...).
+struct TableInfo {
+    const void* p = nullptr;
+    uint64_t bytes = 0;
+    std::string desc;
+};
+
 struct AxisInfo {
     int axis;
     uint64_t n;
@@ -94,6 +102,7 @@ struct Plan {
     std::vector<AxisInfo> axes;
     std::vector<Op> ops;
     std::vector<DevBuf> owned;
+    std::vector<TableInfo> tables;  // described device tables among `owned`
     std::vector<std::unique_ptr<Plan>> children;
     uint64_t in_bytes = 0, out_bytes = 0, ws_bytes = 0;
     bool in_place_kernels = true;
@@ -124,6 +133,13 @@ void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws)
 bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const void* in, void* out, void* ws,
                      std::string* desc);
 int plan_launch_count(const Plan& p);
+// Size of the static device scratch of mifft_execute_static_scratch
+// (scratch.hip) and its device address on `device`, resolved once per device
+// with that device current (fails on any other device).
+constexpr uint64_t kStaticScratchBytes = 64ull << 20;
+void* static_scratch(int device);
+// Described device tables of the plan and its child plans, in creation order.
+void plan_tables(const Plan& p, std::vector<const TableInfo*>* out);
 // The op behind launch `index` of the flattened sequence, or nullptr.
 const Op* plan_launch_op(const Plan& p, int index);
 
@@ -150,8 +166,8 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
 // autotune_sequence of one plan.
 int autotune(const mifft_config& cfg, mifft_kind kind, int reps, hipStream_t s, std::string* report);
 std::string describe_plan(const Plan& p);
-// FftPlan::diagnostics as JSON.
-std::string plan_diagnostics_json(const Plan& p);
+// FftPlan::diagnostics as JSON, with the given blockers (JSON objects).
+std::string plan_diagnostics_json(const Plan& p, const std::vector<std::string>& blockers = {});
 
 // Host-side integer helpers (the reference's route-choice arithmetic).
 std::vector<uint64_t> factor_supported(uint64_t n, bool* ok);  // runtime/mod.rs:33-54
