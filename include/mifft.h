@@ -370,6 +370,36 @@ mifft_status mifft_server_prepare(mifft_server* s, const mifft_server_request* r
  * batching; mode 1: sequential per-plan dispatch on one stream (the baseline). */
 mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* reqs, const double* arrival_s,
                                  int32_t n, int32_t mode, mifft_server_result* results);
+/* Online serving (no counterpart in the reference): a request of any shape is
+ * submitted as it arrives; its plans are built on first sight of its shape
+ * and kept resident (a bounded cache, least recently used shapes evicted when
+ * nothing queued or in flight uses them); it waits in a FIFO queue and is
+ * admitted, by the rules of mifft_server_replay, into the next tick on a free
+ * stream slot while the bytes on the GPU stay within the budget.  Submission
+ * and polling advance the loop; nothing runs in the background.  The caller
+ * keeps the request's buffers until its completion is polled.  Results do not
+ * depend on what else is in flight (bit for bit).  A request that can never
+ * fit fails at submission with MIFFT_ERR_OVER_BUDGET. */
+typedef struct mifft_server_completion {
+    uint64_t ticket;  /* from mifft_server_submit */
+    double submit_s;  /* seconds from the server's first submission */
+    double admit_s;   /* its tick was enqueued */
+    double done_s;    /* its completion was observed */
+    int32_t status;   /* MIFFT_SUCCESS */
+    int32_t path;     /* 0 batched launch, 1 resident plan */
+} mifft_server_completion;
+mifft_status mifft_server_submit(mifft_server* s, const mifft_server_request* req, uint64_t* ticket);
+/* Advances the loop and returns up to `cap` completions in completion order;
+ * with wait != 0 blocks (polling the device) until at least one completion is
+ * available or nothing is queued or in flight. */
+mifft_status mifft_server_poll(mifft_server* s, int32_t wait, mifft_server_completion* out, int32_t cap,
+                               int32_t* count);
+/* Requests queued, in flight, and completed but not yet polled. */
+mifft_status mifft_server_pending(const mifft_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* completed);
+/* Resident request shapes of the online path (default 256, at least 1) and how many are cached now. */
+mifft_status mifft_server_set_plan_cache(mifft_server* s, uint64_t max_shapes);
+uint64_t mifft_server_cached_shapes(const mifft_server* s);
+/* Waits for the server's work on the GPU, then frees it. */
 void mifft_server_destroy(mifft_server* s);
 
 /* ------------------------------------------------------ logical views */

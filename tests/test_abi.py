@@ -67,9 +67,14 @@ def test_validation_errors():
     assert err(W.FftConfig.new_nd([4, 4]).with_axes([1, 1])) == "DuplicateAxis"
     assert err(W.FftConfig.new_nd([1, 4])) == "UnsupportedLength"  # a length-1 axis of a >1-point transform
     W.FftConfig.new_nd([1, 1]).validate()  # one point total is the direct route
-    assert err(W.FftConfig.new_nd([65536, 65536])) == "LengthTooLarge"  # > u32::MAX elements
-    assert err(W.FftConfig.new(2 ** 20).with_batch(2 ** 12)) == "LengthTooLarge"
+    assert err(W.FftConfig.new_nd([65536, 65536])) == "LengthTooLarge"  # one transform > u32::MAX elements
+    assert err(W.FftConfig.new(2 ** 33)) == "LengthTooLarge"
     W.FftConfig.new(2 ** 20).with_batch(2 ** 12 - 1).validate()
+    # more batches than u32::MAX elements: the large-chunk route (independent
+    # batch chunks), up to 2^59 elements
+    W.FftConfig.new(2 ** 20).with_batch(2 ** 12).validate()
+    W.FftConfig.new_nd([65536, 65535]).with_batch(3).validate()
+    assert err(W.FftConfig.new(2 ** 30).with_batch(2 ** 30)) == "LengthTooLarge"
 
 
 def test_tuning_validation_errors():

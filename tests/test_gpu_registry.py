@@ -80,7 +80,7 @@ def _pool():
         P.append(("c2c", [n0, 4096], None, 1, "f32", ""))
         P.append(("r2c", [n0, 512], None, 1, "f32", ""))
         P.append(("c2r", [n0, 512], None, 1, "f32", ""))
-    for lg in range(12, 23):
+    for lg in range(12, 25):
         for split in ("2", "3"):
             b = 2 if lg <= 20 else 1
             P.append(("c2c", [1 << lg], None, b, "f32", f"split|f32|n{1 << lg} = {split}\n"))

@@ -29,14 +29,21 @@ def expected(spec, x_bytes):
     return oracle.c2c_nd(xc, spec["shape"], batch=spec["batch"], nthreads=8)
 
 
-def small_mix(count, seed):
+def small_mix(count, seed, large=False):
     specs = [s for s in S.request_mix(count * 3, seed) if np.prod(s["shape"]) <= 300_000][:count]
+    if large:
+        # the serving mix's largest requests: 2^19..2^20-point 1D and 1024^2 2D
+        specs += [dict(kind=S.C2C, precision=S.F32, shape=[1 << 19], batch=1),
+                  dict(kind=S.C2C, precision=S.F32, shape=[1 << 20], batch=1),
+                  dict(kind=S.C2C, precision=S.F64, shape=[786432], batch=1),
+                  dict(kind=S.C2C, precision=S.F32, shape=[1024, 1024], batch=1),
+                  dict(kind=S.R2C, precision=S.F32, shape=[1024, 1024], batch=1)]
     return specs
 
 
 def test_serve_mix_matches_oracle_and_alone(torch_cuda):
     import torch
-    specs = small_mix(40, 12)
+    specs = small_mix(40, 12, large=True)
     bufs = S.allocate(specs)
     # f64 inputs: the f32 fill gives raw bytes; replace by finite f64 values
     for s, (x, _) in zip(specs, bufs):

@@ -78,6 +78,17 @@ class ServerRequest(ctypes.Structure):
     ]
 
 
+class ServerCompletion(ctypes.Structure):
+    _fields_ = [
+        ("ticket", u64),
+        ("submit_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("status", i32),
+        ("path", i32),
+    ]
+
+
 class ServerResult(ctypes.Structure):
     _fields_ = [
         ("arrival_s", ctypes.c_double),
@@ -162,6 +173,11 @@ SIGNATURES = [
     ("mifft_server_replay", i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(ctypes.c_double), i32, i32,
                                   ctypes.POINTER(ServerResult)]),
     ("mifft_server_destroy", None, [vp]),
+    ("mifft_server_submit", i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(u64)]),
+    ("mifft_server_poll", i32, [vp, i32, ctypes.POINTER(ServerCompletion), i32, ctypes.POINTER(i32)]),
+    ("mifft_server_pending", i32, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    ("mifft_server_set_plan_cache", i32, [vp, u64]),
+    ("mifft_server_cached_shapes", u64, [vp]),
     ("mifft_status_string", cp, [i32]),
     ("mifft_last_error_detail", cp, []),
     ("mifft_version", cp, []),

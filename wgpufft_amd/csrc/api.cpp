@@ -478,6 +478,9 @@ static void exec_logical(const mifft_plan* p, hipStream_t s, const void* in, uin
     Plan& P = *p->p;
     const LogicalExec e = plan_logical(P, in, out, li, lo);
     const uint64_t batch = std::max<uint64_t>(1, P.cfg.batch);
+    if (P.large_chunk && !(e.in.direct && e.out.direct))
+        fail(MIFFT_ERR_LAYOUT_TOO_LARGE, "strided logical views of large-chunk plans (more than u32::MAX elements) "
+                                         "are not supported; use contiguous views");
     const uint64_t need_in = checked_mul(e.in.span, e.in_es), need_out = checked_mul(e.out.span, e.out_es);
     if (in_bytes < need_in)
         fail(MIFFT_ERR_LAYOUT_OUT_OF_BOUNDS, "input layout is out of bounds: required " + std::to_string(need_in) +
@@ -603,8 +606,13 @@ static std::vector<Blocker> exec_blockers(const Plan& P, const void* in, uint64_
     side(true, li, in_bytes);
     side(false, lo, out_bytes);
     if (ws_bytes && layouts_ok) {
-        const LogicalExec e = plan_logical(P, in, out, li, lo);
-        const uint64_t need = (e.in.direct && e.out.direct) ? P.ws_bytes : e.ws_bytes;
+        // plain executions take the plan's workspace; logical views add
+        // staging for their strided sides
+        uint64_t need = P.ws_bytes;
+        if (li || lo) {
+            const LogicalExec e = plan_logical(P, in, out, li, lo);
+            need = (e.in.direct && e.out.direct) ? P.ws_bytes : e.ws_bytes;
+        }
         if (*ws_bytes < need) {
             Blocker b;
             b.set("Workspace", route, "workspace", "").req(need).act(*ws_bytes);

@@ -107,6 +107,12 @@ static std::string config_key(const mifft_config& c, mifft_kind kind) {
     return k;
 }
 
+void BatchPlanCache::erase(const mifft_config& c, mifft_kind kind) {
+    const std::string key = config_key(c, kind);
+    plans.erase(key);
+    rejected.erase(key);
+}
+
 const Plan* BatchPlanCache::get(const mifft_config& c, mifft_kind kind, std::string* why) {
     const std::string key = config_key(c, kind);
     auto f = plans.find(key);

@@ -30,6 +30,8 @@ struct BatchPlanCache {
     BatchPlanCache();
     const Plan* get_1d(uint64_t n, int direction, int normalization, int algorithm, std::string* why = nullptr);
     const Plan* get(const mifft_config& c, mifft_kind kind, std::string* why = nullptr);
+    // Drops the cached plan (or rejection) of a configuration.
+    void erase(const mifft_config& c, mifft_kind kind);
 };
 
 // One transform of a batch: a batched-mode plan, its line multiplier (the

@@ -2,7 +2,7 @@
 //
 // Twiddles, Bluestein chirps and the Rader/Bluestein filter sequences are
 // generated in FP64 on the device with the phase reduced exactly in integers
-// (t*4 mod n for the quadrant, n^2 mod 2N for the chirp, g^-t mod p for Rader),
+// (t*8 mod n for the octant, n^2 mod 2N for the chirp, g^-t mod p for Rader),
 // evaluated with the ROCm device-library sincospi, and rounded once to the
 // table type.  The reference builds the same values on the host in f64
 // (src/runtime/twiddle.rs:198-217, src/runtime/bluestein_axis.rs:818-867,
@@ -13,22 +13,29 @@
 
 namespace mifft {
 
-// W_n^t = exp(-2*pi*i*t/n) in FP64: exact quadrant reduction, then
-// sincospi on r/(2n) with r < n.
+// W_n^t = exp(-2*pi*i*t/n) in FP64.  The angle theta = 2 pi t / n is reduced
+// exactly in integers to an octant: 8t = o n + r (0 <= r < n), theta =
+// (pi/4)(o + r/n).  sincospi then only sees x = r/(4n) (even octants) or
+// (n - r)/(4n) (odd octants, the complement), both in [0, 1/4], where sin and
+// cos keep their relative accuracy (no value near zero is formed by
+// cancellation); the octant's symmetry gives (cos theta, sin theta).
 __device__ double2 root_f64(uint64_t t, uint64_t n) {
     t %= n;
-    const unsigned __int128 t4 = (unsigned __int128)t * 4u;
-    const uint64_t q = (uint64_t)(t4 / n);
-    const uint64_t r = (uint64_t)(t4 - (unsigned __int128)q * n);
-    const double x = (double)r / (2.0 * (double)n);
+    const unsigned __int128 t8 = (unsigned __int128)t * 8u;
+    const uint64_t o = (uint64_t)(t8 / n);
+    const uint64_t r = (uint64_t)(t8 - (unsigned __int128)o * n);
+    const bool odd = o & 1u;
+    const double x = (double)(odd ? n - r : r) / (4.0 * (double)n);
     double s, c;
     sincospi(x, &s, &c);
+    // angle within the quarter: alpha = phi (even octant) or pi/2 - phi' (odd)
+    const double ca = odd ? s : c, sa = odd ? c : s;
     double ct, st;
-    switch (q & 3u) {
-        case 0: ct = c; st = s; break;
-        case 1: ct = -s; st = c; break;
-        case 2: ct = -c; st = -s; break;
-        default: ct = s; st = -c; break;
+    switch ((o >> 1) & 3u) {
+        case 0: ct = ca; st = sa; break;
+        case 1: ct = -sa; st = ca; break;
+        case 2: ct = -ca; st = -sa; break;
+        default: ct = sa; st = -ca; break;
     }
     // + 0.0 and 0.0 - st give exact zeros a positive sign, as the exact value
     // rounded once has (cos(pi/2) = 0, sin(0) = 0)

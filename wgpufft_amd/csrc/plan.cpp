@@ -165,12 +165,21 @@ static void validate_tuning(const mifft_config& c) {
     }
 }
 
+// prod(shape) * batch.  One transform is limited to u32::MAX elements (the
+// reference's dispatch limit, config.rs:218-229); more batches than that span
+// run as a large-chunk plan of independent batch chunks, up to 2^59 elements
+// (byte offsets of 16-byte elements stay within 63 bits).
 uint64_t config_total(const mifft_config& c) {
-    unsigned __int128 t = c.batch;
+    unsigned __int128 per = 1;
     for (int i = 0; i < c.rank; i++) {
-        t *= c.shape[i];
-        if (t > 0xFFFFFFFFull) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "prod(shape)*batch exceeds u32::MAX elements");
+        per *= c.shape[i];
+        if (per > 0xFFFFFFFFull)
+            fail(MIFFT_ERR_LENGTH_TOO_LARGE, "prod(shape) exceeds u32::MAX elements in one transform");
     }
+    const unsigned __int128 t = per * c.batch;
+    if (t > 0xFFFFFFFFull && c.batch < 2)
+        fail(MIFFT_ERR_LENGTH_TOO_LARGE, "prod(shape)*batch exceeds u32::MAX elements");
+    if (t > (1ull << 59)) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "prod(shape)*batch exceeds 2^59 elements");
     return (uint64_t)t;
 }
 
@@ -1316,6 +1325,64 @@ static int axis_kind_for(const mifft_config& c, int axis) {
     return MIFFT_AXIS_BLUESTEIN;
 }
 
+// Large-chunk route (the reference's LargeChunkPlan, runtime/large_chunk.rs:5-120,
+// large_policy.rs:325): a configuration of more than u32::MAX elements is
+// split into chunks of whole batches of at most kChunkElems elements, each an
+// ordinary plan run at its 64-bit byte offsets, one after another on the
+// caller's stream (sharing one workspace).  Every kernel therefore keeps
+// 32-bit element indices within a chunk.
+static constexpr uint64_t kChunkElems = 1ull << 31;
+
+static std::unique_ptr<Plan> build_large_chunk_plan(const mifft_config& cfg, mifft_kind kind,
+                                                    const PlanOptions& opt) {
+    uint64_t per = 1;
+    for (int i = 0; i < cfg.rank; i++) per *= cfg.shape[i];
+    const uint64_t total = per * cfg.batch;
+    const uint64_t chunk_batch = std::max<uint64_t>(1, std::min<uint64_t>(cfg.batch, kChunkElems / per));
+    auto P = std::make_unique<Plan>();
+    P->lds_budget = opt.lds_budget;
+    P->cfg = cfg;
+    P->kind = kind;
+    P->prec = cfg.precision;
+    P->arith = cfg.precision == MIFFT_F32 ? P_F32 : P_F64;
+    P->large_chunk = true;
+    hip_check(hipGetDevice(&P->device), "hipGetDevice");
+    auto child = [&](uint64_t b) -> Plan* {
+        mifft_config c = cfg;
+        c.batch = b;
+        P->children.push_back(build_plan(c, kind, opt));
+        return P->children.back().get();
+    };
+    Plan* full = child(chunk_batch);
+    const uint64_t rem = cfg.batch % chunk_batch;
+    Plan* tail = rem ? child(rem) : nullptr;
+    P->route = full->route;
+    P->axes = full->axes;
+    P->in_place_kernels = full->in_place_kernels;
+    const uint64_t in_pb = full->in_bytes / chunk_batch, out_pb = full->out_bytes / chunk_batch;
+    P->in_bytes = in_pb * cfg.batch;
+    P->out_bytes = out_pb * cfg.batch;
+    P->ws_bytes = std::max(full->ws_bytes, tail ? tail->ws_bytes : 0);
+    for (uint64_t b0 = 0, k = 0; b0 < cfg.batch; b0 += chunk_batch, k++) {
+        Plan* sub = b0 + chunk_batch <= cfg.batch ? full : tail;
+        Op op;
+        op.kind = OP_CHUNK;
+        op.sub = sub;
+        op.src = B_IN;
+        op.src_off = b0 * in_pb;
+        op.dst = B_OUT;
+        op.dst_off = b0 * out_pb;
+        op.prec = P->prec;
+        op.count = sub->cfg.batch;
+        std::ostringstream d;
+        d << "large-chunk " << k << " batches " << b0 << ".." << (b0 + sub->cfg.batch) << " of " << cfg.batch;
+        op.desc = d.str();
+        P->ops.push_back(op);
+    }
+    (void)total;
+    return P;
+}
+
 std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const PlanOptions& opt) {
     validate_config(cfg);
     if (kind != MIFFT_C2C && kind != MIFFT_R2C && kind != MIFFT_C2R)
@@ -1332,6 +1399,10 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
         if (cfg.precision != MIFFT_F32)
             fail(MIFFT_ERR_PRECISION_UNSUPPORTED,
                  cfg.precision == MIFFT_F64 ? "real-f64-not-implemented" : "real-df64-not-implemented");
+    }
+    if (config_total(cfg) > 0xFFFFFFFFull) {
+        if (opt.batch_mode) fail(MIFFT_ERR_LENGTH_TOO_LARGE, "batched launches take at most u32::MAX elements");
+        return build_large_chunk_plan(cfg, kind, opt);
     }
     auto P = std::make_unique<Plan>();
     P->lds_budget = opt.lds_budget;
@@ -1644,6 +1715,9 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             run_plan(*op.sub, s, b, b, (char*)ws + op.sub_ws_off);
             break;
         }
+        case OP_CHUNK:
+            run_plan(*op.sub, s, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off), ws);
+            break;
         case OP_SCALE_COPY:
             hip_check(launch_scale_copy(op.prec, ptr(op.src, op.src_off), ptr(op.dst, op.dst_off),
                                         op.count, op.scale, s),
@@ -1683,7 +1757,7 @@ void run_plan(const Plan& p, hipStream_t s, const void* in, void* out, void* ws)
 
 int plan_launch_count(const Plan& p) {
     int c = 0;
-    for (const Op& op : p.ops) c += op.kind == OP_SUBPLAN ? plan_launch_count(*op.sub) : 1;
+    for (const Op& op : p.ops) c += (op.kind == OP_SUBPLAN || op.kind == OP_CHUNK) ? plan_launch_count(*op.sub) : 1;
     return c;
 }
 
@@ -1693,6 +1767,11 @@ static bool launch_at(const Plan& p, hipStream_t s, bool run, int* index, const 
         if (op.kind == OP_SUBPLAN) {
             char* b = (op.dst == B_IN ? (char*)in : op.dst == B_OUT ? (char*)out : (char*)ws) + op.dst_off;
             if (launch_at(*op.sub, s, run, index, b, b, (char*)ws + op.sub_ws_off, desc)) return true;
+            continue;
+        }
+        if (op.kind == OP_CHUNK) {
+            if (launch_at(*op.sub, s, run, index, (const char*)in + op.src_off, (char*)out + op.dst_off, ws, desc))
+                return true;
             continue;
         }
         if (*index == 0) {
@@ -1714,7 +1793,7 @@ bool run_plan_launch(const Plan& p, hipStream_t s, bool run, int index, const vo
 
 static const Op* op_at(const Plan& p, int* index) {
     for (const Op& op : p.ops) {
-        if (op.kind == OP_SUBPLAN) {
+        if (op.kind == OP_SUBPLAN || op.kind == OP_CHUNK) {
             if (const Op* f = op_at(*op.sub, index)) return f;
             continue;
         }
@@ -1760,6 +1839,7 @@ static const char* op_kind_name(OpKind k) {
         case OP_POINTWISE: return "bluestein-filter";
         case OP_CHIRP_POST: return "bluestein-chirp-post";
         case OP_SUBPLAN: return "subplan";
+        case OP_CHUNK: return "large-chunk";
         case OP_SCALE_COPY: return "direct-dft";
         case OP_REAL_WIDEN: return "real-widen";
         case OP_PACK: return "real-pack";
@@ -1786,7 +1866,7 @@ static std::string json_str(const std::string& v) {
 
 static void diag_stages(const Plan& p, const std::string& route, std::ostringstream& d, bool& first) {
     for (const Op& op : p.ops) {
-        if (op.kind == OP_SUBPLAN && op.sub) {
+        if ((op.kind == OP_SUBPLAN || op.kind == OP_CHUNK) && op.sub) {
             diag_stages(*op.sub, route, d, first);
             continue;
         }
@@ -1815,8 +1895,10 @@ std::string plan_diagnostics_json(const Plan& p, const std::vector<std::string>&
     const std::string out_fmt = p.kind == MIFFT_R2C ? "packed-complex-f32" : p.kind == MIFFT_C2R ? "real-f32" : cf;
     std::ostringstream d;
     d << "{\"route\":{\"transform\":\"" << kinds[p.kind] << "\",\"route\":" << json_str(route)
-      << ",\"large_route_mode\":\"normal\",\"execution_kind\":\"normal\",\"reason_codes\":[\"within-bindings\"],"
-         "\"attempted_routes\":[" << json_str(route) << "]},";
+      << (p.large_chunk ? ",\"large_route_mode\":\"large-chunk\",\"execution_kind\":\"large-chunk\","
+                          "\"reason_codes\":[\"exceeds-u32-index-space\"],\"attempted_routes\":[\"direct\",\"batch-chunk\"]},"
+                        : ",\"large_route_mode\":\"normal\",\"execution_kind\":\"normal\","
+                          "\"reason_codes\":[\"within-bindings\"],\"attempted_routes\":[\"direct\"]},");
     d << "\"stages\":[";
     bool first = true;
     diag_stages(p, route, d, first);

@@ -40,7 +40,8 @@ enum OpKind {
     OP_C2R_PRE,     // half-length C2R: packed spectrum -> h-point complex lines
     OP_SMALL_VOLUME, // every axis of small f32 volumes, one workgroup per volume
     OP_DIRECT,       // direct DFT of short prime lines (n = p, m = lines per workgroup, filt = W_p table)
-    OP_STOCKHAM      // one radix stage of a long axis over HBM (st)
+    OP_STOCKHAM,     // one radix stage of a long axis over HBM (st)
+    OP_CHUNK         // large-chunk route: a child plan over batches at byte offsets src_off / dst_off
 };
 
 struct Plan;
@@ -108,6 +109,7 @@ struct Plan {
     bool in_place_kernels = true;
     size_t lds_budget = kLdsBudget;
     bool batch_mode = false;
+    bool large_chunk = false;  // ops are OP_CHUNK over child plans (more than u32::MAX elements)
     // Plan-owned workspace used when the caller passes none (one in-flight
     // execution per plan, as the reference's plan-owned temporaries).
     std::mutex ws_mu;

@@ -78,6 +78,26 @@ struct mifft_server {
     uint64_t seq_ws_cap = 0;
     hipStream_t seq_stream = nullptr;
     std::vector<hipEvent_t> seq_events;
+    // online submission (mifft_server_submit / _poll)
+    struct Item {
+        mifft_server_request r;
+        uint64_t ticket = 0;
+        uint64_t mem = 0, desc = 0;
+        bool batched = false;
+        double submit_s = 0, admit_s = 0;
+    };
+    std::deque<Item> queue;                   // submitted, not yet admitted
+    std::vector<std::vector<Item>> slot_items;  // per slot, in flight
+    std::deque<mifft_server_completion> completions;
+    uint64_t next_ticket = 1, online_inflight = 0;
+    bool clock_started = false;
+    std::chrono::steady_clock::time_point t0;
+    // resident plans of the online path: at most max_shapes request shapes,
+    // least recently used evicted first among shapes with nothing in flight
+    size_t max_shapes = 256;
+    uint64_t use_clock = 0;
+    std::map<PlanKey, uint64_t> last_use;
+    std::map<PlanKey, int> refs;
     ~mifft_server() {
         for (Slot& s : slots) {
             if (s.ev) (void)hipEventDestroy(s.ev);
@@ -325,6 +345,141 @@ static void replay(mifft_server& S, const mifft_server_request* reqs, const doub
     }
 }
 
+// ------------------------------------------------------------- online path
+static double online_now(mifft_server& S) {
+    if (!S.clock_started) {
+        S.t0 = std::chrono::steady_clock::now();
+        S.clock_started = true;
+    }
+    return now_s(S.t0);
+}
+
+static size_t online_busy(const mifft_server& S) {
+    size_t n = S.queue.size();
+    for (const auto& v : S.slot_items) n += v.size();
+    return n;
+}
+
+// Frees cache room for a new shape: drops least recently used shapes that
+// nothing in flight or queued uses.
+static void evict_for(mifft_server& S, const PlanKey& k) {
+    if (S.plans.count(k) || S.plans.size() < S.max_shapes) return;
+    std::vector<std::pair<uint64_t, PlanKey>> cand;
+    for (auto& kv : S.plans) {
+        auto r = S.refs.find(kv.first);
+        if (r == S.refs.end() || r->second == 0) cand.emplace_back(S.last_use[kv.first], kv.first);
+    }
+    std::sort(cand.begin(), cand.end());
+    for (size_t i = 0; i < cand.size() && S.plans.size() >= S.max_shapes; i++) {
+        const PlanKey& v = cand[i].second;
+        mifft_server_request r{};
+        r.kind = v.kind;
+        r.precision = v.prec;
+        r.rank = v.rank;
+        r.direction = v.dir;
+        r.normalization = v.norm;
+        r.shape[0] = v.s0;
+        r.shape[1] = v.s1;
+        r.batch = v.batch;
+        S.bcache.erase(config_of(r), (mifft_kind)v.kind);
+        S.plans.erase(v);
+        S.last_use.erase(v);
+        S.refs.erase(v);
+    }
+}
+
+// Completes finished ticks, then admits queued requests into free slots as
+// the replay loop does (the same admission rules, one tick per free slot).
+static void online_progress(mifft_server& S) {
+    for (size_t i = 0; i < S.slots.size(); i++) {
+        Slot& sl = S.slots[i];
+        if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+        const double t = online_now(S);
+        for (auto& it : S.slot_items[i]) {
+            S.completions.push_back(
+                mifft_server_completion{it.ticket, it.submit_s, it.admit_s, t, MIFFT_SUCCESS, it.batched ? 0 : 1});
+            S.refs[key_of(it.r)]--;
+        }
+        S.online_inflight -= sl.mem;
+        S.slot_items[i].clear();
+        sl.busy = false;
+        sl.mem = 0;
+    }
+    const size_t desc_cap = S.slots.empty() ? 0 : S.slots[0].desc_cap;
+    for (size_t i = 0; i < S.slots.size() && !S.queue.empty(); i++) {
+        Slot& sl = S.slots[i];
+        if (sl.busy) continue;
+        std::vector<mifft_server_request> reqs;
+        std::vector<char> batched;
+        std::vector<mifft_server::Item> items;
+        uint64_t tick_mem = 0, batch_ws = 0, plan_ws = 0, tick_desc = 0;
+        while (!S.queue.empty() && (int)items.size() < S.max_items) {
+            const mifft_server::Item& it = S.queue.front();
+            const uint64_t ws = it.mem - it.r.in_bytes - it.r.out_bytes;
+            const uint64_t nb = it.batched ? batch_ws + ws : batch_ws;
+            const uint64_t np = it.batched ? plan_ws : std::max(plan_ws, ws);
+            const uint64_t nd = it.batched ? tick_desc + it.desc : tick_desc;
+            if (S.online_inflight + tick_mem + it.mem > S.budget || nb + np > sl.ws_cap || nd > desc_cap) break;
+            tick_desc = nd;
+            batch_ws = nb;
+            plan_ws = np;
+            tick_mem += it.mem;
+            items.push_back(it);
+            S.queue.pop_front();
+        }
+        if (items.empty()) break;  // the head waits for memory to free up
+        const double ta = online_now(S);
+        std::vector<int> take;
+        for (size_t k = 0; k < items.size(); k++) {
+            items[k].admit_s = ta;
+            reqs.push_back(items[k].r);
+            batched.push_back(items[k].batched ? 1 : 0);
+            take.push_back((int)k);
+        }
+        launch_tick(S, sl, reqs.data(), take, batched);
+        sl.busy = true;
+        sl.mem = tick_mem;
+        S.online_inflight += tick_mem;
+        S.slot_items[i] = std::move(items);
+    }
+}
+
+static uint64_t online_submit(mifft_server& S, const mifft_server_request& r) {
+    if (!r.in || !r.out) fail(MIFFT_ERR_NULL_ARGUMENT, "null request buffer");
+    if (((uintptr_t)r.in | (uintptr_t)r.out) & 15u) fail(MIFFT_ERR_UNALIGNED, "device buffers must be 16-byte aligned");
+    if (S.slots.empty()) prepare(S, nullptr, 0);
+    if (S.slot_items.size() != S.slots.size()) S.slot_items.resize(S.slots.size());
+    const PlanKey k = key_of(r);
+    evict_for(S, k);
+    // the plans of a new shape are built here, on first sight (plan tables
+    // upload synchronously on the plan's own stream)
+    const Plan& sp = standalone(S, r);
+    if (r.in_bytes < sp.in_bytes || r.out_bytes < sp.out_bytes)
+        fail(MIFFT_ERR_BUFFER_TOO_SMALL, "request buffers smaller than the transform needs");
+    if (r.in == r.out && !sp.in_place_kernels)
+        fail(MIFFT_ERR_IN_PLACE_UNSUPPORTED, "served real transforms run out of place");
+    const Plan* bp = batched_plan(S, r);
+    mifft_server::Item it;
+    it.r = r;
+    if (bp) it.desc = batch_desc_bytes(*bp, 1);
+    uint64_t elems = r.batch;
+    for (int d = 0; d < r.rank; d++) elems *= r.shape[d];
+    const size_t desc_cap = S.slots[0].desc_cap;
+    it.batched = bp != nullptr && it.desc <= desc_cap && elems < S.batch_max_elems;
+    const uint64_t ws = it.batched ? bp->ws_bytes : sp.ws_bytes;
+    it.mem = r.in_bytes + r.out_bytes + ws;
+    if (it.mem > S.budget || ws > S.slots[0].ws_cap)
+        fail(MIFFT_ERR_OVER_BUDGET, "request needs " + std::to_string(it.mem) + " bytes; the budget is " +
+                                        std::to_string(S.budget));
+    it.ticket = S.next_ticket++;
+    it.submit_s = online_now(S);
+    S.last_use[k] = ++S.use_clock;
+    S.refs[k]++;
+    S.queue.push_back(it);
+    online_progress(S);
+    return it.ticket;
+}
+
 template <class F>
 static mifft_status sguard(F&& f) {
     try {
@@ -373,6 +528,10 @@ mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* re
                                  int32_t n, int32_t mode, mifft_server_result* results) {
     if (!s || !reqs || !arrival_s || !results) return MIFFT_ERR_NULL_ARGUMENT;
     if (mode < 0 || mode > 1) return MIFFT_ERR_INVALID_ENUM;
+    if (online_busy(*s) > 0) {
+        set_last_detail("submitted requests are still queued or in flight: poll them first");
+        return MIFFT_ERR_BUSY;
+    }
     return sguard([&] {
         prepare(*s, reqs, n);
         replay(*s, reqs, arrival_s, n, mode, results);
@@ -380,6 +539,56 @@ mifft_status mifft_server_replay(mifft_server* s, const mifft_server_request* re
     });
 }
 
-void mifft_server_destroy(mifft_server* s) { delete s; }
+mifft_status mifft_server_submit(mifft_server* s, const mifft_server_request* req, uint64_t* ticket) {
+    if (!s || !req || !ticket) return MIFFT_ERR_NULL_ARGUMENT;
+    *ticket = 0;
+    return sguard([&] { *ticket = online_submit(*s, *req); });
+}
+
+mifft_status mifft_server_poll(mifft_server* s, int32_t wait, mifft_server_completion* out, int32_t cap,
+                               int32_t* count) {
+    if (!s || !count || (cap > 0 && !out)) return MIFFT_ERR_NULL_ARGUMENT;
+    *count = 0;
+    return sguard([&] {
+        online_progress(*s);
+        while (wait && s->completions.empty() && online_busy(*s) > 0) online_progress(*s);
+        int32_t n = 0;
+        while (n < cap && !s->completions.empty()) {
+            out[n++] = s->completions.front();
+            s->completions.pop_front();
+        }
+        *count = n;
+    });
+}
+
+mifft_status mifft_server_pending(const mifft_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* completed) {
+    if (!s) return MIFFT_ERR_NULL_ARGUMENT;
+    uint64_t f = 0;
+    for (const auto& v : s->slot_items) f += v.size();
+    if (queued) *queued = s->queue.size();
+    if (in_flight) *in_flight = f;
+    if (completed) *completed = s->completions.size();
+    return MIFFT_SUCCESS;
+}
+
+mifft_status mifft_server_set_plan_cache(mifft_server* s, uint64_t max_shapes) {
+    if (!s) return MIFFT_ERR_NULL_ARGUMENT;
+    if (max_shapes < 1) {
+        set_last_detail("the plan cache holds at least one shape");
+        return MIFFT_ERR_INVALID_ENUM;
+    }
+    s->max_shapes = (size_t)max_shapes;
+    return MIFFT_SUCCESS;
+}
+
+uint64_t mifft_server_cached_shapes(const mifft_server* s) { return s ? s->plans.size() : 0; }
+
+void mifft_server_destroy(mifft_server* s) {
+    if (s) {
+        for (Slot& sl : s->slots)
+            if (sl.busy) (void)hipEventSynchronize(sl.ev);
+    }
+    delete s;
+}
 
 }  // extern "C"

@@ -170,6 +170,45 @@ class FftServer:
             out[k] = (res[k].arrival_s, res[k].admit_s, res[k].done_s, res[k].status, res[k].path)
         return out
 
+    # online serving (mifft_server_submit / _poll)
+    def submit(self, spec, buffers) -> int:
+        """Submits one request (a spec as from request_mix and its (input,
+        output) buffers, kept alive by the caller until completion); returns
+        its ticket."""
+        arr = self._requests([spec], [buffers])
+        t = ctypes.c_uint64()
+        _check(self._lib.mifft_server_submit(self._h, arr, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, wait: bool = False, cap: int = 1024):
+        """Completions since the last poll: a list of dicts ticket, submit_s,
+        admit_s, done_s, status, path."""
+        out = (_lib.ServerCompletion * cap)()
+        n = ctypes.c_int32()
+        _check(self._lib.mifft_server_poll(self._h, int(bool(wait)), out, cap, ctypes.byref(n)))
+        return [dict(ticket=out[k].ticket, submit_s=out[k].submit_s, admit_s=out[k].admit_s,
+                     done_s=out[k].done_s, status=out[k].status, path=out[k].path) for k in range(n.value)]
+
+    def drain(self):
+        """Polls until nothing is queued or in flight; returns every completion."""
+        got = []
+        while True:
+            got += self.poll(wait=True)
+            q, f, c = self.pending()
+            if q == 0 and f == 0 and c == 0:
+                return got
+
+    def pending(self):
+        q, f, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self._lib.mifft_server_pending(self._h, ctypes.byref(q), ctypes.byref(f), ctypes.byref(c)))
+        return q.value, f.value, c.value
+
+    def set_plan_cache(self, max_shapes: int) -> None:
+        _check(self._lib.mifft_server_set_plan_cache(self._h, int(max_shapes)))
+
+    def cached_shapes(self) -> int:
+        return int(self._lib.mifft_server_cached_shapes(self._h))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
