@@ -30,15 +30,15 @@ except Exception:  # library not built: collection still succeeds, the GPU run f
 
 PREC = {"f32": FftPrecision.F32, "f64": FftPrecision.F64, "df64": FftPrecision.Df64}
 TOL = {"f32": 1e-6, "f64": 1e-13, "df64": 1e-11}
-NAME_RE = re.compile(r"reg:(\w+):n(\d+):t(\d+):l(\d+):lf(\d)(\d):m(\d)/(\d):p(\d):v(\d):([0-9x]+)(:pipe\d)?(:fold)?$")
+NAME_RE = re.compile(r"reg:(\w+):n(\d+):t(\d+):l(\d+):lf(\d)(\d):m(\d)/(\d):p(\d):v(\d):([0-9x]+)(:pipe\d)?(:fold)?(:adj)?$")
 
 
 def parse_name(name):
     m = NAME_RE.match(name)
     assert m, name
-    prec, n, tpl, lines, a, b, lm, sm, post, vec, rad, pipe, fold = m.groups()
+    prec, n, tpl, lines, a, b, lm, sm, post, vec, rad, pipe, fold, adj = m.groups()
     return dict(prec=prec, n=int(n), tpl=int(tpl), lines=int(lines), lf=a + b, m=f"{lm}/{sm}", post=int(post),
-                vec=int(vec), rad=rad, pipe=bool(pipe), fold=bool(fold))
+                vec=int(vec), rad=rad, pipe=bool(pipe), fold=bool(fold), adj=bool(adj))
 
 
 def key_matches(v, key):
@@ -84,7 +84,15 @@ def _pool():
         for split in ("2", "3"):
             b = 2 if lg <= 20 else 1
             P.append(("c2c", [1 << lg], None, b, "f32", f"split|f32|n{1 << lg} = {split}\n"))
+    # split passes of a strided long axis (two adjacent columns: pairable)
+    for lg in range(12, 25):
+        for split in ("2", "3"):
+            P.append(("c2c", [2, 1 << lg], [1], 1, "f32", f"split|f32|n{1 << lg} = {split}\n"))
+    for n in (120, 343, 1001):
+        P.append(("c2c", [n], None, 37, "f32", ""))
     for prec in ("f64", "df64"):
+        for lg in (20, 22, 24):
+            P.append(("c2c", [1 << lg], None, 1, prec, f"split|{prec}|n{1 << lg} = 2\n"))
         P.append(("c2c", [256], None, 37, prec, ""))
         P.append(("c2c", [37, 256], [1], 1, prec, ""))
         P.append(("c2c", [40, 256], [1], 1, prec, ""))

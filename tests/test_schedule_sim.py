@@ -327,3 +327,89 @@ def test_small_volume_gather(dims):
     # axis 0 is the fastest-varying index: numpy's C order reversed
     ref = np.fft.fftn(x0.reshape(dims[::-1])).reshape(-1)
     assert np.abs(vol - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def jof(s, t, q, ns, nb, tpl, adj):
+    """Butterfly q of thread t in stage s (RegFft::jof): adjacent pairs in the
+    HBM stages (0 and the last) when adj, else strided by the thread count."""
+    return t * nb + q if adj and (s == 0 or s == ns - 1) else t + q * tpl
+
+
+@pytest.mark.parametrize("rad,tpl", [([8, 8, 8, 8], 256), ([8, 8, 8, 8], 128), ([16, 16, 16], 128),
+                                     ([8, 8, 4, 4], 64), ([16, 16, 4], 32), ([8, 8, 8, 4], 128)])
+@pytest.mark.parametrize("adj", [False, True])
+def test_register_kernel_thread_map(rad, tpl, adj):
+    """RegFft's per-thread schedule: stage-0 loads into register slots
+    (q, m) of butterfly jof(0, t, q), LDS exchange writes with the computing
+    stage's map and reads with the next stage's, last-stage stores from slot
+    (q, m) to output j + m L.  With adj, slots (q, m) and (q + 1, m) load
+    and store adjacent elements (one 16-byte access)."""
+    N = int(np.prod(rad))
+    ns = len(rad)
+    x = crand(N)
+    tab = stage_table(rad)
+    regs = {}
+    nr = N // rad[0]
+    nb = -(-nr // tpl)
+    for t in range(tpl):
+        for q in range(nb):
+            j = jof(0, t, q, ns, nb, tpl, adj)
+            for m in range(rad[0]):
+                regs[(t, q, m)] = x[j + m * nr]
+            if adj and q % 2 == 0:
+                # the pair load covers elements j + m nr and j + 1 + m nr
+                assert jof(0, t, q + 1, ns, nb, tpl, adj) == j + 1
+    out = np.zeros(N, dtype=complex)
+    L = 1
+    for s, R in enumerate(rad):
+        nr = N // R
+        nb = -(-nr // tpl)
+        for t in range(tpl):
+            for q in range(nb):
+                j = jof(s, t, q, ns, nb, tpl, adj)
+                if j >= nr:
+                    continue
+                k = j % L
+                w = np.array([regs[(t, q, m)] for m in range(R)])
+                if L > 1:
+                    for m in range(1, R):
+                        w[m] *= tab[twoff(rad, s) + (m - 1) * L + k]
+                w = np.fft.fft(w)
+                for m in range(R):
+                    regs[(t, q, m)] = w[m]
+        if s == ns - 1:
+            for t in range(tpl):
+                for q in range(nb):
+                    j = jof(s, t, q, ns, nb, tpl, adj)
+                    if j >= nr:
+                        continue
+                    k = j % L
+                    for m in range(R):
+                        out[(j - k) * R + k + m * L] = regs[(t, q, m)]
+                    if adj and q % 2 == 0:
+                        j1 = jof(s, t, q + 1, ns, nb, tpl, adj)
+                        assert (j1 - j1 % L) * R + j1 % L == (j - k) * R + k + 1
+            break
+        # exchange: write with stage s's map, read with stage s + 1's
+        lds = np.zeros(N, dtype=complex)
+        for t in range(tpl):
+            for q in range(nb):
+                j = jof(s, t, q, ns, nb, tpl, adj)
+                if j >= nr:
+                    continue
+                k = j % L
+                for m in range(R):
+                    lds[(j - k) * R + k + m * L] = regs[(t, q, m)]
+        L *= R
+        R2 = rad[s + 1]
+        nr2 = N // R2
+        nb2 = -(-nr2 // tpl)
+        regs = {}
+        for t in range(tpl):
+            for q in range(nb2):
+                j = jof(s + 1, t, q, ns, nb2, tpl, adj)
+                if j >= nr2:
+                    continue
+                for m in range(R2):
+                    regs[(t, q, m)] = lds[j + m * nr2]
+    np.testing.assert_allclose(out, np.fft.fft(x), rtol=1e-9, atol=1e-9 * N)

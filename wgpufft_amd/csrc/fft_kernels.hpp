@@ -262,8 +262,12 @@ __device__ __forceinline__ void store_pair_at(const LineArgs& a, uint32_t i, uin
 //   rows r + c Q (c < F) of the next axis (M = F Q points, fold_in /
 //   fold_out apart), and the store applies that axis's first radix-F stage,
 //   so that its pass runs Q-point lines (1/F of the tile of an M-point line).
+//   ADJ (contiguous complex lines): in stage 0 and the last stage a thread
+//   takes adjacent butterflies j = 2t + q instead of j = t + q TPL, so its
+//   points j and j + 1 are adjacent in memory and each global load and store
+//   moves 16 bytes per lane (two points) instead of 8.
 template <class Fmt, int N, int TPL, int LINES, bool LD_LF, bool ST_LF, int LDM, int STM, bool POST,
-          class RL, int VEC = 1, bool PF = false, bool FOLD = false>
+          class RL, int VEC = 1, bool PF = false, bool FOLD = false, bool ADJ = false>
 struct RegFft {
     using T = typename Fmt::T;
     static constexpr int NS = RL::n;
@@ -309,6 +313,25 @@ struct RegFft {
                               : max_regs(s + 1));
     }
     static constexpr int EMAX = max_regs(0);
+    static_assert(!ADJ || (!LD_LF && !ST_LF && LDM == LD_COMPLEX && STM == ST_COMPLEX && !POST &&
+                           (VEC == 1 || FOLD) && sizeof(cpx<T>) == 8),
+                  "adjacent butterflies: contiguous f32 complex lines");
+    // Stages whose butterflies are taken in adjacent pairs (ADJ): the HBM ones.
+    template <int s>
+    static constexpr bool adj_stage() {
+        return ADJ && (s == 0 || s == NS - 1);
+    }
+    // Butterfly q of thread t in stage s.
+    template <int s>
+    static __device__ __forceinline__ uint32_t jof(uint32_t t, int q) {
+        constexpr int NB = ceil_div(N / RL::at(s), TPL);
+        if constexpr (adj_stage<s>()) {
+            static_assert((N / RL::at(s)) % TPL == 0 && NB % 2 == 0, "adjacent butterflies need pairs per thread");
+            return t * NB + q;
+        } else {
+            return t + q * TPL;
+        }
+    }
 
     struct Map {
         uint32_t lg, t;  // line group within the workgroup, thread within the group
@@ -337,7 +360,7 @@ struct RegFft {
                 cpx<T>* ln = lds + mw.lg * LSTR;
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
-                    const uint32_t j = mw.t + q * TPL;
+                    const uint32_t j = jof<s>(mw.t, q);
                     if (NR % TPL == 0 || j < (uint32_t)NR) {
                         const uint32_t k = j % L;
 #pragma unroll
@@ -348,7 +371,7 @@ struct RegFft {
                 const cpx<T>* lr = lds + mr.lg * LSTR;
 #pragma unroll
                 for (int q = 0; q < NB2; q++) {
-                    const uint32_t j = mr.t + q * TPL;
+                    const uint32_t j = jof<s + 1>(mr.t, q);
                     if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
 #pragma unroll
                         for (int m = 0; m < R2; m++) w[v][q * R2 + m] = lr[lds_pad(j + m * NR2)];
@@ -364,7 +387,7 @@ struct RegFft {
                     T* ln = lt + mw.lg * LSTR;
 #pragma unroll
                     for (int q = 0; q < NB; q++) {
-                        const uint32_t j = mw.t + q * TPL;
+                        const uint32_t j = jof<s>(mw.t, q);
                         if (NR % TPL == 0 || j < (uint32_t)NR) {
                             const uint32_t k = j % L;
 #pragma unroll
@@ -376,7 +399,7 @@ struct RegFft {
                     const T* lr = lt + mr.lg * LSTR;
 #pragma unroll
                     for (int q = 0; q < NB2; q++) {
-                        const uint32_t j = mr.t + q * TPL;
+                        const uint32_t j = jof<s + 1>(mr.t, q);
                         if (NR2 % TPL == 0 || j < (uint32_t)NR2) {
 #pragma unroll
                             for (int m = 0; m < R2; m++) {
@@ -399,7 +422,7 @@ struct RegFft {
             const Map mp = map_of<s>();
 #pragma unroll
             for (int q = 0; q < NB; q++) {
-                const uint32_t j = mp.t + q * TPL;
+                const uint32_t j = jof<s>(mp.t, q);
                 const uint32_t k = (NR % TPL == 0 || j < (uint32_t)NR) ? j % L : 0;
 #pragma unroll
                 for (int m = 1; m < R; m++)
@@ -429,6 +452,22 @@ struct RegFft {
         if constexpr (UNFOLD) {
             const cpx<T> t = post_twiddle<T>(a, b.j);
             tin = mk<T>(t.x, -t.y);  // W_M^{-n}
+        }
+        if constexpr (ADJ) {
+            // adjacent butterflies j, j + 1: points j + m NR and j + 1 + m NR
+            // in one 16-byte load (contiguous line, even element index)
+#pragma unroll
+            for (int q = 0; q < NB; q += 2) {
+                const uint32_t o0 = b.in + jof<0>(mp.t, q);
+#pragma unroll
+                for (int m = 0; m < R; m++) {
+#pragma unroll
+                    for (int v = 0; v < VEC; v++)
+                        load_pair_at<Fmt, NTL>(a, o0 + (uint32_t)(m * NR) + (FOLD ? v * a.fold_in : 0u),
+                                               w[v][q * R + m], w[v][(q + 1) * R + m]);
+                }
+            }
+            return;
         }
 #pragma unroll
         for (int q = 0; q < NB; q++) {
@@ -487,7 +526,7 @@ struct RegFft {
             // twiddle + butterfly (one twiddle load serves the VEC lines)
 #pragma unroll
             for (int q = 0; q < NB; q++) {
-                const uint32_t j = mp.t + q * TPL;
+                const uint32_t j = jof<s>(mp.t, q);
                 if (NR % TPL == 0 || j < (uint32_t)NR) {
                     if constexpr (L > 1) {
                         const uint32_t k = j % L;
@@ -515,6 +554,39 @@ struct RegFft {
                 if constexpr (FOLD && !UNFOLD) {
 #pragma unroll
                     for (int c = 0; c < VEC; c++) tf[c] = post_twiddle<T>(a, (uint64_t)b.j * c);
+                }
+                if constexpr (adj_stage<s>()) {
+                    // adjacent butterflies j, j + 1 of the last stage write
+                    // adjacent outputs j + m L, j + 1 + m L: 16-byte stores
+                    if (valid) {
+#pragma unroll
+                        for (int q = 0; q < NB; q += 2) {
+                            const uint32_t j = jof<s>(mp.t, q);
+                            const uint32_t o0 = b.out + ((j - j % L) * R + j % L);
+#pragma unroll
+                            for (int m = 0; m < R; m++) {
+                                const uint32_t idx = o0 + (uint32_t)(m * L);
+                                if constexpr (FOLD) {
+                                    cpx<T> z0[VEC], z1[VEC];
+#pragma unroll
+                                    for (int c = 0; c < VEC; c++) {
+                                        z0[c] = w[c][q * R + m];
+                                        z1[c] = w[c][(q + 1) * R + m];
+                                    }
+                                    Dft<VEC, T>::run(z0);
+                                    Dft<VEC, T>::run(z1);
+                                    store_pair_at<Fmt, false>(a, idx, b.j, 0, z0[0], z1[0]);
+#pragma unroll
+                                    for (int c = 1; c < VEC; c++)
+                                        store_pair_at<Fmt, false>(a, idx + c * a.fold_out, b.j, 0, cmul(z0[c], tf[c]),
+                                                                  cmul(z1[c], tf[c]));
+                                } else {
+                                    store_pair_at<Fmt, false>(a, idx, b.j, 0, w[0][q * R + m], w[0][(q + 1) * R + m]);
+                                }
+                            }
+                        }
+                    }
+                    return;
                 }
 #pragma unroll
                 for (int q = 0; q < NB; q++) {

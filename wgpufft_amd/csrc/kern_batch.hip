@@ -191,7 +191,7 @@ int batch_lds_class(int prec, int ldm, int stm, int post) {
 }
 
 int batch_class_for(const RegVariant* v) {
-    if (!v || v->persist || v->nthreads != 256 || v->ldm != LD_COMPLEX || v->stm != ST_COMPLEX) return -1;
+    if (!v || v->persist || v->adj || v->nthreads != 256 || v->ldm != LD_COMPLEX || v->stm != ST_COMPLEX) return -1;
     if (v->prec == P_F64 && v->n == 256 && v->vec == 1 && !v->post)
         return v->ld_lf == 0 && v->st_lf == 0 ? BC_REG_C256_F64 : (v->ld_lf && v->st_lf ? BC_REG_S256_F64 : -1);
     if (v->prec != P_F32) return -1;

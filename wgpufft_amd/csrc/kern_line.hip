@@ -89,6 +89,21 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
                                        MINB>,                                                      \
      RADSTR, MINB, F}
 
+// Adjacent-butterfly forms (16-byte global accesses) of contiguous complex
+// lines: REGVA one line per thread group, REGFA the fold.
+#define MIFFT_KA(N, TPL, LINES, ...) \
+    RegFft<FmtF32, N, TPL, LINES, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), 1, false, false, true>
+#define REGVA(N, TPL, LINES, RADSTR, ...)                                                            \
+    {P_F32, N, TPL, LINES, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, MIFFT_KA(N, TPL, LINES, __VA_ARGS__)::NT, \
+     MIFFT_KA(N, TPL, LINES, __VA_ARGS__)::lds_bytes(),                                            \
+     (const void*)&reg_fft_kernel<MIFFT_KA(N, TPL, LINES, __VA_ARGS__)>, RADSTR, 0, 0, 1}
+#define MIFFT_KFA(N, TPL, F, ...) \
+    RegFft<FmtF32, N, TPL, F, false, false, LD_COMPLEX, ST_COMPLEX, false, MIFFT_RL(__VA_ARGS__), F, false, true, true>
+#define REGFA(N, TPL, F, RADSTR, ...)                                                                \
+    {P_F32, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F, MIFFT_KFA(N, TPL, F, __VA_ARGS__)::NT,       \
+     MIFFT_KFA(N, TPL, F, __VA_ARGS__)::lds_bytes(),                                                \
+     (const void*)&reg_fft_kernel<MIFFT_KFA(N, TPL, F, __VA_ARGS__)>, RADSTR, 0, F, 1}
+
 // f32 power-of-two lines: 16 points per thread and line, radix-16 stages.
 #define F32_POW2(LDLF, STLF, LDM, STM, POST, VEC, L4096, L2048, L1024, L512, L256, L128, L64)          \
     REGV(FmtF32, P_F32, 4096, 256, L4096, LDLF, STLF, LDM, STM, POST, VEC, "16x16x16", 16, 16, 16),    \
@@ -112,7 +127,14 @@ static const RegVariant kRegistry[] = {
     // C2R axis 0: Hermitian-extended packed in, real out
     F32_POW2(0, 0, LD_HERMITIAN, ST_REAL, 0, 1, 1, 2, 4, 8, 16, 32, 64),
     // four-step split passes: columns with twiddle on store; rows with transposed store
-    F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, 8, 16, 32, 32, 32, 64, 128),
+    // (line pairs: the columns of a strided long axis, whose balanced splits
+    // of 2^14..2^24 points give 128- to 4096-point columns)
+    REGV(FmtF32, P_F32, 4096, 256, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x16x16", 16, 16, 16),
+    REGV(FmtF32, P_F32, 2048, 128, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x16x8", 16, 16, 8),
+    REGV(FmtF32, P_F32, 1024, 64, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 512, 32, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 256, 16, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 64, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x8", 16, 8),
     F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, 4, 8, 16, 16, 16, 32, 64),
     F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, 4, 8, 16, 16, 16, 32, 64),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
@@ -219,6 +241,17 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 4096, 256, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x16", 16, 16, 16),
     REGV(FmtF64, P_F64, 4096, 128, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
+    // adjacent-butterfly forms: 16-byte loads and stores of contiguous lines
+    REGFA(4096, 256, 2, "8x8x8x8", 8, 8, 8, 8),
+    REGFA(4096, 128, 2, "8x8x8x8", 8, 8, 8, 8),
+    REGFA(4096, 128, 2, "16x16x16", 16, 16, 16),
+    REGFA(2048, 128, 2, "8x8x8x4", 8, 8, 8, 4),
+    REGFA(1024, 64, 2, "8x8x4x4", 8, 8, 4, 4),
+    REGVA(4096, 256, 1, "8x8x8x8", 8, 8, 8, 8),
+    REGVA(4096, 128, 1, "16x16x16", 16, 16, 16),
+    REGVA(1024, 64, 2, "8x8x4x4", 8, 8, 4, 4),
+    REGVA(1024, 64, 4, "8x8x4x4", 8, 8, 4, 4),
+    REGVA(1024, 32, 4, "16x16x4", 16, 16, 4),
 };
 
 int reg_variant_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }
@@ -244,6 +277,7 @@ std::string reg_variant_name(const RegVariant* v) {
     std::string name = b;
     if (v->persist) name += ":pipe" + std::to_string(v->persist);
     if (v->fold) name += ":fold";
+    if (v->adj) name += ":adj";
     return name;
 }
 
@@ -257,7 +291,7 @@ const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ld
                                    int post, int vec, int fold) {
     for (const RegVariant& v : kRegistry) {
         if (v.prec == prec && v.n == n && v.ld_lf == ld_lf && v.st_lf == st_lf && v.ldm == ldm &&
-            v.stm == stm && v.post == post && v.vec == vec && v.fold == fold)
+            v.stm == stm && v.post == post && v.vec == vec && v.fold == fold && !v.adj)
             return &v;
     }
     return nullptr;

@@ -35,6 +35,9 @@ struct RegVariant {
     // F > 0: fold variant (F contiguous lines M/F rows apart per thread, the
     // next axis's first radix-F stage on store); looked up only by fold passes.
     int fold = 0;
+    // 1: stage 0 and the last stage take adjacent butterflies, so global
+    // loads and stores move 16 bytes (two points) per lane.
+    int adj = 0;
 };
 
 // Registry lookup; nullptr when no compiled variant matches.

@@ -586,6 +586,15 @@ struct Builder {
                               ls.S1 % 2 == 0 && in.si == 1 && out.si == 1 && in.sj % 2 == 0 &&
                               in.so % 2 == 0 && in.sn % 2 == 0 && out.sj % 2 == 0 && out.so % 2 == 0 &&
                               out.sn % 2 == 0;
+        // Adjacent-butterfly variants read and write points j, j + 1 of a line
+        // in one 16-byte access: unit point strides, even line length, every
+        // line at an even element offset.
+        auto even_lines = [&](const Layout& l, uint64_t off_bytes) {
+            return l.sn == 1 && off_bytes % 16 == 0 && (ls.S1 <= 1 || l.si % 2 == 0) &&
+                   (ls.S2 <= 1 || l.sj % 2 == 0) && l.so % 2 == 0;
+        };
+        op.adj_ok = P.prec == P_F32 && n % 2 == 0 && !ld_lf && !st_lf && ldm == LD_COMPLEX && stm == ST_COMPLEX &&
+                    !post && even_lines(in, src_off) && even_lines(out, dst_off);
         op.n = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
         op.ld_lf = ld_lf;
         op.st_lf = st_lf;
@@ -600,7 +609,8 @@ struct Builder {
         if (choice.rfind("reg:", 0) == 0) {
             const RegVariant* t = find_reg_variant_named(choice);
             if (t && t->prec == P.prec && t->n == (int)n && t->ld_lf == (int)ld_lf && t->st_lf == (int)st_lf &&
-                t->ldm == ldm && t->stm == stm && t->post == post && (t->vec == 1 || pairable)) {
+                t->ldm == ldm && t->stm == stm && t->post == post && (t->vec == 1 || pairable) &&
+                (!t->adj || op.adj_ok)) {
                 rv = t;
                 tuned = true;
             }
@@ -1072,8 +1082,14 @@ struct Builder {
         op.ldm = ldm;
         op.stm = stm;
         op.tkey = tune_key(P.prec, n0, false, false, ldm, stm, 0, false, ls.lines) + "|fold" + std::to_string(F);
+        // adjacent-butterfly variants: unit point strides, even row pitches
+        // and fold offsets (16-byte accesses)
+        op.adj_ok = ldm == LD_COMPLEX && stm == ST_COMPLEX && n0 % 2 == 0 && lay_in.sn == 1 && lay_out.sn == 1 &&
+                    lay_in.sj % 2 == 0 && lay_in.so % 2 == 0 && lay_out.sj % 2 == 0 && lay_out.so % 2 == 0 &&
+                    fold_in % 2 == 0 && fold_out % 2 == 0;
         const RegVariant* rv = find_reg_variant_named(tuning_lookup(op.tkey));
-        if (!rv || rv->fold != (int)F || rv->n != (int)n0 || rv->prec != P.prec || rv->ldm != ldm || rv->stm != stm)
+        if (!rv || rv->fold != (int)F || rv->n != (int)n0 || rv->prec != P.prec || rv->ldm != ldm ||
+            rv->stm != stm || (rv->adj && !op.adj_ok))
             rv = fv[0];
         op.rv = rv;
         op.a = base_args(ls, lay_in, lay_out, conj, conj, scale);
@@ -1995,9 +2011,16 @@ int autotune_sequence(const mifft_config* cfgs, const mifft_kind* kinds, int cou
                     if (!ref && o.tkey == key) ref = &o;
             // candidate (name, op template: kind, rv or schedule, tables)
             std::vector<std::pair<std::string, Op>> cands;
+            // a variant applies to every op of the key: adjacent-butterfly
+            // forms only where each of them allows 16-byte accesses
+            bool adj_ok = true;
+            for (auto& p : plans)
+                for (const Op& o : p->ops)
+                    if (o.tkey == key) adj_ok = adj_ok && o.adj_ok;
             for (const RegVariant* v : find_reg_variants(ref->prec, (int)ref->n, ref->ld_lf, ref->st_lf, ref->ldm,
                                                          ref->stm, ref->post, ref->fold ? ref->rv->fold : 0)) {
-                if ((v->vec == 2 && !ref->pairable && !v->fold) || v->lds_bytes > kLdsBudget) continue;
+                if ((v->vec == 2 && !ref->pairable && !v->fold) || v->lds_bytes > kLdsBudget || (v->adj && !adj_ok))
+                    continue;
                 Op c = *ref;
                 c.kind = OP_REG;
                 c.rv = v;

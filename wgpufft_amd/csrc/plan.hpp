@@ -70,6 +70,9 @@ struct Op {
     // tuning-table key of the op's shape
     bool ld_lf = false, st_lf = false, pairable = false;
     bool fold = false;  // the fold pass of a 2D transform (fold register variants only)
+    // unit point strides and every line at a 16-byte-aligned offset on both
+    // sides: adjacent-butterfly variants (16-byte accesses) may run the op
+    bool adj_ok = false;
     std::string tkey;
 };
 
