@@ -284,7 +284,20 @@ struct RegFft {
     static constexpr int kLines = LINES;
     static constexpr int GROUPS = LINES / VEC;
     static constexpr int NT = TPL * GROUPS;
-    static constexpr int LSTR = lds_pad(N) + 1;
+    // LDS elements between the line groups' exchange slots.  Where a stage
+    // maps adjacent lanes to adjacent line groups (LD_LF / ST_LF), a wave's
+    // lanes span GROUPS slots at once, and the slot stride mod 32 points
+    // decides how their lanes share banks: 8 for two groups, 12 for four, 6
+    // for eight spread the slots' lanes evenly over 32 or 64 four-byte banks
+    // (counted lane by lane for the stage writes and reads of each variant).
+    static constexpr int lds_stride() {
+        constexpr int b = lds_pad(N);
+        constexpr int g = LINES / VEC;
+        constexpr int r = g == 2 ? 8 : g == 4 ? 12 : g == 8 ? 6 : -1;
+        if ((LD_LF || ST_LF) && r >= 0 && sizeof(cpx<T>) == 8) return b + ((r - b % 32) + 32) % 32;
+        return b + 1;
+    }
+    static constexpr int LSTR = lds_stride();
     static_assert(VEC == 1 || FOLD || (LDM == LD_COMPLEX && STM == ST_COMPLEX), "pairs are complex only");
     // Unfold (FOLD with the Hermitian load, C2R): the two rows are n and
     // n + H of the halves E_0, E_1 left by the previous axis's half-length
@@ -437,7 +450,8 @@ struct RegFft {
     // group and a butterfly past N/R reads point N/R-1; those values are never
     // stored.  A conditional load per point would make the compiler wait for
     // each load before issuing the next.
-    static __device__ __forceinline__ void load_lines(const LineArgs& a, uint32_t g0, cpx<T> (&w)[VEC][EMAX]) {
+    static __device__ __forceinline__ void load_lines(const LineArgs& a, uint32_t g0, cpx<T> (&w)[VEC][EMAX],
+                                                      cpx<T> (&tf)[VEC]) {
         constexpr int R = RL::at(0);
         constexpr int NR = N / R;
         constexpr int NB = ceil_div(NR, TPL);
@@ -452,6 +466,13 @@ struct RegFft {
         if constexpr (UNFOLD) {
             const cpx<T> t = post_twiddle<T>(a, b.j);
             tin = mk<T>(t.x, -t.y);  // W_M^{-n}
+        }
+        // the fold's store twiddles W_M^{r c}, loaded with the tile (a
+        // twiddle load at the store would wait for every load issued before
+        // it, the next tile's included, in the pipelined form)
+        if constexpr (FOLD && !UNFOLD) {
+#pragma unroll
+            for (int c = 0; c < VEC; c++) tf[c] = post_twiddle<T>(a, (uint64_t)b.j * c);
         }
         if constexpr (ADJ) {
             // adjacent butterflies j, j + 1: points j + m NR and j + 1 + m NR
@@ -509,7 +530,8 @@ struct RegFft {
     // stores.  With LOAD false, stage 0 takes w as already loaded.
     template <int s, bool LOAD = true>
     static __device__ __forceinline__ void stage(const LineArgs& a, cpx<T>* lds, uint32_t g0,
-                                                 cpx<T> (&w)[VEC][EMAX], cpx<T> (&twr)[TWREG]) {
+                                                 cpx<T> (&w)[VEC][EMAX], cpx<T> (&twr)[TWREG],
+                                                 cpx<T> (&tf)[VEC]) {
         if constexpr (s < NS) {
             constexpr int R = RL::at(s);
             constexpr int L = RL::lprod(s);
@@ -520,7 +542,7 @@ struct RegFft {
             const uint32_t g = g0 + mp.lg * VEC;
             const bool valid = g < a.lines;
             if constexpr (FIRST && LOAD) {
-                load_lines(a, g0, w);
+                load_lines(a, g0, w, tf);
                 if constexpr (PF) prefetch<1>(a, twr);
             }
             // twiddle + butterfly (one twiddle load serves the VEC lines)
@@ -550,11 +572,6 @@ struct RegFft {
                 // fold: rows y[r + b Q] (b < F) of the next axis (M = F Q)
                 // leave as z_c = W_M^{r c} sum_b y[r + b Q] W_F^{b c}, its first
                 // radix-F decimation-in-frequency stage
-                cpx<T> tf[VEC];
-                if constexpr (FOLD && !UNFOLD) {
-#pragma unroll
-                    for (int c = 0; c < VEC; c++) tf[c] = post_twiddle<T>(a, (uint64_t)b.j * c);
-                }
                 if constexpr (adj_stage<s>()) {
                     // adjacent butterflies j, j + 1 of the last stage write
                     // adjacent outputs j + m L, j + 1 + m L: 16-byte stores
@@ -635,7 +652,7 @@ struct RegFft {
                 }
             } else {
                 exchange<s>(lds, w);
-                stage<s + 1>(a, lds, g0, w, twr);
+                stage<s + 1>(a, lds, g0, w, twr, tf);
             }
         }
     }
@@ -644,7 +661,8 @@ struct RegFft {
     static __device__ __forceinline__ void tile(const LineArgs& a, cpx<T>* lds, uint32_t g0) {
         cpx<T> w[VEC][EMAX];
         cpx<T> twr[TWREG];
-        stage<0>(a, lds, g0, w, twr);
+        cpx<T> tf[VEC];
+        stage<0>(a, lds, g0, w, twr, tf);
     }
 
     static __device__ __forceinline__ void run(const LineArgs& a, cpx<T>* lds) {
@@ -673,20 +691,21 @@ struct RegFft {
         if constexpr (PF) prefetch<1>(a, twr);
         // The loads inside the loop are unconditional: a load under a branch
         // makes the compiler wait for it at the join.
-        load_lines(a, t0 * LINES, wa);
+        cpx<T> fa[VEC], fb[VEC];
+        load_lines(a, t0 * LINES, wa, fa);
         uint32_t i = 0;
         for (; i + 2 < nt; i += 2) {
-            load_lines(a, (t0 + (i + 1) * G) * LINES, wb);
-            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
-            load_lines(a, (t0 + (i + 2) * G) * LINES, wa);
-            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr);
+            load_lines(a, (t0 + (i + 1) * G) * LINES, wb, fb);
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr, fa);
+            load_lines(a, (t0 + (i + 2) * G) * LINES, wa, fa);
+            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr, fb);
         }
         if (i + 1 < nt) {
-            load_lines(a, (t0 + (i + 1) * G) * LINES, wb);
-            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
-            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr);
+            load_lines(a, (t0 + (i + 1) * G) * LINES, wb, fb);
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr, fa);
+            stage<0, false>(a, lds, (t0 + (i + 1) * G) * LINES, wb, twr, fb);
         } else {
-            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr);
+            stage<0, false>(a, lds, (t0 + i * G) * LINES, wa, twr, fa);
         }
     }
 };
@@ -697,8 +716,9 @@ __global__ void __launch_bounds__(K::NT) reg_fft_kernel(LineArgs a) {
     K::run(a, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
 }
 
-// Persistent pipelined launch of K (grid below the tile count; MINB blocks
-// per CU).
+// Persistent pipelined launch of K (grid below the tile count: the
+// workgroups resident at once); MINB is the launch bound's minimum of waves
+// per SIMD, which sets the register budget.
 template <class K, int MINB>
 __global__ void __launch_bounds__(K::NT, MINB) reg_fft_pipe_kernel(LineArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];

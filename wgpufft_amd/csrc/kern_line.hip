@@ -47,7 +47,8 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
      RADSTR}
 
 // Persistent pipelined form (twiddles held in registers, see
-// RegFft::run_pipelined), MINB workgroups per CU.
+// RegFft::run_pipelined), at least MINB waves per SIMD (the launch bound's
+// register budget).
 #define MIFFT_KP(FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, ...) \
     RegFft<FMT, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, MIFFT_RL(__VA_ARGS__), VEC, true>
 #define REGP(FMT, PREC, N, TPL, LINES, LDLF, STLF, LDM, STM, POST, VEC, MINB, RADSTR, ...)         \
@@ -71,7 +72,7 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
 
 // Fold form (F contiguous lines M/F rows apart per thread, the next axis's
 // first radix-F stage on store): REGF one tile of F lines per workgroup, REGFP
-// the persistent pipelined form with MINB workgroups per CU.
+// the persistent pipelined form (at least MINB waves per SIMD).
 #define MIFFT_KF(FMT, N, TPL, F, LDM, STM, PF, ...) \
     RegFft<FMT, N, TPL, F, false, false, LDM, STM, false, MIFFT_RL(__VA_ARGS__), F, PF, true>
 #define REGFM(FMT, PREC, N, TPL, F, LDM, STM, RADSTR, ...)                                            \
@@ -210,6 +211,9 @@ static const RegVariant kRegistry[] = {
     REGP(FmtF32, P_F32, 4096, 512, 1, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 1, "8x8x8x8", 8, 8, 8, 8),
     REGP(FmtF32, P_F32, 2048, 256, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 2, "8x8x8x4", 8, 8, 8, 4),
     REGP(FmtF32, P_F32, 1024, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, 2, "16x16x4", 16, 16, 4),
+    // persistent pipelined forms of the strided column-pair passes of 2D
+    // folds (1024-point columns of a 4096 x 2048 fold)
+    REGP(FmtF32, P_F32, 1024, 128, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, 2, "8x8x4x4", 8, 8, 4, 4),
     // persistent pipelined forms of the 2048^2 real passes (axis 0)
     REGP(FmtF32, P_F32, 2048, 128, 1, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
     REGP(FmtF32, P_F32, 2048, 128, 2, 0, 0, LD_REAL, ST_PACKED, 0, 1, 2, "16x16x8", 16, 16, 8),
@@ -243,6 +247,8 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 4096, 128, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     // adjacent-butterfly forms: 16-byte loads and stores of contiguous lines
     REGFA(4096, 256, 2, "8x8x8x8", 8, 8, 8, 8),
+    REGFA(4096, 256, 2, "4x16x16x4", 4, 16, 16, 4),
+    REGFA(4096, 256, 2, "8x16x8x4", 8, 16, 8, 4),
     REGFA(4096, 128, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFA(4096, 128, 2, "16x16x16", 16, 16, 16),
     REGFA(2048, 128, 2, "8x8x8x4", 8, 8, 8, 4),
@@ -310,11 +316,27 @@ static uint32_t cu_count() {
     return cus;
 }
 
+// Workgroups of a persistent variant resident on one CU at once (registers,
+// LDS, waves): its grid holds no more, since a workgroup beyond them would
+// start only when one of them has finished all of its tiles.
+static uint32_t resident_per_cu(const RegVariant* v) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, uint32_t> done;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = done.find(v->fn);
+    if (it != done.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, v->fn, v->nthreads, v->lds_bytes) != hipSuccess || n < 1)
+        n = 1;
+    done[v->fn] = (uint32_t)n;
+    return (uint32_t)n;
+}
+
 hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s) {
     hipError_t e = ensure_lds_attr(v->fn, v->lds_bytes);
     if (e != hipSuccess) return e;
     uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
-    if (v->persist) blocks = std::min<uint32_t>(blocks, cu_count() * (uint32_t)v->persist);
+    if (v->persist) blocks = std::min<uint32_t>(blocks, cu_count() * resident_per_cu(v));
     LineArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchKernel(v->fn, dim3(blocks), dim3(v->nthreads), args, v->lds_bytes, s);

@@ -28,9 +28,11 @@ struct RegVariant {
     size_t lds_bytes;
     const void* fn;
     const char* radices;  // e.g. "16x16x16"
-    // 0: one tile per workgroup.  k > 0: persistent pipelined form (k
-    // workgroups per CU, each loading its next tile before transforming the
-    // current one).
+    // 0: one tile per workgroup.  k > 0: persistent pipelined form, each
+    // workgroup loading its next tile before transforming the current one;
+    // k is its launch bound's minimum of waves per SIMD (the register
+    // budget), and its grid holds as many workgroups per CU as are resident
+    // at once (hipOccupancyMaxActiveBlocksPerMultiprocessor).
     int persist = 0;
     // F > 0: fold variant (F contiguous lines M/F rows apart per thread, the
     // next axis's first radix-F stage on store); looked up only by fold passes.
