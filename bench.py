@@ -566,20 +566,21 @@ def profiled_step_traffic(workload):
 def profiled_traffic(desc, prec="f32"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on precision,
-    length, radices and lines per thread; (None, None) when no summary covers
-    it."""
+    length, threads per line, radices and lines per thread; (None, None) when
+    no summary covers it."""
     import glob
     import re
-    m = re.search(r"N=(\d+) (\S+) .*vec=(\d)", desc)
+    m = re.search(r"N=(\d+) (\S+) .*tpl=(\d+) vec=(\d)", desc)
     if not m:
         return None, None
-    n, rad, vec = m.groups()
+    n, rad, tpl, vec = m.groups()
     lf = "lf=" + ("t" if "ldLF" in desc else "f") + ("t" if "stLF" in desc else "f")
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             if not line.startswith("traffic_bytes_per_launch:"):
                 continue
-            if (f" {prec} N={n} " in line and f"radices={rad} " in line and f"vec={vec}" in line
+            if (f" {prec} N={n} " in line and f" tpl={tpl} " in line and f"radices={rad} " in line
+                    and f"vec={vec}" in line
                     and f" {lf} " in line
                     and (" post" in line) == (" post" in desc) and ("pipe" in line) == ("pipe" in desc)
                     and (" fold" in line) == (" fold" in desc)):

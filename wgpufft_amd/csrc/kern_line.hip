@@ -201,6 +201,11 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 2048, 256, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8x4", 8, 8, 8, 4),
     REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
     REGV(FmtF32, P_F32, 1024, 64, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    // ... and more threads per line (8 points each): more waves in flight
+    // for the 513 to 1025 workgroups of those columns
+    REGV(FmtF32, P_F32, 1024, 128, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4x4", 8, 8, 4, 4),
+    REGV(FmtF32, P_F32, 1024, 128, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x4x4", 8, 8, 4, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
     // strided line pairs in 256-thread workgroups (the batched kernel's shape)
     REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "16x16x4", 16, 16, 4),
     REGV(FmtF32, P_F32, 512, 32, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 2, "8x8x8", 8, 8, 8),
@@ -247,7 +252,6 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF64, P_F64, 4096, 128, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x16", 16, 16, 16),
     // adjacent-butterfly forms: 16-byte loads and stores of contiguous lines
     REGFA(4096, 256, 2, "8x8x8x8", 8, 8, 8, 8),
-    REGFA(4096, 256, 2, "4x16x16x4", 4, 16, 16, 4),
     REGFA(4096, 256, 2, "8x16x8x4", 8, 16, 8, 4),
     REGFA(4096, 128, 2, "8x8x8x8", 8, 8, 8, 8),
     REGFA(4096, 128, 2, "16x16x16", 16, 16, 16),
