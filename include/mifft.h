@@ -270,7 +270,7 @@ mifft_status mifft_execute_with_workspace(const mifft_plan* p, hipStream_t strea
  * streams, other threads) race and must not be issued; calls in order on one
  * stream are safe.  The plan's kernels write every scratch element they read,
  * so nothing needs clearing between calls. */
-#define MIFFT_STATIC_SCRATCH_BYTES (64ull << 20)
+#define MIFFT_STATIC_SCRATCH_BYTES (80ull << 20)
 mifft_status mifft_execute_static_scratch(const mifft_plan* p, hipStream_t stream, const void* in,
                                           uint64_t in_bytes, void* out, uint64_t out_bytes);
 /* FftPlan::execute_in_place (plan.rs:827-835). */

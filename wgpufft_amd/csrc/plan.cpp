@@ -397,6 +397,16 @@ static std::string sched_str(const Sched& sc) {
     return s.empty() ? "1" : s;
 }
 
+// Row pitch (elements) of a workspace copy that a later pass reads down its
+// columns: a pass reading narrow row segments (64-byte column pairs) one row
+// after another at a pitch that is a multiple of 128 bytes sends a
+// workgroup's rows to the same few HBM channels; 64 more bytes per row spread
+// them.  Measured on MI355X (4096 x 2048 f32 pair): pads of 64, 192, 320 and
+// 576 bytes run 4-5 % faster than none, 128 and 256 bytes no faster.
+static uint64_t workspace_pitch(uint64_t elems, size_t es) {
+    return (elems * es) % 128 == 0 ? elems + 64 / es : elems;
+}
+
 static bool adjacent_lines(const LineSet& ls, const Layout& l) {
     if (l.sn == 1) return false;
     if (ls.S1 > 1) return l.si == 1;
@@ -1156,8 +1166,8 @@ struct Builder {
     // at least 512.  R2C
     // (real load, packed store) folds the packed rows of P0 = n0/2 + 1 bins,
     // and pass 2 runs over P0 columns.
-    bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale, int ldm = LD_COMPLEX,
-                 int stm = ST_COMPLEX) {
+    bool fold_2d(uint64_t n0, uint64_t n1, uint64_t O, bool inverse, double scale, uint64_t* ws_bytes,
+                 int ldm = LD_COMPLEX, int stm = ST_COMPLEX) {
         if (P.batch_mode || P.prec != P_F32) return false;
         const bool real = ldm == LD_REAL;
         const std::string choice = tuning_lookup(std::string(real ? "fold|f32|r" : "fold|f32|n") +
@@ -1175,17 +1185,21 @@ struct Builder {
         // pass 1: axis 0 over row groups (r + b Q, b < F), fold on store
         const LineSet ls1{(uint32_t)(n1 * O), 1, (uint32_t)Q};
         const Layout lay_in{0, (int64_t)pin, (int64_t)(pin * n1), 1};
-        const Layout lay_out{0, (int64_t)pout, (int64_t)(pout * n1), 1};
-        fold_row_op(n0, n1, F, ldm, stm, ls1, lay_in, lay_out, Q * pin, Q * pout, B_IN, B_WS, inverse, 1.0, fv);
+        // the workspace rows at a pitch of pw elements (see workspace_pitch)
+        const uint64_t pw = workspace_pitch(pout, elem_bytes(P.prec));
+        if (pw * n1 * O > 0xFFFFFFFFull) return false;
+        const Layout lay_out{0, (int64_t)pw, (int64_t)(pw * n1), 1};
+        fold_row_op(n0, n1, F, ldm, stm, ls1, lay_in, lay_out, Q * pin, Q * pw, B_IN, B_WS, inverse, 1.0, fv);
         // pass 2: Q-point lines, column i of part c of array o, to rows F k + c
         const uint64_t C = pout;
         const LineSet ls2{(uint32_t)(C * F * O), (uint32_t)C, (uint32_t)F};
-        const Layout in2{1, (int64_t)(Q * C), (int64_t)(C * n1), (int64_t)C};
+        const Layout in2{1, (int64_t)(Q * pw), (int64_t)(pw * n1), (int64_t)pw};
         const Layout out2{1, (int64_t)C, (int64_t)(C * n1), (int64_t)(F * C)};
         if (!line_op(Q, ls2, in2, out2, B_WS, 0, B_OUT, 0, inverse, inverse, scale, LD_COMPLEX, ST_COMPLEX)) {
             P.ops.pop_back();
             return false;
         }
+        *ws_bytes = pw * n1 * O * elem_bytes(P.prec);
         return true;
     }
 
@@ -1458,6 +1472,7 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
         const size_t cb = elem_bytes(P->prec);
         if (kind == MIFFT_C2C) {
             P->in_bytes = P->out_bytes = total * cb;
+            uint64_t fold_ws = 0;
             if (total == 1) {
                 Op op;
                 op.kind = OP_SCALE_COPY;
@@ -1471,9 +1486,9 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
             } else if (cfg.naxes == 2 && cfg.axes[0] == 0 && cfg.axes[1] == 1 &&
                        P->axes[0].kind == MIFFT_AXIS_MIXED && P->axes[1].kind == MIFFT_AXIS_MIXED &&
                        B.fold_2d(cfg.shape[0], cfg.shape[1], total / (cfg.shape[0] * cfg.shape[1]), inverse,
-                                 scale)) {
+                                 scale, &fold_ws)) {
                 // the fold pair ping-pongs through one workspace copy of the array
-                P->ws_bytes = total * cb;
+                P->ws_bytes = fold_ws;
             } else {
                 // Ping-pong: with several axes, consecutive passes alternate between
                 // the output and a workspace copy of the array (the last pass lands
@@ -1569,9 +1584,8 @@ std::unique_ptr<Plan> build_plan(const mifft_config& cfg, mifft_kind kind, const
             };
             uint64_t ws = 0;
             if (kind == MIFFT_R2C && cfg.rank == 2 && smooth0 && P->axes[1].kind == MIFFT_AXIS_MIXED &&
-                B.fold_2d(N0, cfg.shape[1], cfg.batch, false, scale, LD_REAL, ST_PACKED)) {
+                B.fold_2d(N0, cfg.shape[1], cfg.batch, false, scale, &ws, LD_REAL, ST_PACKED)) {
                 // 2D fold: real rows -> folded packed rows in the workspace -> output
-                ws = packed * 8;
             } else if (kind == MIFFT_R2C) {
                 const double s0 = cfg.rank == 1 ? scale : 1.0;
                 const bool fused0 = smooth0 && B.line_op(N0, l0, real_l, pack_l, B_IN, 0, B_OUT, 0, false, false,

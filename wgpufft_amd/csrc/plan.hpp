@@ -141,7 +141,7 @@ int plan_launch_count(const Plan& p);
 // Size of the static device scratch of mifft_execute_static_scratch
 // (scratch.hip) and its device address on `device`, resolved once per device
 // with that device current (fails on any other device).
-constexpr uint64_t kStaticScratchBytes = 64ull << 20;
+constexpr uint64_t kStaticScratchBytes = 80ull << 20;
 void* static_scratch(int device);
 // Described device tables of the plan and its child plans, in creation order.
 void plan_tables(const Plan& p, std::vector<const TableInfo*>* out);
