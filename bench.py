@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2c2d", choices=["c2c2d", "c1", "r2c", "c4a_f64", "c4a_df64", "c4b_f64", "batched", "serve"])
+    ap.add_argument("--workload", default="c2c2d",
+                    choices=["c2c2d", "c1", "r2c", "c4a_f64", "c4a_df64", "c4b_f64", "batched", "serve", "reference"])
     ap.add_argument("--requests", type=int, default=6000, help="serve workload: trace length")
     ap.add_argument("--rate", type=float, default=20000.0, help="serve workload: Poisson arrival rate (req/s)")
     ap.add_argument("--tick", type=int, default=256, help="serve workload: most requests admitted per tick")
@@ -111,6 +112,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return
+    if args.workload == "reference":
+        print(json.dumps(run_reference_configs(args)), flush=True)
         return
     spec = workload_spec(args.workload)
     torch.cuda.init()
@@ -307,6 +311,120 @@ def main():
 
 
 BATCH_LENGTHS = [120, 343, 1001, 127, 1009, 4096, 65536]
+
+
+# The reference's published FFT+iFFT pair timings (unstated GPU, Vulkan; the
+# bench records every pair of a run into one submission on one buffer, in
+# place: benches/fft_bench.rs:880-990): (name, shape, batch, published ms,
+# source).  "over 1 GiB" batches the lines of a 1 GiB f32 complex buffer.
+GIB_LINES = lambda n: (1 << 30) // (8 * n)
+REFERENCE_CONFIGS = [
+    ("4096x2048 in place", [4096, 2048], 1, 0.165, "README.md:104-106, CHANGELOG.md:79-81"),
+    ("2048x2048", [2048, 2048], 1, 0.114, "CHANGELOG.md:71-73"),
+    ("4096x4096", [4096, 4096], 1, 0.75, "CHANGELOG.md:74"),
+    ("1024x1024", [1024, 1024], 1, 0.029, "CHANGELOG.md:85-86"),
+    ("128x128x64", [128, 128, 64], 1, 0.032, "CHANGELOG.md:85"),
+    ("256x256x128 in place", [256, 256, 128], 1, 0.172, "CHANGELOG.md:80-81"),
+    ("65536 x 64 lines", [65536], 64, 0.079, "CHANGELOG.md:154"),
+    ("2^19", [1 << 19], 1, 0.017, "CHANGELOG.md:152"),
+    ("2^20", [1 << 20], 1, 0.025, "CHANGELOG.md:152"),
+    ("2^21", [1 << 21], 1, 0.049, "CHANGELOG.md:153"),
+    ("2^24", [1 << 24], 1, 0.78, "CHANGELOG.md:153"),
+    ("17x17", [17, 17], 1, 0.0085, "CHANGELOG.md:54"),
+    ("N=17 over 1 GiB", [17], GIB_LINES(17), 2.8, "CHANGELOG.md:55-56"),
+    ("N=1009 over 1 GiB", [1009], GIB_LINES(1009), 3.8, "CHANGELOG.md:63-65"),
+    ("N=2003 over 1 GiB", [2003], GIB_LINES(2003), 3.7, "CHANGELOG.md:66-67"),
+    ("N=3256 over 1 GiB", [3256], GIB_LINES(3256), 7.4, "CHANGELOG.md:57-60"),
+    ("N=4093 over 1 GiB", [4093], GIB_LINES(4093), 6.5, "CHANGELOG.md:57-60"),
+    ("1229x1229", [1229, 1229], 1, 0.131, "CHANGELOG.md:69-70"),
+    ("6841x6841", [6841, 6841], 1, 7.5, "CHANGELOG.md:60-61"),
+]
+
+
+def run_reference_configs(args):
+    """The configurations the reference publishes pair timings for, measured
+    as its bench runs them: forward then inverse in place on one buffer, K
+    pairs recorded into one submission (a hipGraph here) and replayed, median
+    of 5 replays; the forward output of each is checked against torch.fft in
+    complex128."""
+    import numpy as np
+    import torch
+
+    from wgpufft_amd import FftConfig, FftPlan, FftRecorder, clear_tuning, fill_uniform_f32, load_tuning
+    from wgpufft_amd.tune import DEFAULT_TABLE
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    clear_tuning()
+    if args.tuning == "table" and os.path.exists(DEFAULT_TABLE):
+        load_tuning(DEFAULT_TABLE)
+    rows = []
+    for name, shape, batch, ref_ms, src in REFERENCE_CONFIGS:
+        fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch))
+        inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch))
+        nbytes = fwd.required_buffer_size_bytes()
+        buf = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        fill_uniform_f32(buf, 11)
+        # forward of (at most 64 MiB of) the buffer against torch.fft in complex128
+        per = int(np.prod(shape))
+        nchk = max(1, min(batch, (64 << 20) // (8 * per)))
+        x = buf.view(torch.complex64)[:per * nchk].reshape([nchk] + shape[::-1]).to(torch.complex128)
+        want = torch.fft.fftn(x, dim=list(range(1, len(shape) + 1)))
+        y = buf.clone()
+        fwd.execute_in_place(y, stream)
+        torch.cuda.synchronize()
+        got = y.view(torch.complex64)[:per * nchk].reshape([nchk] + shape[::-1]).to(torch.complex128)
+        err = (torch.linalg.norm(got - want) / torch.linalg.norm(want)).item()
+        del x, want, y, got
+
+        # as the reference records every pair of a run into one command
+        # buffer, the K pairs are recorded into one hipGraph (FftRecorder) on
+        # a side stream and replayed; the stream-launched rate is kept beside
+        def pairs(k):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(k):
+                fwd.execute_in_place(buf, stream)
+                inv.execute_in_place(buf, stream)
+            e1.record(stream)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / k
+        first = pairs(2)
+        k = int(max(3, min(200, 200.0 / max(first, 1e-3) / 5)))
+        ms_stream = sorted(pairs(k) for _ in range(5))[2]
+        side = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        rec = FftRecorder(stream=side, capture_graph=True)
+        for _ in range(k):
+            fwd.record_in_place(rec, buf)
+            inv.record_in_place(rec, buf)
+        rec.end()
+
+        def replay():
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(side)
+            rec.launch(side)
+            e1.record(side)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / k
+        replay()
+        ms = sorted(replay() for _ in range(5))[2]
+        del rec
+        rows.append({"config": name, "shape": shape, "batch": batch, "ms_per_pair": round(ms, 5),
+                     "ms_per_pair_stream_launched": round(ms_stream, 5),
+                     "reference_ms": ref_ms, "speedup_vs_reference": round(ref_ms / ms, 2),
+                     "forward_rel_l2_vs_fp64": err, "pairs": k, "route": fwd.describe(), "source": src})
+        print(f"{name:22s} {ms:9.4f} ms/pair (stream-launched {ms_stream:8.4f})  reference {ref_ms:7.4f} ms  "
+              f"x{ref_ms / ms:6.2f}  rel_l2 {err:.1e}",
+              file=sys.stderr, flush=True)
+        del buf, fwd, inv
+        torch.cuda.empty_cache()
+    return {"metric": "FFT+iFFT pair ms on the reference's published configurations", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic (uniform[-1,1) on device)",
+            "reference_hardware": "unstated GPU (Vulkan), README.md:402-403", "configs": rows}
 
 
 def batched_items(count, seed=4):
