@@ -1,0 +1,87 @@
+"""GPU parity of the register-resident Bluestein kernel (the reference's
+register Bluestein, src/runtime/register_fft.rs:510-664, bluestein_axis.rs:
+937-969; Rader primes whose convolution would be zero-padded take it too,
+rader_axis.rs:1262-1280): f32 Bluestein lengths and such Rader primes on
+contiguous lines, every power-of-two convolution length 64..8192, ragged
+line counts, both directions, in place and out of place, against the oracle
+(relative L2 <= 1e-6, tests/gpu_accuracy.rs:94-104); forced Rader keeps the
+Rader route, and strided axes keep theirs."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import c2c_expected, empty_dev, from_dev, seeded_complex, to_dev
+from wgpufft_amd import FftConfig, FftDirection, FftPlan, FftTuning, Normalization
+
+pytestmark = pytest.mark.gpu
+
+
+def run(cfg, x, in_place=False):
+    import torch
+    plan = FftPlan.c2c(cfg)
+    d = to_dev(x)
+    if in_place:
+        plan.execute_in_place(d)
+        out = d
+    else:
+        out = empty_dev(x.nbytes)
+        plan.execute(d, out)
+    torch.cuda.synchronize()
+    return plan, from_dev(out, np.complex64)
+
+
+# (length, batch, forced Bluestein, convolution length)
+CASES = [
+    (31, 37, True, 64),      # forced: M = 64
+    (34, 101, False, 128),   # 2 * 17: Bluestein, M = 128
+    (37, 16, True, 128),
+    (100, 5, True, 256),
+    (131, 33, False, 512),   # prime, 130 = 2 * 5 * 13 smooth: forced below
+    (263, 9, False, 1024),   # prime, 262 = 2 * 131: zero-padded Rader -> register Bluestein
+    (1000, 3, True, 2048),
+    (1229, 5, False, 4096),  # prime, 1228 = 4 * 307
+    (3256, 3, False, 8192),  # 8 * 11 * 37: Bluestein (the reference's 1 GiB case)
+    (4093, 2, False, 8192),  # prime, 4092 = 4 * 3 * 11 * 31
+]
+
+
+@pytest.mark.parametrize("n,batch,forced,m", CASES)
+@pytest.mark.parametrize("direction", [FftDirection.Forward, FftDirection.Inverse])
+def test_register_bluestein_matches_oracle(torch_cuda, n, batch, forced, m, direction):
+    if n == 131:
+        forced = True  # cyclic Rader by default; the Bluestein route when forced
+    tuning = FftTuning().with_force_bluestein_axes([0]) if forced else FftTuning()
+    cfg = FftConfig.new(n).with_batch(batch).with_direction(direction).with_tuning(tuning)
+    x = seeded_complex(900 + n + batch, n * batch)
+    plan, y = run(cfg, x, in_place=batch % 2 == 1)
+    assert f"bluestein-reg N={n} M={m} " in plan.describe(), plan.describe()
+    ref = c2c_expected(x, [n], batch=batch, inverse=direction == FftDirection.Inverse)
+    e = oracle.rel_l2(y, ref)
+    print(f"N={n} b={batch} M={m} {direction.name} rel_l2={e:.3e} max_rel={oracle.max_rel(y, ref):.3e} "
+          f"[{plan.describe()}]", flush=True)
+    assert e <= 1e-6
+
+
+def test_register_bluestein_normalization_and_2d(torch_cuda):
+    # [1229, 6]: the contiguous prime axis through the register kernel, the
+    # strided one as usual; orthogonal normalization (1/sqrt(N) each way)
+    shape = [1229, 6]
+    x = seeded_complex(77, 1229 * 6 * 2)
+    cfg = FftConfig.new_nd(shape).with_batch(2).with_normalization(Normalization.Orthogonal)
+    plan, y = run(cfg, x)
+    assert "bluestein-reg N=1229 M=4096" in plan.describe(), plan.describe()
+    ref = c2c_expected(x, shape, batch=2, norm=oracle.NORM_ORTHO)
+    e = oracle.rel_l2(y, ref)
+    print(f"{shape} orthogonal rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= 1e-6
+
+
+def test_forced_rader_and_strided_axes_keep_their_routes(torch_cuda):
+    p1 = FftPlan.c2c(FftConfig.new(1229).with_batch(2).with_tuning(FftTuning().with_force_rader_axes([0])))
+    assert "bluestein-reg" not in p1.describe() and "rader" in p1.describe(), p1.describe()
+    # a strided Bluestein axis (axis 1 of [8, 1229]) does not take the register kernel
+    p2 = FftPlan.c2c(FftConfig.new_nd([8, 1229]).with_axes([1]))
+    assert "bluestein-reg" not in p2.describe(), p2.describe()
+    x = seeded_complex(78, 1229 * 2)
+    _, y = run(FftConfig.new(1229).with_batch(2).with_tuning(FftTuning().with_force_rader_axes([0])), x)
+    assert oracle.rel_l2(y, c2c_expected(x, [1229], batch=2)) <= 1e-6

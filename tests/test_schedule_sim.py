@@ -413,3 +413,94 @@ def test_register_kernel_thread_map(rad, tpl, adj):
                 for m in range(R2):
                     regs[(t, q, m)] = lds[j + m * nr2]
     np.testing.assert_allclose(out, np.fft.fft(x), rtol=1e-9, atol=1e-9 * N)
+
+
+def _reg_fft_threads(w, rad, tpl, tab):
+    """RegBluestein::stages: registers w[t][q R_0 + m] hold point j + m M/R_0
+    (j = t + q tpl); stage s multiplies register q R + m by table entry
+    twoff(s) + (m-1) L + (j mod L), runs the radix-R DFT, and exchanges
+    through LDS index (j - j mod L) R + (j mod L) + m L, read back as point
+    j + m M/R_{s+1}.  Returns the registers after the last stage."""
+    M = int(np.prod(rad))
+    L = 1
+    for s, R in enumerate(rad):
+        nr = M // R
+        nb = nr // tpl
+        for t in range(tpl):
+            for q in range(nb):
+                j = t + q * tpl
+                k = j % L
+                v = np.array([w[t][q * R + m] for m in range(R)])
+                if L > 1:
+                    for m in range(1, R):
+                        v[m] *= tab[twoff(rad, s) + (m - 1) * L + k]
+                v = np.fft.fft(v)
+                for m in range(R):
+                    w[t][q * R + m] = v[m]
+        if s + 1 == len(rad):
+            break
+        lds = np.zeros(M, dtype=complex)
+        for t in range(tpl):
+            for q in range(nb):
+                j = t + q * tpl
+                k = j % L
+                for m in range(R):
+                    lds[(j - k) * R + k + m * L] = w[t][q * R + m]
+        L *= R
+        R2 = rad[s + 1]
+        nr2 = M // R2
+        for t in range(tpl):
+            w[t] = [0] * len(w[t])
+            for q in range(nr2 // tpl):
+                j = t + q * tpl
+                for m in range(R2):
+                    w[t][q * R2 + m] = lds[j + m * nr2]
+    return w
+
+
+@pytest.mark.parametrize("n,rad,tpl", [(37, [8, 2, 8], 8), (100, [8, 8, 8], 32), (509, [16, 4, 16], 64),
+                                       (1000, [16, 8, 16], 128), (31, [8, 8], 8)])
+def test_register_bluestein_map(n, rad, tpl):
+    """RegBluestein::run: a[p] = x[p] c[p] loaded into the first stage's
+    registers, forward transform, register q R_0 + m multiplied by F[p] and
+    swapped with p = j + m M/R_0 (the last stage's output index of that
+    register, since R_0 = R_last), forward transform again, X[p] = swap(.)
+    c[p] for p < n: the DFT of x."""
+    M = int(np.prod(rad))
+    assert rad[0] == rad[-1] and M >= 2 * n - 1
+    x = crand(n)
+    c = W(2 * n, np.arange(n, dtype=np.int64) ** 2)          # chirp W_2n^(p^2)
+    seq = np.zeros(M, dtype=complex)
+    seq[:n] = np.conj(c)
+    seq[M - n + 1:] = np.conj(c[1:][::-1])
+    F = np.fft.fft(seq) / M
+    tab = stage_table(rad)
+    R0 = rad[0]
+    nr0 = M // R0
+    nb0 = nr0 // tpl
+    w = [[0j] * (M // tpl) for _ in range(tpl)]
+    for t in range(tpl):
+        for q in range(nb0):
+            j = t + q * tpl
+            for m in range(R0):
+                p = j + m * nr0
+                w[t][q * R0 + m] = x[p] * c[p] if p < n else 0
+    w = _reg_fft_threads(w, rad, tpl, tab)
+    for t in range(tpl):
+        for q in range(nb0):
+            j = t + q * tpl
+            for m in range(R0):
+                p = j + m * nr0
+                v = w[t][q * R0 + m] * F[p]
+                w[t][q * R0 + m] = complex(v.imag, v.real)
+    w = _reg_fft_threads(w, rad, tpl, tab)
+    out = np.zeros(n, dtype=complex)
+    for t in range(tpl):
+        for q in range(nb0):
+            j = t + q * tpl
+            for m in range(R0):
+                p = j + m * nr0
+                if p < n:
+                    v = w[t][q * R0 + m]
+                    out[p] = complex(v.imag, v.real) * c[p]
+    np.testing.assert_allclose(out, np.fft.fft(x), rtol=1e-9, atol=1e-9 * n)

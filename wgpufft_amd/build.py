@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmifft.so")
-SOURCES = ["kern_line.hip", "kern_conv.hip", "kern_util.hip", "kern_batch.hip", "kern_small.hip", "scratch.hip", "plan.cpp",
+SOURCES = ["kern_line.hip", "kern_conv.hip", "kern_regconv.hip", "kern_util.hip", "kern_batch.hip", "kern_small.hip", "scratch.hip", "plan.cpp",
            "api.cpp", "diagnostics.cpp", "batch.cpp", "serve.cpp", "cpu_plan.cpp", "cpu_fft_avx2.cpp", "cpu_fft_base.cpp"]
 # Host-only translation units of the CPU FFT engine: plain C++ (no device
 # code), one built for AVX2 + FMA and one for the x86-64 baseline; the library

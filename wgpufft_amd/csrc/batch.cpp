@@ -24,6 +24,22 @@ static void flat_ops(const Plan& p, std::vector<const Op*>& out) {
     }
 }
 
+// Batched class of the register Bluestein kernel of convolution length m.
+static uint32_t regb_batch_class(int m) {
+    switch (m) {
+        case 64: return BC_REGB_64;
+        case 128: return BC_REGB_128;
+        case 256: return BC_REGB_256;
+        case 512: return BC_REGB_512;
+        case 1024: return BC_REGB_1024;
+        case 2048: return BC_REGB_2048;
+        case 4096: return BC_REGB_4096;
+        case 8192: return BC_REGB_8192;
+    }
+    fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "no batched class for register Bluestein M=" + std::to_string(m));
+    return 0;
+}
+
 static std::string batch_reject_reason(const Plan& p) {
     std::vector<const Op*> ops;
     flat_ops(p, ops);
@@ -210,6 +226,12 @@ void batch_build(const BatchEntry* entries, int32_t count, char* ws_base, BatchB
                     d.conv = op.conv;
                     lpw = op.conv.sc.lpw;
                     op_lds = conv_kernel_bytes(op.prec, op.conv.m, op.conv.sc.lpw);
+                    if (op.rcv) {
+                        // the register Bluestein kernel's own class (same tile body)
+                        cls = regb_batch_class(op.rcv->m);
+                        lpw = (uint32_t)op.rcv->lines;
+                        op_lds = op.rcv->lds_bytes;
+                    }
                     unit = 3ull * op.conv.m;  // convolutions count 3x
                     break;
                 case OP_CHIRP_PRE:

@@ -49,6 +49,15 @@ enum BatchClass : uint32_t {
     BC_REG_S256V2,
     BC_REG_S128V2,
     BC_REG_S64V2,
+    // register-resident Bluestein, one class per convolution length
+    BC_REGB_64,
+    BC_REGB_128,
+    BC_REGB_256,
+    BC_REGB_512,
+    BC_REGB_1024,
+    BC_REGB_2048,
+    BC_REGB_4096,
+    BC_REGB_8192,
     BC_COUNT
 };
 

@@ -5,7 +5,7 @@
 // one pass, sized for one 256-thread workgroup) and sorts them by cost.
 // Resident workgroups pull tile indices from a device counter and run the
 // tile with the device function of its class (register-resident line kernel,
-// runtime-schedule LDS kernel, fused Rader or Bluestein).  A tile computes
+// runtime-schedule LDS kernel, fused Rader or Bluestein, register Bluestein).  A tile computes
 // exactly what the same op computes in a standalone plan, so every transform
 // is bit-identical alone and in the mix.  Passes that depend on a previous
 // pass of the same transform (the next axis, the second half of a four-step
@@ -15,6 +15,7 @@
 // This is the gfx950 counterpart of recording many plans into one shared
 // compute pass with FftRecorder (src/plan.rs:38-95, src/runtime/recorder.rs).
 #include "batch_kernels.hpp"
+#include "reg_conv.hpp"
 
 namespace mifft {
 
@@ -72,6 +73,12 @@ template <class Fmt>
 __device__ __noinline__ uint32_t rader_tiles(MIFFT_TILE_ARGS) {
     return tile_run(tiles, ntiles, counter, ops, t,
                     [&](const OpDesc& d, uint32_t g0) { rader_tile<Fmt>(d.conv, smem, g0); });
+}
+template <class K>
+__device__ __noinline__ uint32_t regb_tiles(MIFFT_TILE_ARGS) {
+    return tile_run(tiles, ntiles, counter, ops, t, [&](const OpDesc& d, uint32_t g0) {
+        K::tile(d.conv, reinterpret_cast<cpx<typename K::T>*>(smem), g0);
+    });
 }
 template <class Fmt>
 __device__ __noinline__ uint32_t bluestein_tiles(MIFFT_TILE_ARGS) {
@@ -157,6 +164,14 @@ __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ 
             case BC_REG_S256V2: t = reg_tiles<MIFFT_BK(256, 16, 32, 1, 1, 0, 2, 16, 16)>(MIFFT_TILE_PASS); break;
             case BC_REG_S128V2: t = reg_tiles<MIFFT_BK(128, 8, 64, 1, 1, 0, 2, 16, 8)>(MIFFT_TILE_PASS); break;
             case BC_REG_S64V2: t = reg_tiles<MIFFT_BK(64, 4, 128, 1, 1, 0, 2, 16, 4)>(MIFFT_TILE_PASS); break;
+            case BC_REGB_64: t = regb_tiles<RegB64>(MIFFT_TILE_PASS); break;
+            case BC_REGB_128: t = regb_tiles<RegB128>(MIFFT_TILE_PASS); break;
+            case BC_REGB_256: t = regb_tiles<RegB256>(MIFFT_TILE_PASS); break;
+            case BC_REGB_512: t = regb_tiles<RegB512>(MIFFT_TILE_PASS); break;
+            case BC_REGB_1024: t = regb_tiles<RegB1024>(MIFFT_TILE_PASS); break;
+            case BC_REGB_2048: t = regb_tiles<RegB2048>(MIFFT_TILE_PASS); break;
+            case BC_REGB_4096: t = regb_tiles<RegB4096>(MIFFT_TILE_PASS); break;
+            case BC_REGB_8192: t = regb_tiles<RegB8192>(MIFFT_TILE_PASS); break;
             default: t = claim_tile(counter); break;  // unknown class: skip the tile
         }
     }

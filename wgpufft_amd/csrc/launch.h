@@ -75,6 +75,17 @@ struct ConvArgs {
 };
 hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s);
 hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s);
+// Register-resident f32 Bluestein kernels (reg_conv.hpp, kern_regconv.hip),
+// one per power-of-two convolution length M: `lines` lines of `tpl` threads
+// per workgroup, radix schedule `radices` (stage twiddles in that format).
+struct RegConvVariant {
+    int m, tpl, lines, nthreads;
+    size_t lds_bytes;
+    const void* fn;
+    const char* radices;
+};
+const RegConvVariant* find_reg_bluestein(uint32_t m);
+hipError_t launch_reg_bluestein(const RegConvVariant* v, const ConvArgs& c, hipStream_t s);
 // Direct f32 DFT of prime lines of length p (kern_conv.hip), lpw lines per
 // workgroup, lines-fastest slots when lf; wtab = W_p^t, t < p.
 size_t direct_prime_bytes(uint32_t p, uint32_t lpw);

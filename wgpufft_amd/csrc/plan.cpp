@@ -806,6 +806,38 @@ struct Builder {
         P.ops.push_back(op);
     }
 
+    // Register-resident Bluestein (reg_conv.hpp) over the power-of-two
+    // convolution length M >= 2N - 1, when a compiled kernel has that length:
+    // one kernel per axis whose two transforms run in registers.
+    bool reg_bluestein_op(uint64_t n, const LineSet& ls, const Layout& in, const Layout& out, int src,
+                          uint64_t src_off, int dst, uint64_t dst_off, bool conj_in, bool conj_out, double scale) {
+        if (P.prec != P_F32 || n < 2 || n > 4096) return false;
+        uint64_t m = 64;
+        while (m < 2 * n - 1) m *= 2;
+        const RegConvVariant* v = find_reg_bluestein((uint32_t)m);
+        if (!v || v->lds_bytes > P.lds_budget) return false;
+        Op op;
+        op.kind = OP_BLUESTEIN;
+        op.rcv = v;
+        op.src = src;
+        op.src_off = src_off;
+        op.dst = dst;
+        op.dst_off = dst_off;
+        op.prec = P.prec;
+        op.conv.io = base_args(ls, in, out, conj_in, conj_out, scale);
+        op.conv.io.tw = stage_twiddles(P.arith, parse_radices(v->radices));
+        op.conv.filt = bluestein_filter(n, m);
+        op.conv.chirp = chirp_table(n);
+        op.conv.n = (uint32_t)n;
+        op.conv.m = (uint32_t)m;
+        std::ostringstream d;
+        d << "bluestein-reg N=" << n << " M=" << m << " " << v->radices << " lines=" << ls.lines
+          << " tpl=" << v->tpl << " lpw=" << v->lines;
+        op.desc = d.str();
+        P.ops.push_back(op);
+        return true;
+    }
+
     // Convolution length of a multi-pass Bluestein axis: any smooth M >= 2N-1
     // serves (bluestein_axis.rs:1035-1041 takes the smallest); among the smooth
     // lengths up to 6 % above the smallest, the one whose padded transform
@@ -979,6 +1011,11 @@ struct Builder {
     // cyclic one of length p - 1 (p - 1 not smooth), unless Rader is forced
     // on the axis; not in batched plans (no batched class).
     int cur_axis = -1;  // axis being planned, for the forced-axis tunings
+    static bool rader_forced(const mifft_tuning& t, int axis) {
+        for (int i = 0; i < t.n_force_rader_axes; i++)
+            if (t.force_rader_axes[i] == axis) return true;
+        return false;
+    }
     bool direct_prime_op(uint64_t p, const LineSet& ls, const Layout& lay, int src, uint64_t src_off, int dst,
                          uint64_t dst_off, bool inverse, double scale) {
         const mifft_tuning& t = P.cfg.tuning;
@@ -1303,6 +1340,14 @@ struct Builder {
             return stockham_passes(n, S, O, src, src_off, dst, dst_off, ws_off, inverse, scale);
         }
         if (kind == MIFFT_AXIS_RADER && direct_prime_op(n, ls, lay, src, src_off, dst, dst_off, inverse, scale))
+            return 0;
+        // Bluestein axes, and Rader primes whose convolution would be
+        // zero-padded (p - 1 not smooth) unless Rader is forced, on contiguous
+        // lines: the register-resident Bluestein kernel (rader_axis.rs:1262-1280)
+        const bool rader_padded = kind == MIFFT_AXIS_RADER && rader_conv_len(n) != n - 1 &&
+                                  !(cur_axis >= 0 && rader_forced(P.cfg.tuning, cur_axis));
+        if ((kind == MIFFT_AXIS_BLUESTEIN || rader_padded) && !lf &&
+            reg_bluestein_op(n, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale))
             return 0;
         if (kind == MIFFT_AXIS_RADER) {
             const uint64_t m = rader_conv_len(n);
@@ -1697,7 +1742,9 @@ static void run_op(const Op& op, hipStream_t s, const void* in, void* out, void*
             ConvArgs c = op.conv;
             c.io.in = ptr(op.src, op.src_off);
             c.io.out = ptr(op.dst, op.dst_off);
-            hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s) : launch_bluestein(op.prec, c, s),
+            hip_check(op.kind == OP_RADER ? launch_rader(op.prec, c, s)
+                      : op.rcv            ? launch_reg_bluestein(op.rcv, c, s)
+                                          : launch_bluestein(op.prec, c, s),
                       "convolution kernel");
             break;
         }

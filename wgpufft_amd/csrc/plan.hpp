@@ -57,6 +57,7 @@ struct Op {
     SmallVolArgs sv{};  // OP_SMALL_VOLUME (volumes in count)
     StockhamStage st{}; // OP_STOCKHAM
     const RegVariant* rv = nullptr;
+    const RegConvVariant* rcv = nullptr;  // OP_BLUESTEIN: register-resident form
     int ldm = LD_COMPLEX, stm = ST_COMPLEX, post = 0;
     Plan* sub = nullptr;     // OP_SUBPLAN
     uint64_t sub_ws_off = 0; // child workspace region
