@@ -307,7 +307,7 @@ const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ld
     return nullptr;
 }
 
-static uint32_t cu_count() {
+uint32_t device_cu_count() {
     static uint32_t cus = 0;
     if (!cus) {
         int dev = 0, n = 0;
@@ -340,7 +340,7 @@ hipError_t launch_reg(const RegVariant* v, const LineArgs& a, hipStream_t s) {
     hipError_t e = ensure_lds_attr(v->fn, v->lds_bytes);
     if (e != hipSuccess) return e;
     uint32_t blocks = (a.lines + v->lines - 1) / v->lines;
-    if (v->persist) blocks = std::min<uint32_t>(blocks, cu_count() * resident_per_cu(v));
+    if (v->persist) blocks = std::min<uint32_t>(blocks, device_cu_count() * resident_per_cu(v));
     LineArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchKernel(v->fn, dim3(blocks), dim3(v->nthreads), args, v->lds_bytes, s);

@@ -42,6 +42,9 @@ struct RegVariant {
     int adj = 0;
 };
 
+// Compute units of the current device (256 when it cannot be queried).
+uint32_t device_cu_count();
+
 // Registry lookup; nullptr when no compiled variant matches.
 const RegVariant* find_reg_variant(int prec, int n, int ld_lf, int st_lf, int ldm, int stm,
                                    int post, int vec, int fold = 0);

@@ -569,6 +569,27 @@ struct Builder {
         return a;
     }
 
+    // Few lines: the default shape of a pass (the first registered variant,
+    // measured on arrays of 64 MiB and more) can leave most CUs idle (2048
+    // lines of 512 points in 64 workgroups); then the variant of the same
+    // shape and radix schedule with the most workgroups, up to two per CU,
+    // runs instead.  The schedule fixes a line's arithmetic, so a line's
+    // result does not depend on how many lines its plan has.
+    const RegVariant* filling_variant(const RegVariant* rv, uint64_t n, uint64_t lines, bool ld_lf, bool st_lf,
+                                      int ldm, int stm, int post, bool pairable) {
+        const uint64_t want = 2ull * device_cu_count();
+        auto wgs = [&](const RegVariant* v) { return std::min<uint64_t>(want, (lines + v->lines - 1) / v->lines); };
+        if (wgs(rv) >= want) return rv;
+        const RegVariant* best = rv;
+        for (const RegVariant* v : find_reg_variants(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post)) {
+            if (v->adj || v->persist || v->fold || (v->vec == 2 && !pairable) || v->lds_bytes > P.lds_budget ||
+                std::strcmp(v->radices, rv->radices) != 0)
+                continue;
+            if (wgs(v) > wgs(best)) best = v;
+        }
+        return best;
+    }
+
     // One line kernel (register variant when compiled, else the LDS kernel).
     // Returns false if the line does not fit on chip.
     bool line_op(uint64_t n, const LineSet& ls, const Layout& in, const Layout& out, int src,
@@ -640,6 +661,7 @@ struct Builder {
         } else if (!tuned) {
             rv = pairable ? find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 2) : nullptr;
             if (!rv) rv = find_reg_variant(P.prec, (int)n, ld_lf, st_lf, ldm, stm, post, 1);
+            if (rv) rv = filling_variant(rv, n, ls.lines, ld_lf, st_lf, ldm, stm, post, pairable);
         }
         const size_t es = elem_bytes(P.arith);
         const uint64_t lstr = lds_line_stride((uint32_t)std::min<uint64_t>(n, 1u << 30));
@@ -710,10 +732,16 @@ struct Builder {
         return f;
     }
 
-    bool conv_fits(uint64_t m, uint32_t* lpw_out, bool lf) {
+    // Lines per workgroup of a fused convolution of length m: the most that
+    // fit LDS up to the preferred count, fewer when the axis has too few
+    // lines to give every CU two workgroups (17 lines of a 17 x 17 volume ran
+    // in one workgroup).
+    bool conv_fits(uint64_t m, uint32_t* lpw_out, bool lf, uint64_t lines) {
         const uint32_t want = lf ? (P.arith == P_F32 ? 16u : 8u) : (uint32_t)std::max<uint64_t>(1, 2048 / m);
         for (uint32_t lpw = want; lpw >= 1; lpw = lpw / 2) {
             if (conv_kernel_bytes(P.arith, (uint32_t)m, lpw) <= P.lds_budget) {
+                const uint64_t fill = 2ull * device_cu_count();
+                while (lpw > 1 && (lines + lpw - 1) / lpw < fill) lpw /= 2;
                 *lpw_out = lpw;
                 return true;
             }
@@ -1352,7 +1380,7 @@ struct Builder {
         if (kind == MIFFT_AXIS_RADER) {
             const uint64_t m = rader_conv_len(n);
             uint32_t lpw = 0;
-            if (conv_fits(m, &lpw, lf)) {
+            if (conv_fits(m, &lpw, lf, ls.lines)) {
                 rader_op(n, m, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale);
                 return 0;
             }
@@ -1360,7 +1388,7 @@ struct Builder {
         }
         const uint64_t m = bluestein_conv_len(n);
         uint32_t lpw = 0;
-        if (conv_fits(m, &lpw, lf)) {
+        if (conv_fits(m, &lpw, lf, ls.lines)) {
             bluestein_fused_op(n, m, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale);
             return 0;
         }
