@@ -1,7 +1,9 @@
 """GPU parity of the small-volume route (SmallVolumePlan,
 src/runtime/small_volume.rs:65-275): every axis of an f32 volume of at most
-4096 points in one workgroup, against the FP64 oracle; the route is taken only
-where the reference takes it and switched off by fuse_small_volumes."""
+4096 points in one workgroup, prime axes up to direct_max_prime as direct
+DFTs (small_volume.rs:9-10, 220-245), against the FP64 oracle; the route is
+taken only where the reference takes it and switched off by
+fuse_small_volumes."""
 import numpy as np
 import pytest
 
@@ -12,7 +14,9 @@ from wgpufft_amd import FftConfig, FftDirection, FftPlan, FftPrecision, FftTunin
 pytestmark = pytest.mark.gpu
 
 SHAPES = [([16, 16], 1), ([32, 64], 3), ([8, 8, 8], 2), ([16, 16, 16], 1), ([32, 128], 5), ([12, 10, 9], 7),
-          ([2, 2, 2, 2, 2, 2, 2, 2], 9), ([120, 9], 4), ([4, 1024], 2), ([13, 11, 7], 3)]
+          ([2, 2, 2, 2, 2, 2, 2, 2], 9), ([120, 9], 4), ([4, 1024], 2), ([13, 11, 7], 3),
+          # prime axes (direct DFTs inside the workgroup)
+          ([17, 17], 3), ([17, 16], 2), ([31, 8, 5], 4), ([101, 11], 1), ([3, 127], 5), ([59, 47], 2)]
 
 
 def run(cfg, x, in_place=False):
@@ -62,9 +66,12 @@ def test_small_volume_route_selection(torch_cuda):
     assert not p_off.describe().startswith("small-volume") and p_on.describe().startswith("small-volume")
     ref = c2c_expected(x, [32, 64])
     assert oracle.rel_l2(y_off, ref) <= 1e-6 and oracle.rel_l2(y_on, ref) <= 1e-6
-    # not taken: more than 4096 points, a prime axis, a subset of axes, f64,
-    # 2D with both sides >= 64
-    for cfg in (FftConfig.new_nd([128, 64]), FftConfig.new_nd([17, 16]),
+    # not taken: more than 4096 points, a prime axis above direct_max_prime
+    # or with Rader forced, a subset of axes, f64, 2D with both sides >= 64
+    for cfg in (FftConfig.new_nd([128, 64]), FftConfig.new_nd([131, 16]),
+                FftConfig.new_nd([17, 16]).with_tuning(FftTuning().with_force_rader_axes([0])),
+                FftConfig.new_nd([17, 16]).with_tuning(FftTuning().with_direct_max_prime(13)),
                 FftConfig.new_nd([16, 16]).with_axes([1]),
                 FftConfig.new_nd([16, 16]).with_precision(FftPrecision.F64), FftConfig.new_nd([64, 64])):
-        assert not FftPlan.c2c(cfg).describe().startswith("small-volume")
+        assert not FftPlan.c2c(cfg).describe().startswith("small-volume"), cfg
+    assert FftPlan.c2c(FftConfig.new_nd([17, 16])).describe().startswith("small-volume")

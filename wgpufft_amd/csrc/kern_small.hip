@@ -2,8 +2,8 @@
 // one workgroup (SmallVolumePlan, src/runtime/small_volume.rs:65-130,215-275).
 // A workgroup loads its volume into LDS, and for each axis gathers the axis
 // lines into a contiguous line buffer, runs the runtime-schedule Stockham
-// stages there (ping-pong in LDS) and scatters them back; then it stores the
-// volume once.  One launch and one HBM round trip instead of one per axis.
+// stages there (ping-pong in LDS), or a prime axis's direct DFT, and scatters
+// them back; then it stores the volume once.  One launch and one HBM round trip instead of one per axis.
 #include "launch.h"
 
 namespace mifft {
@@ -80,7 +80,29 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
             bufA[l * lstr + lpad(p)] = vol[o * S * n + i + p * S];
         }
         __syncthreads();
-        const cpx<float>* res = lds_fft_lines<float>(bufA, bufB, lines, sc, twl + two);
+        const cpx<float>* res;
+        if (sc.ns == 0 && n > 1) {
+            // a prime axis up to direct_max_prime: the direct DFT of each line,
+            // X[k] = sum_p x[p] W_n^{pk}, one output per thread, W_n^t (t < n)
+            // from LDS (small_volume.rs:220-245)
+            const cpx<float>* wt = twl + two;
+            for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
+                const uint32_t l = fdiv(t, dn), k = t - l * n;
+                const cpx<float>* ln = bufA + l * lstr;
+                cpx<float> acc = mk<float>(0.f, 0.f);
+                uint32_t e = 0;
+                for (uint32_t q = 0; q < n; q++) {
+                    acc = acc + cmul(ln[lpad(q)], wt[e]);
+                    e += k;
+                    e = e >= n ? e - n : e;
+                }
+                bufB[l * lstr + lpad(k)] = acc;
+            }
+            __syncthreads();
+            res = bufB;
+        } else {
+            res = lds_fft_lines<float>(bufA, bufB, lines, sc, twl + two);
+        }
         two += v.tw_len[ax];
         for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
             uint32_t l, p;

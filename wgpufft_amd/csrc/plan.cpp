@@ -970,9 +970,9 @@ struct Builder {
 
     // small_volume_key (src/runtime/small_volume.rs:215-275): an f32
     // transform of every axis of a volume of at most 4096 points whose axis
-    // lengths are {2,3,5,7,11,13}-smooth (the reference also takes direct
-    // prime axes; those lengths keep the per-axis route here) runs as one
-    // workgroup per volume.  Not in batched plans (the batched kernel has no
+    // lengths are {2,3,5,7,11,13}-smooth or primes up to direct_max_prime
+    // (as direct DFTs, unless Rader is forced on the axis; small_volume.rs:
+    // 9-10, 220-245) runs as one workgroup per volume.  Not in batched plans (the batched kernel has no
     // class for it), nor for 2D volumes with both sides of 64 points or more:
     // measured on MI355X, 64x64 runs 1.7x faster as two register-kernel
     // passes, while 16x16 ... 32x128, 4x1024, 16^3 and 12x10x9 run 1.1-2x
@@ -982,8 +982,13 @@ struct Builder {
             !cfg.tuning.fuse_small_volumes || elems > 4096)
             return false;
         if (cfg.rank == 2 && std::min(cfg.shape[0], cfg.shape[1]) >= 64) return false;
-        for (const AxisInfo& ai : P.axes)
-            if (ai.kind != MIFFT_AXIS_MIXED) return false;
+        const mifft_tuning& tun = cfg.tuning;
+        auto direct_axis = [&](int a) {
+            return P.axes[a].kind == MIFFT_AXIS_RADER && cfg.shape[a] <= tun.direct_max_prime &&
+                   !rader_forced(tun, a);
+        };
+        for (int a = 0; a < cfg.rank; a++)
+            if (P.axes[a].kind != MIFFT_AXIS_MIXED && !direct_axis(a)) return false;
         SmallVolArgs v;
         std::memset(&v, 0, sizeof(v));
         v.elems = (uint32_t)elems;
@@ -993,12 +998,25 @@ struct Builder {
         uint32_t S = 1;
         for (int a = 0; a < cfg.rank; a++) {
             const uint32_t n = (uint32_t)cfg.shape[a];
-            Sched sc = make_sched(n, 1, false, false);
+            Sched sc;
             v.S[a] = S;
             v.dS[a] = make_magic(S);
             v.dL[a] = make_magic((uint32_t)(elems / n));
-            tabs.push_back(stage_twiddles(P_F32, sched_radices(sc)));
-            v.tw_len[a] = (uint32_t)stage_twiddle_count(sched_radices(sc).data(), (int)sc.ns);
+            if (direct_axis(a)) {
+                // no stages (ns = 0): the kernel runs the direct DFT with W_n^t, t < n
+                std::memset(&sc, 0, sizeof(sc));
+                sc.n = n;
+                sc.dn = make_magic(n);
+                void* wt = alloc(n * sizeof(cpx<float>),
+                                 "roots prec=f32 n=" + std::to_string(n) + " count=" + std::to_string(n));
+                hip_check(gen_twiddles(P_F32, wt, n, n, s), "direct twiddles");
+                tabs.push_back(wt);
+                v.tw_len[a] = n;
+            } else {
+                sc = make_sched(n, 1, false, false);
+                tabs.push_back(stage_twiddles(P_F32, sched_radices(sc)));
+                v.tw_len[a] = (uint32_t)stage_twiddle_count(sched_radices(sc).data(), (int)sc.ns);
+            }
             v.tw_total += v.tw_len[a];
             v.buf_elems = std::max<uint32_t>(v.buf_elems, (uint32_t)(elems / n) * lds_line_stride(n));
             scs.push_back(sc);
