@@ -138,6 +138,21 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 128, 8, 64, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 2, "16x8", 16, 8),
     F32_POW2(1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, 4, 8, 16, 16, 16, 32, 64),
     F32_POW2(0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, 4, 8, 16, 16, 16, 32, 64),
+    // the same passes with a quarter and a half of the lines per workgroup:
+    // up to 512 workgroups for the 4096- to 16384-line passes of 2^19..2^21
+    // points (64 threads per workgroup at a quarter)
+    REGV(FmtF32, P_F32, 256, 16, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 256, 16, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 128, 8, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 64, 4, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 64, 4, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 256, 16, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 256, 16, 8, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 8, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 128, 8, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 64, 4, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 64, 4, 32, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
     REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "15x8", 15, 8),
     REGV(FmtF32, P_F32, 343, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),
