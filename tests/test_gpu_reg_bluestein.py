@@ -41,7 +41,7 @@ CASES = [
     (1000, 3, True, 2048),
     (1229, 5, False, 4096),  # prime, 1228 = 4 * 307
     (3256, 3, False, 8192),  # 8 * 11 * 37: Bluestein (the reference's 1 GiB case)
-    (4093, 2, False, 8192),  # prime, 4092 = 4 * 3 * 11 * 31
+    (4093, 2, True, 8192),   # prime, 4092 = 4 * 3 * 11 * 31: medium-prime Rader unless forced
 ]
 
 

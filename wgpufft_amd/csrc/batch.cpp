@@ -222,7 +222,8 @@ void batch_build(const BatchEntry* entries, int32_t count, char* ws_base, BatchB
                 case OP_RADER:
                 case OP_BLUESTEIN:
                     if (f64) cls = op.kind == OP_RADER ? BC_RADER_F64 : BC_BLUESTEIN_F64;
-                    else cls = op.kind == OP_RADER ? BC_RADER : BC_BLUESTEIN;
+                    else if (op.kind == OP_RADER) cls = sched_has_medium_prime(op.conv.sc) ? BC_RADER_MED : BC_RADER;
+                    else cls = BC_BLUESTEIN;
                     d.conv = op.conv;
                     lpw = op.conv.sc.lpw;
                     op_lds = conv_kernel_bytes(op.prec, op.conv.m, op.conv.sc.lpw);

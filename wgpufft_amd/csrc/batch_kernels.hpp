@@ -58,6 +58,8 @@ enum BatchClass : uint32_t {
     BC_REGB_2048,
     BC_REGB_4096,
     BC_REGB_8192,
+    // fused Rader convolution with medium prime radices (17..61), f32
+    BC_RADER_MED,
     BC_COUNT
 };
 

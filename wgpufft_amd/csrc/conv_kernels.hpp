@@ -24,8 +24,24 @@
 
 namespace mifft {
 
-template <class Fmt>
-__device__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t g0) {
+// Rader's filter step on the forward spectra in LDS: keeps bin 0 (the sum
+// of the permuted inputs) in s0 and leaves swap(A[k] F[k]) for the inverse.
+template <class T>
+__device__ __forceinline__ void rader_filter(const ConvArgs& c, cpx<T>* r, cpx<T>* s0, uint32_t total,
+                                             uint32_t lstr) {
+    const uint32_t M = c.m;
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+        const uint32_t l = fdiv(e, c.sc.dn);  // sc.n == M
+        const uint32_t k = e - l * M;
+        cpx<T> v = r[l * lstr + lpad(k)];
+        if (k == 0) s0[l] = v;
+        r[l * lstr + lpad(k)] = swap_ri(cmul(v, ld_tw<T>(c.filt, k)));
+    }
+}
+
+// MED: the schedule holds medium prime radices (17..61), f32 only.
+template <class Fmt, bool MED = false>
+__device__ __forceinline__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t g0) {
     using T = typename Fmt::T;
     const Sched& sc = c.sc;
     const LineArgs& a = c.io;
@@ -35,7 +51,6 @@ __device__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t 
     cpx<T>* bufB = bufA + lpw * lstr;
     cpx<T>* x0 = bufB + lpw * lstr;  // lpw entries
     cpx<T>* s0 = x0 + lpw;           // lpw entries
-    const DivMagic dM = sc.dn;  // sc.n == M
     // gather a[q] = x[g^q] (zero-padded to M), x0 = x[0]; lines past the end
     // read the last line and are never stored
     const uint32_t total = lpw * M;
@@ -63,17 +78,31 @@ __device__ void rader_tile(const ConvArgs& c, unsigned char* smem_raw, uint32_t 
         }
     }
     __syncthreads();
-    cpx<T>* r = lds_fft_lines<T>(bufA, bufB, lpw, sc, a.tw);
-    cpx<T>* o = (r == bufA) ? bufB : bufA;
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-        const uint32_t l = fdiv(e, dM);
-        const uint32_t k = e - l * M;
-        cpx<T> v = r[l * lstr + lpad(k)];
-        if (k == 0) s0[l] = v;
-        r[l * lstr + lpad(k)] = swap_ri(cmul(v, ld_tw<T>(c.filt, k)));
+    cpx<T>* r2;
+    if constexpr (MED) {
+        // Both transforms through one inlined copy of the stages (two copies
+        // measured 256 VGPRs with scratch, one 176 without).
+        cpx<T>* cur = bufA;
+        cpx<T>* oth = bufB;
+#pragma nounroll
+        for (int pass = 0;; pass++) {
+            cpx<T>* r = lds_fft_lines_inl<T, true>(cur, oth, lpw, sc, a.tw);
+            if (pass == 1) {
+                r2 = r;
+                break;
+            }
+            rader_filter<T>(c, r, s0, total, lstr);
+            __syncthreads();
+            oth = (r == bufA) ? bufB : bufA;
+            cur = r;
+        }
+    } else {
+        cpx<T>* r = lds_fft_lines<T>(bufA, bufB, lpw, sc, a.tw);
+        cpx<T>* o = (r == bufA) ? bufB : bufA;
+        rader_filter<T>(c, r, s0, total, lstr);
+        __syncthreads();
+        r2 = lds_fft_lines<T>(r, o, lpw, sc, a.tw);
     }
-    __syncthreads();
-    cpx<T>* r2 = lds_fft_lines<T>(r, o, lpw, sc, a.tw);
     // scatter X[g^-u] = x0 + c[u], X[0] = x0 + sum a
     for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
         const LineSlot m = line_slot(s, lpw, tpl, sc.st_lf);

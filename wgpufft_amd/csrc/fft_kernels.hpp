@@ -742,6 +742,14 @@ struct Sched {
     DivMagic dl[kMaxStages];     // divide by L
 };
 
+// Whether a schedule holds a medium prime radix (17..61), which only the
+// kernels instantiated for them run.
+__host__ __device__ inline bool sched_has_medium_prime(const Sched& sc) {
+    for (uint32_t s = 0; s < sc.ns; s++)
+        if (sc.rad[s] > 16) return true;
+    return false;
+}
+
 // LDS line stride (elements) of the runtime-schedule kernels.
 __host__ __device__ constexpr uint32_t lds_line_stride(uint32_t n) { return n + (n >> 4) + 1; }
 __device__ __forceinline__ uint32_t lpad(uint32_t p) { return p + (p >> 4); }
@@ -774,12 +782,90 @@ __device__ __forceinline__ void lds_stage(const cpx<T>* src, cpx<T>* dst, uint32
     }
 }
 
+// Output pair q, R - q of a symmetric-pair prime DFT (as DftOdd) with the
+// constants W_R^(m q) resolved at compile time, stored to LDS at stride L.
+template <int R, int q, int m, class T>
+__device__ __forceinline__ void prime_acc(cpx<T>& A, cpx<T>& B, const cpx<T>* ap, const cpx<T>* am) {
+    if constexpr (m <= (R - 1) / 2) {
+        constexpr int e = (m * q) % R;
+        constexpr T c = (T)root_cos<R>(e);
+        constexpr T sn = (T)root_sin<R>(e);
+        A.x += c * ap[m - 1].x;
+        A.y += c * ap[m - 1].y;
+        B.x += sn * am[m - 1].x;
+        B.y += sn * am[m - 1].y;
+        prime_acc<R, q, m + 1, T>(A, B, ap, am);
+    }
+}
+template <int R, int q, class T>
+__device__ __forceinline__ void prime_out(cpx<T>* d, uint32_t ob, uint32_t L, cpx<T> v0, const cpx<T>* ap,
+                                          const cpx<T>* am) {
+    if constexpr (q <= (R - 1) / 2) {
+        cpx<T> A = v0, B = mk<T>(T(0), T(0));
+        prime_acc<R, q, 1, T>(A, B, ap, am);
+        d[lpad(ob + q * L)] = mk<T>(A.x + B.y, A.y - B.x);
+        d[lpad(ob + (R - q) * L)] = mk<T>(A.x - B.y, A.y + B.x);
+        prime_out<R, q + 1, T>(d, ob, L, v0, ap, am);
+    }
+}
+
+// Inputs m and R - m of a prime butterfly (twiddled when L > 1) as their
+// sum and difference.
+template <int R, int m, class T>
+__device__ __forceinline__ void prime_in(cpx<T>* ap, cpx<T>* am, cpx<T>& sum, const cpx<T>* s, uint32_t j,
+                                         uint32_t nr, uint32_t L, uint32_t k, uint32_t twoff, const void* tw) {
+    if constexpr (m <= (R - 1) / 2) {
+        cpx<T> x = s[lpad(j + m * nr)], y = s[lpad(j + (R - m) * nr)];
+        if (L > 1) {
+            x = cmul(x, ld_tw<T>(tw, twoff + (m - 1) * L + k));
+            y = cmul(y, ld_tw<T>(tw, twoff + (R - m - 1) * L + k));
+        }
+        ap[m - 1] = x + y;
+        am[m - 1] = x - y;
+        sum = sum + ap[m - 1];
+        prime_in<R, m + 1, T>(ap, am, sum, s, j, nr, L, k, twoff, tw);
+    }
+}
+
+// One Stockham stage of a medium prime radix R (17..61) over `nl` lines in
+// LDS.  The butterfly is the symmetric-pair DFT of DftOdd, written so that
+// only the pair sums and differences stay in registers: inputs j and R - j
+// are loaded (and twiddled) together, and each output pair k, R - k goes to
+// LDS as soon as it is formed.
+template <int R, class T>
+__device__ __forceinline__ void lds_stage_prime(const cpx<T>* src, cpx<T>* dst, uint32_t nl, uint32_t n,
+                                                uint32_t lstr, uint32_t L, DivMagic dnr, DivMagic dl,
+                                                uint32_t twoff, const void* tw) {
+    constexpr int H = (R - 1) / 2;
+    const uint32_t nr = n / R;
+    const uint32_t total = nl * nr;
+#pragma nounroll
+    for (uint32_t b = threadIdx.x; b < total; b += blockDim.x) {
+        const uint32_t l = fdiv(b, dnr);
+        const uint32_t j = b - l * nr;
+        const uint32_t kq = fdiv(j, dl);
+        const uint32_t k = j - kq * L;
+        const cpx<T>* s = src + l * lstr;
+        const cpx<T> v0 = s[lpad(j)];
+        cpx<T> ap[H], am[H];
+        cpx<T> sum = v0;
+        prime_in<R, 1, T>(ap, am, sum, s, j, nr, L, k, twoff, tw);
+        cpx<T>* d = dst + l * lstr;
+        const uint32_t ob = kq * L * R + k;
+        d[lpad(ob)] = sum;
+        prime_out<R, 1, T>(d, ob, L, v0, ap, am);
+    }
+}
+
 // Transforms nl lines held in LDS buffer `a` (line stride lstr) using `b` as
 // ping-pong space; returns the buffer holding the result.  Every thread of the
-// block must call it (it synchronises after each stage).
-template <class T>
-__device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched& sc,
-                                 const void* tw) {
+// block must call it (it synchronises after each stage).  MED adds the
+// medium prime radices 17..61 (Rader convolutions over such p - 1) to the
+// stage switch; it is a separate instantiation so that the register demand
+// of those stages does not reach the other kernels.
+template <class T, bool MED>
+__device__ __forceinline__ cpx<T>* lds_fft_lines_inl(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched& sc,
+                                                     const void* tw) {
     const uint32_t lstr = lds_line_stride(sc.n);
     cpx<T>* src = a;
     cpx<T>* dst = b;
@@ -806,6 +892,25 @@ __device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched&
             MIFFT_CASE(15)
             MIFFT_CASE(16)
 #undef MIFFT_CASE
+            // medium primes (MED only)
+#define MIFFT_PCASE(RR)                                                                                     \
+    case RR:                                                                                                \
+        if constexpr (MED)                                                                                  \
+            lds_stage_prime<RR, T>(src, dst, nl, sc.n, lstr, sc.L[s], sc.dnr[s], sc.dl[s], sc.twoff[s], tw); \
+        break;
+            MIFFT_PCASE(17)
+            MIFFT_PCASE(19)
+            MIFFT_PCASE(23)
+            MIFFT_PCASE(29)
+            MIFFT_PCASE(31)
+            MIFFT_PCASE(37)
+            MIFFT_PCASE(41)
+            MIFFT_PCASE(43)
+            MIFFT_PCASE(47)
+            MIFFT_PCASE(53)
+            MIFFT_PCASE(59)
+            MIFFT_PCASE(61)
+#undef MIFFT_PCASE
             default:
                 break;
         }
@@ -815,6 +920,16 @@ __device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched&
         dst = t;
     }
     return src;
+}
+
+// The same as a call (the schedule's own kernels), or inlined for MED: as a
+// called function the medium prime stages see a divergent radix and a
+// caller-independent register budget and measured 256 VGPRs with scratch;
+// inlined into the kernel, 175 without.
+template <class T, bool MED = false>
+__device__ cpx<T>* lds_fft_lines(cpx<T>* a, cpx<T>* b, uint32_t nl, const Sched& sc, const void* tw) {
+    if constexpr (MED) return lds_fft_lines_inl<T, true>(a, b, nl, sc, tw);
+    else return lds_fft_lines_inl<T, false>(a, b, nl, sc, tw);
 }
 
 // Moving `lpw` lines of `len` points between memory and LDS: slot s of

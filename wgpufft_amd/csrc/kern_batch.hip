@@ -69,10 +69,10 @@ __device__ __noinline__ uint32_t lds_tiles(MIFFT_TILE_ARGS) {
         lds_fft_tile<Fmt, LDM, STM, POST>(d.a, d.sc, smem, g0);
     });
 }
-template <class Fmt>
+template <class Fmt, bool MED = false>
 __device__ __noinline__ uint32_t rader_tiles(MIFFT_TILE_ARGS) {
     return tile_run(tiles, ntiles, counter, ops, t,
-                    [&](const OpDesc& d, uint32_t g0) { rader_tile<Fmt>(d.conv, smem, g0); });
+                    [&](const OpDesc& d, uint32_t g0) { rader_tile<Fmt, MED>(d.conv, smem, g0); });
 }
 template <class K>
 __device__ __noinline__ uint32_t regb_tiles(MIFFT_TILE_ARGS) {
@@ -128,6 +128,7 @@ __global__ void __launch_bounds__(256, 4) k_batch(const BatchTile* __restrict__ 
             case BC_RADER: t = rader_tiles<FmtF32>(MIFFT_TILE_PASS); break;
             case BC_BLUESTEIN: t = bluestein_tiles<FmtF32>(MIFFT_TILE_PASS); break;
             case BC_RADER_F64: t = rader_tiles<FmtF64>(MIFFT_TILE_PASS); break;
+            case BC_RADER_MED: t = rader_tiles<FmtF32, true>(MIFFT_TILE_PASS); break;
             case BC_BLUESTEIN_F64: t = bluestein_tiles<FmtF64>(MIFFT_TILE_PASS); break;
             case BC_REG_C256_F64:
                 t = reg_tiles<RegFft<FmtF64, 256, 16, 16, false, false, LD_COMPLEX, ST_COMPLEX, false,

@@ -4,10 +4,10 @@
 
 namespace mifft {
 
-template <class Fmt>
+template <class Fmt, bool MED = false>
 __global__ void __launch_bounds__(256) k_rader(ConvArgs c) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    rader_tile<Fmt>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
+    rader_tile<Fmt, MED>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
 }
 
 template <class Fmt>
@@ -33,7 +33,9 @@ static hipError_t launch_conv(K kern, int prec, const ConvArgs& c, hipStream_t s
 
 hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s) {
     switch (prec) {
-        case P_F32: return launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
+        case P_F32:
+            return sched_has_medium_prime(c.sc) ? launch_conv<FmtF32>(k_rader<FmtF32, true>, prec, c, s)
+                                                : launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
         case P_F64: return launch_conv<FmtF64>(k_rader<FmtF64>, prec, c, s);
         default: return launch_conv<FmtDF64>(k_rader<FmtDF64>, prec, c, s);
     }

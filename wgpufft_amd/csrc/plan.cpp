@@ -365,10 +365,41 @@ static std::vector<int> lds_schedule(uint64_t n) {
     return r;
 }
 
-static Sched make_sched(uint64_t n, uint32_t lpw, bool ld_lf, bool st_lf) {
+// Medium prime radices of the fused Rader convolution
+// (FUSED_PRIME_RADICES, src/runtime/axis_plan.rs:2392).
+static const int kMediumPrimes[] = {17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
+// Fewest butterflies of the largest medium prime per line
+// (MIN_MEDIUM_PRIME_BUTTERFLIES, src/runtime/rader_axis.rs:1185-1192).
+constexpr uint64_t kMinMediumPrimeButterflies = 36;
+
+// Schedule of the cyclic Rader convolution over p - 1 points when p - 1 has
+// prime factors from 17 to 61 besides {2,3,5,7,11,13}
+// (medium_prime_rader_schedule, src/runtime/rader_axis.rs:1194-1225): those
+// primes' stages, largest first, then the LDS schedule of the smooth rest.
+// False when p - 1 is smooth, has a larger prime factor, or holds fewer than
+// 36 butterflies of its largest medium prime.
+bool medium_rader_schedule(uint64_t p, std::vector<int>* out) {
+    if (p < 3) return false;
+    const uint64_t l = p - 1;
+    if (l < 2 || is_smooth(l)) return false;
+    uint64_t rest = l;
+    std::vector<int> r;
+    for (int i = (int)(sizeof(kMediumPrimes) / sizeof(int)) - 1; i >= 0; i--)
+        while (rest % (uint64_t)kMediumPrimes[i] == 0) {
+            r.push_back(kMediumPrimes[i]);
+            rest /= (uint64_t)kMediumPrimes[i];
+        }
+    if (r.empty() || l / (uint64_t)r[0] < kMinMediumPrimeButterflies || !is_smooth(rest)) return false;
+    if (rest > 1)
+        for (int x : lds_schedule(rest)) r.push_back(x);
+    if (out) *out = r;
+    return true;
+}
+
+static Sched make_sched(uint64_t n, uint32_t lpw, bool ld_lf, bool st_lf, const std::vector<int>* radices = nullptr) {
     Sched sc;
     std::memset(&sc, 0, sizeof(sc));
-    std::vector<int> rad = lds_schedule(n);
+    const std::vector<int> rad = radices ? *radices : lds_schedule(n);
     if (rad.size() > (size_t)kMaxStages) fail(MIFFT_ERR_UNSUPPORTED_LENGTH, "too many stages");
     sc.n = (uint32_t)n;
     sc.ns = (uint32_t)rad.size();
@@ -752,7 +783,7 @@ struct Builder {
 
     void rader_op(uint64_t p, uint64_t m, uint32_t lpw, const LineSet& ls, const Layout& in,
                   const Layout& out, int src, uint64_t src_off, int dst, uint64_t dst_off,
-                  bool conj_in, bool conj_out, double scale) {
+                  bool conj_in, bool conj_out, double scale, const std::vector<int>* radices = nullptr) {
         const uint64_t g = primitive_root(p);
         const uint64_t ginv = mod_pow_u64(g, p - 2, p);
         std::vector<uint32_t> pin(p - 1), pout(p - 1);
@@ -780,7 +811,7 @@ struct Builder {
         op.dst_off = dst_off;
         op.prec = P.prec;
         op.conv.io = base_args(ls, in, out, conj_in, conj_out, scale);
-        op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out));
+        op.conv.sc = make_sched(m, lpw, adjacent_lines(ls, in), adjacent_lines(ls, out), radices);
         op.conv.io.tw = stage_twiddles(P.arith, sched_radices(op.conv.sc));
         op.conv.filt = filt;
         op.conv.perm_in = dpin;
@@ -1062,11 +1093,27 @@ struct Builder {
             if (t.force_rader_axes[i] == axis) return true;
         return false;
     }
+    // Whether the Rader axis of the prime p convolves cyclically over p - 1
+    // points with medium prime radices (f32, the fused kernel fits on chip;
+    // rader_axis_convolution, src/runtime/rader_axis.rs:1227-1250).
+    bool medium_rader(uint64_t p, const LineSet& ls, const Layout& lay, std::vector<int>* radices,
+                      uint32_t* lpw) {
+        const uint64_t linear = 2 * (p - 1) - 1;
+        return P.prec == P_F32 && medium_rader_schedule(p, radices) &&
+               std::max<uint64_t>(linear, p - 1) >= P.cfg.tuning.fused_min_convolution_length &&
+               conv_fits(p - 1, lpw, adjacent_lines(ls, lay), ls.lines);
+    }
     bool direct_prime_op(uint64_t p, const LineSet& ls, const Layout& lay, int src, uint64_t src_off, int dst,
                          uint64_t dst_off, bool inverse, double scale) {
         const mifft_tuning& t = P.cfg.tuning;
         if (P.batch_mode || P.prec != P_F32 || p < 3 || p > t.direct_max_prime || rader_conv_len(p) == p - 1)
             return false;
+        {
+            // a cyclic fused convolution takes priority (c2c.rs:244-253)
+            std::vector<int> med;
+            uint32_t ml = 0;
+            if (medium_rader(p, ls, lay, &med, &ml)) return false;
+        }
         for (int i = 0; i < t.n_force_rader_axes; i++)
             if (t.force_rader_axes[i] == cur_axis) return false;
         const bool lf = adjacent_lines(ls, lay);
@@ -1387,6 +1434,18 @@ struct Builder {
         }
         if (kind == MIFFT_AXIS_RADER && direct_prime_op(n, ls, lay, src, src_off, dst, dst_off, inverse, scale))
             return 0;
+        if (kind == MIFFT_AXIS_RADER) {
+            // p - 1 with prime factors from 17 to 61: the cyclic convolution
+            // over p - 1 points with those radices, ahead of a zero-padded or
+            // register Bluestein one (rader_prefers_register_bluestein,
+            // src/runtime/rader_axis.rs:1256-1268)
+            std::vector<int> med;
+            uint32_t lpw = 0;
+            if (medium_rader(n, ls, lay, &med, &lpw)) {
+                rader_op(n, n - 1, lpw, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale, &med);
+                return 0;
+            }
+        }
         // Bluestein axes, and Rader primes whose convolution would be
         // zero-padded (p - 1 not smooth) unless Rader is forced, on contiguous
         // lines: the register-resident Bluestein kernel (rader_axis.rs:1262-1280)

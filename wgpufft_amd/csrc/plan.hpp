@@ -182,6 +182,7 @@ uint64_t mod_pow_u64(uint64_t b, uint64_t e, uint64_t m);
 uint64_t primitive_root(uint64_t p);
 uint64_t next_smooth(uint64_t n);
 uint64_t rader_conv_len(uint64_t p);
+bool medium_rader_schedule(uint64_t p, std::vector<int>* out);
 uint64_t bluestein_conv_len(uint64_t n);
 void validate_config(const mifft_config& c);
 double config_scale(const mifft_config& c);
