@@ -1,11 +1,11 @@
 """GPU parity of the register-resident Bluestein kernel (the reference's
 register Bluestein, src/runtime/register_fft.rs:510-664, bluestein_axis.rs:
 937-969; Rader primes whose convolution would be zero-padded take it too,
-rader_axis.rs:1262-1280): f32 Bluestein lengths and such Rader primes on
-contiguous lines, every power-of-two convolution length 64..8192, ragged
-line counts, both directions, in place and out of place, against the oracle
-(relative L2 <= 1e-6, tests/gpu_accuracy.rs:94-104); forced Rader keeps the
-Rader route, and strided axes keep theirs."""
+rader_axis.rs:1262-1280): f32 Bluestein lengths and such Rader primes, every
+power-of-two convolution length 64..8192, ragged line counts, both
+directions, in place and out of place, against the oracle (relative L2 <=
+1e-6, tests/gpu_accuracy.rs:94-104); strided axes of 2D and 3D shapes; forced
+Rader keeps the Rader route."""
 import numpy as np
 import pytest
 
@@ -76,12 +76,37 @@ def test_register_bluestein_normalization_and_2d(torch_cuda):
     assert e <= 1e-6
 
 
-def test_forced_rader_and_strided_axes_keep_their_routes(torch_cuda):
+def test_forced_rader_keeps_its_route(torch_cuda):
     p1 = FftPlan.c2c(FftConfig.new(1229).with_batch(2).with_tuning(FftTuning().with_force_rader_axes([0])))
     assert "bluestein-reg" not in p1.describe() and "rader" in p1.describe(), p1.describe()
-    # a strided Bluestein axis (axis 1 of [8, 1229]) does not take the register kernel
-    p2 = FftPlan.c2c(FftConfig.new_nd([8, 1229]).with_axes([1]))
-    assert "bluestein-reg" not in p2.describe(), p2.describe()
     x = seeded_complex(78, 1229 * 2)
     _, y = run(FftConfig.new(1229).with_batch(2).with_tuning(FftTuning().with_force_rader_axes([0])), x)
     assert oracle.rel_l2(y, c2c_expected(x, [1229], batch=2)) <= 1e-6
+
+
+# strided axes (the register kernel reads a line's points a row pitch apart):
+# (shape, axes, batch, inverse, convolution length of the strided axis)
+STRIDED = [
+    ([8, 1229], [1], 3, False, 4096),
+    ([1229, 1229], None, 1, True, 4096),
+    ([5, 3256], [1], 2, False, 8192),
+    ([33, 100], [1], 1, True, 256),      # Bluestein 100 = 4 * 25 forced below
+    ([3, 263, 4], [1], 2, False, 1024),  # middle axis, stride 3
+]
+
+
+@pytest.mark.parametrize("shape,axes,batch,inverse,m", STRIDED)
+def test_register_bluestein_strided_axes(torch_cuda, shape, axes, batch, inverse, m):
+    cfg = (FftConfig.inverse_nd(shape) if inverse else FftConfig.new_nd(shape)).with_batch(batch)
+    if axes is not None:
+        cfg = cfg.with_axes(axes)
+    if shape == [33, 100]:
+        cfg = cfg.with_tuning(FftTuning().with_force_bluestein_axes([1]))
+    n = shape[1]
+    x = seeded_complex(990 + n + batch, int(np.prod(shape)) * batch)
+    plan, y = run(cfg, x, in_place=batch % 2 == 1)
+    assert f"bluestein-reg N={n} M={m} " in plan.describe(), plan.describe()
+    ref = c2c_expected(x, shape, axes=axes, batch=batch, inverse=inverse)
+    e = oracle.rel_l2(y, ref)
+    print(f"{shape} axes={axes} b={batch} inverse={inverse} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
+    assert e <= 1e-6

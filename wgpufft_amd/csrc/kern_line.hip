@@ -153,6 +153,23 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 128, 8, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
     REGV(FmtF32, P_F32, 64, 4, 16, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
     REGV(FmtF32, P_F32, 64, 4, 32, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    // ... and of the plain contiguous and strided axes of small volumes
+    // (128 x 128 x 64: 256-workgroup passes at the default widths) and of the
+    // 512-point columns of a 1024^2 fold
+    REGV(FmtF32, P_F32, 256, 16, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 256, 16, 8, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 8, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 128, 8, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 64, 4, 16, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 64, 4, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 512, 32, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 512, 32, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 256, 16, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 256, 16, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16", 16, 16),
+    REGV(FmtF32, P_F32, 128, 8, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 128, 8, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
+    REGV(FmtF32, P_F32, 64, 4, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    REGV(FmtF32, P_F32, 64, 4, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
     REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "15x8", 15, 8),
     REGV(FmtF32, P_F32, 343, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),

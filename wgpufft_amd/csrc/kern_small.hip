@@ -91,6 +91,9 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
                 const cpx<float>* ln = bufA + l * lstr;
                 cpx<float> acc = mk<float>(0.f, 0.f);
                 uint32_t e = 0;
+                // unrolled so that the LDS reads of several terms are in
+                // flight together (the sum's order is unchanged)
+#pragma unroll 4
                 for (uint32_t q = 0; q < n; q++) {
                     acc = acc + cmul(ln[lpad(q)], wt[e]);
                     e += k;
@@ -131,8 +134,9 @@ hipError_t launch_small_volume(const SmallVolArgs& v, uint32_t volumes, hipStrea
     const size_t lds = small_volume_lds_bytes(v);
     const hipError_t e = ensure_lds_attr((const void*)&k_small_volume, lds);
     if (e != hipSuccess) return e;
-    // about four points per thread, whole wavefronts, at most 1024 threads
-    uint32_t threads = ((v.elems / 4 + 63) / 64) * 64;
+    // about four points per thread (one per thread with a direct-DFT axis,
+    // whose outputs each take n terms), whole wavefronts, at most 1024 threads
+    uint32_t threads = ((v.elems / (v.direct ? 1 : 4) + 63) / 64) * 64;
     threads = threads < 64 ? 64 : threads > 1024 ? 1024 : threads;
     k_small_volume<<<volumes, threads, lds, s>>>(v);
     return hipGetLastError();

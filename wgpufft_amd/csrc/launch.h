@@ -145,6 +145,7 @@ struct SmallVolArgs {
     const Sched* sc;  // device array of rank schedules
     float scale;
     uint32_t conj_in, conj_out;
+    uint32_t direct;  // some axis runs a direct DFT
 };
 size_t small_volume_lds_bytes(const SmallVolArgs& v);
 hipError_t launch_small_volume(const SmallVolArgs& v, uint32_t volumes, hipStream_t s);

@@ -1043,6 +1043,7 @@ struct Builder {
                 hip_check(gen_twiddles(P_F32, wt, n, n, s), "direct twiddles");
                 tabs.push_back(wt);
                 v.tw_len[a] = n;
+                v.direct = 1;
             } else {
                 sc = make_sched(n, 1, false, false);
                 tabs.push_back(stage_twiddles(P_F32, sched_radices(sc)));
@@ -1447,11 +1448,16 @@ struct Builder {
             }
         }
         // Bluestein axes, and Rader primes whose convolution would be
-        // zero-padded (p - 1 not smooth) unless Rader is forced, on contiguous
-        // lines: the register-resident Bluestein kernel (rader_axis.rs:1262-1280)
+        // zero-padded (p - 1 not smooth) unless Rader is forced: the
+        // register-resident Bluestein kernel (c2c.rs:206-217,
+        // rader_axis.rs:1256-1268), on strided axes too.  A strided line's
+        // points are a row pitch apart, but the workgroups of neighbouring
+        // lines run together on one XCD and share each row segment through
+        // its L2 (1229^2 pair 254 -> 113 us against the fused Rader
+        // convolution with two lines per workgroup).
         const bool rader_padded = kind == MIFFT_AXIS_RADER && rader_conv_len(n) != n - 1 &&
                                   !(cur_axis >= 0 && rader_forced(P.cfg.tuning, cur_axis));
-        if ((kind == MIFFT_AXIS_BLUESTEIN || rader_padded) && !lf &&
+        if ((kind == MIFFT_AXIS_BLUESTEIN || rader_padded) &&
             reg_bluestein_op(n, ls, lay, lay, src, src_off, dst, dst_off, inverse, inverse, scale))
             return 0;
         if (kind == MIFFT_AXIS_RADER) {
