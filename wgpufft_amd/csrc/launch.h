@@ -75,6 +75,7 @@ struct ConvArgs {
     const uint32_t* perm_out; // Rader scatter g^-u mod p, u < p-1
     uint32_t n;               // transform length
     uint32_t m;               // convolution length
+    DivMagic dn;              // divide by n (Rader)
 };
 hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s);
 hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s);

@@ -818,6 +818,7 @@ struct Builder {
         op.conv.perm_out = dpout;
         op.conv.n = (uint32_t)p;
         op.conv.m = (uint32_t)m;
+        op.conv.dn = make_magic((uint32_t)p);
         std::ostringstream d;
         d << "rader p=" << p << " M=" << m << " " << sched_str(op.conv.sc) << " lines=" << ls.lines
           << " lpw=" << lpw;

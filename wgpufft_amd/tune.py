@@ -35,6 +35,15 @@ def workloads():
         ("3D C2C df64 [256,256,256] forward+inverse", pair([256, 256, 256], P.Df64)),
         ("1D C2C f64 [2^24] forward+inverse", pair([1 << 24], P.F64)),
         ("1D C2C df64 [2^24] forward+inverse", pair([1 << 24], P.Df64)),
+        # the reference's other published pair configurations (CHANGELOG.md)
+        ("ref: 2D C2C f32 [2048,2048] forward+inverse", pair([2048, 2048])),
+        ("ref: 2D C2C f32 [4096,4096] forward+inverse", pair([4096, 4096])),
+        ("ref: 2D C2C f32 [1024,1024] forward+inverse", pair([1024, 1024])),
+        ("ref: 3D C2C f32 [128,128,64] forward+inverse", pair([128, 128, 64])),
+        ("ref: 3D C2C f32 [256,256,128] forward+inverse", pair([256, 256, 128])),
+        ("ref: 1D C2C f32 [2^19] forward+inverse", pair([1 << 19])),
+        ("ref: 1D C2C f32 [2^20] forward+inverse", pair([1 << 20])),
+        ("ref: 1D C2C f32 [2^21] forward+inverse", pair([1 << 21])),
     ]
 
 
