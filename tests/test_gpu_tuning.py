@@ -163,6 +163,23 @@ def test_forced_split_matches_oracle(torch_cuda, n, batch, split):
         assert e <= 1e-6, (split, inverse, e)
 
 
+# Default split of long f32 axes without table entries: three passes for one
+# transform of 2^19 points and for 2^19 x 8 (2^22 points together), two
+# passes from 2^23 points together (2^19 x 16), split factors of at most 2048.
+@pytest.mark.parametrize("batch,passes", [(1, 3), (8, 3), (16, 2)])
+def test_default_split_by_batch(torch_cuda, batch, passes):
+    n = 1 << 19
+    plan = FftPlan.c2c(FftConfig.new(n).with_batch(batch))
+    desc = plan.describe()
+    assert desc.count("post") == passes - 1, desc
+    if batch == 16:
+        x = seeded_complex(19 + batch, n * batch)
+        _, y = _run(FftConfig.new(n).with_batch(batch), x)
+        e = oracle.rel_l2(y, c2c_expected(x, [n], batch=batch))
+        print(f"2^19 x {batch}: rel_l2={e:.3e} [{desc}]", flush=True)
+        assert e <= 1e-6
+
+
 # Persistent pipelined R2C / C2R axis-0 passes (real load + packed store,
 # Hermitian-extended load + real store), more rows than workgroups so that
 # each workgroup loops over several tiles, against the FP64 oracle.

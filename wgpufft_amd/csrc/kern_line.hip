@@ -170,6 +170,21 @@ static const RegVariant kRegistry[] = {
     REGV(FmtF32, P_F32, 128, 8, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x8", 16, 8),
     REGV(FmtF32, P_F32, 64, 4, 16, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
     REGV(FmtF32, P_F32, 64, 4, 32, 1, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x4", 16, 4),
+    // two-pass splits of 2^19..2^21 points (512- and 1024-point lines, 512 to
+    // 2048 of them): column passes with the twiddle on store and row passes
+    // with the transposed store at 1 to 8 lines per workgroup
+    REGV(FmtF32, P_F32, 1024, 64, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 512, 32, 2, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 512, 32, 4, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 512, 32, 8, 1, 1, LD_COMPLEX, ST_COMPLEX, 1, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 1024, 64, 1, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 1024, 64, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "16x16x4", 16, 16, 4),
+    REGV(FmtF32, P_F32, 512, 32, 2, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 512, 32, 4, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8", 8, 8, 8),
+    REGV(FmtF32, P_F32, 512, 32, 8, 0, 1, LD_COMPLEX, ST_COMPLEX, 0, 1, "8x8x8", 8, 8, 8),
     // smooth lengths of the mixed workload (schedules as wgpuFFT's fused_smooth_factors)
     REGV(FmtF32, P_F32, 120, 8, 32, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "15x8", 15, 8),
     REGV(FmtF32, P_F32, 343, 64, 4, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, 1, "7x7x7", 7, 7, 7),

@@ -512,15 +512,23 @@ static bool best_split3(int prec, uint64_t n, size_t budget, uint64_t* o3, uint6
 // Whether a long axis takes the three-pass split: by the tuning-table entry
 // "split|<prec>|n<n>" ("2" or "3") when present, else when the two-pass
 // split's longer factor fills more than 32 KiB per line or the axis has at
-// least 2^19 points.  Measured on MI355X (forward+inverse pairs, f32): three
-// passes win at 2^19-2^21 (1.1-1.4x: the two passes of 2^20 have 1024 lines
-// each, too few workgroups for 256 CUs) and 2^23-2^26 (1.0-3.7x), lose 13 %
-// at 2^22 (the MI355X table keeps two there); f64 / df64 2^24 1.4-1.5x.
-static bool prefer_three_pass(int prec, uint64_t n, uint64_t n1, uint64_t n2) {
+// least 2^19 points, except that f32 splits of at most 2048 x 2048 keep two
+// passes when the O transforms of the plan hold 2^23 points or more
+// together (every pass then has thousands of lines).  Measured on MI355X
+// (forward+inverse pairs, f32): one transform, three passes win at
+// 2^19-2^21 (1.1-1.4x: the two passes of 2^20 have 1024 lines each, too few
+// workgroups for 256 CUs) and 2^23-2^26 (1.0-3.7x), lose 13 % at 2^22 (the
+// MI355X table keeps two there); batched, two passes win at 2^23 points
+// and more (2^19 x 16 114.6 vs 170.8 us, 2^20 x 8 129.7 vs 161.4, 2^21 x 4
+// 135.9 vs 159.6, 2^22 x 2 134.2 vs 157.0) and lose below (2^19 x 8 95.3 vs
+// 87.8, 2^20 x 4 112.4 vs 84.9, 2^21 x 2 87.4 vs 85.8); f64 / df64 2^24
+// 1.4-1.5x.
+static bool prefer_three_pass(int prec, uint64_t n, uint64_t n1, uint64_t n2, uint64_t O) {
     static const char* pn[] = {"f32", "f64", "df64"};
     const std::string forced = tuning_lookup(std::string("split|") + pn[prec] + "|n" + std::to_string(n));
     if (forced == "2") return false;
     if (forced == "3") return true;
+    if (prec == P_F32 && std::max(n1, n2) <= 2048 && n * O >= (1ull << 23)) return false;
     const size_t es = prec == P_F32 ? 8 : 16;
     return std::max(n1, n2) * es > 32768 || n >= (1u << 19);
 }
@@ -1390,7 +1398,7 @@ struct Builder {
         // Batched plans keep two passes (measured: no gain in the batched
         // kernel, which already runs many transforms side by side).
         uint64_t t3 = 0, t1 = 0, t2 = 0;
-        if (S == 1 && !P.batch_mode && prefer_three_pass(P.prec, n, n1, n2) &&
+        if (S == 1 && !P.batch_mode && prefer_three_pass(P.prec, n, n1, n2, O) &&
             best_split3(P.prec, n, P.lds_budget, &t3, &t1, &t2)) {
             const size_t mark = P.ops.size();
             if (three_pass(n, O, src, src_off, dst, dst_off, ws_off, conj_in, conj_out, scale, t3, t1, t2, ws_used))
