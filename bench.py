@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="default workload: skip the serving summary")
     ap.add_argument("--transforms", type=int, default=2000, help="batched workload: transform count")
     ap.add_argument("--nbuf", type=int, default=8)
+    ap.add_argument("--configs", default=None,
+                    help="reference workload: comma-separated configuration names to run (default all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tuning", default="table", choices=["table", "auto", "none"],
                     help="kernel choices: the committed MI355X tuning table (wgpufft_amd/tuning/mi355x.txt, "
@@ -359,8 +361,20 @@ def run_reference_configs(args):
     clear_tuning()
     if args.tuning == "table" and os.path.exists(DEFAULT_TABLE):
         load_tuning(DEFAULT_TABLE)
+    if args.calibrate:
+        # counter calibration for profiles: a plain copy of exactly 64 MiB
+        src = torch.empty(64 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        fill_uniform_f32(src, 99)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     rows = []
+    only = set(args.configs.split(",")) if args.configs else None
     for name, shape, batch, ref_ms, src in REFERENCE_CONFIGS:
+        if only is not None and name not in only:
+            continue
         fwd = FftPlan.c2c(FftConfig.new_nd(shape).with_batch(batch))
         inv = FftPlan.c2c(FftConfig.inverse_nd(shape).with_batch(batch))
         nbytes = fwd.required_buffer_size_bytes()

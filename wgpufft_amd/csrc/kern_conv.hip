@@ -4,8 +4,11 @@
 
 namespace mifft {
 
-template <class Fmt, bool MED = false>
-__global__ void __launch_bounds__(256) k_rader(ConvArgs c) {
+// NT threads per workgroup: 256, or 512 where a tile's LDS leaves room for
+// one workgroup per CU only (the tile's loops stride by blockDim.x, so a
+// line's arithmetic does not depend on NT).
+template <class Fmt, bool MED = false, int NT = 256>
+__global__ void __launch_bounds__(NT) k_rader(ConvArgs c) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     rader_tile<Fmt, MED>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
 }
@@ -22,20 +25,25 @@ size_t conv_kernel_bytes(int prec, uint32_t m, uint32_t lpw) {
 }
 
 template <class Fmt, class K>
-static hipError_t launch_conv(K kern, int prec, const ConvArgs& c, hipStream_t s) {
+static hipError_t launch_conv(K kern, int prec, const ConvArgs& c, hipStream_t s, uint32_t threads = 256) {
     const size_t bytes = conv_kernel_bytes(prec, c.m, c.sc.lpw);
     hipError_t e = ensure_lds_attr((const void*)kern, bytes);
     if (e != hipSuccess) return e;
     const uint32_t blocks = (c.io.lines + c.sc.lpw - 1) / c.sc.lpw;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), bytes, s, c);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), bytes, s, c);
     return hipGetLastError();
 }
 
 hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s) {
+    // one workgroup per CU (a tile over half the LDS): twice the threads
+    const bool wide = conv_kernel_bytes(prec, c.m, c.sc.lpw) > kLdsBudget / 2;
     switch (prec) {
         case P_F32:
-            return sched_has_medium_prime(c.sc) ? launch_conv<FmtF32>(k_rader<FmtF32, true>, prec, c, s)
-                                                : launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
+            if (sched_has_medium_prime(c.sc))
+                return wide ? launch_conv<FmtF32>(k_rader<FmtF32, true, 512>, prec, c, s, 512)
+                            : launch_conv<FmtF32>(k_rader<FmtF32, true>, prec, c, s);
+            return wide ? launch_conv<FmtF32>(k_rader<FmtF32, false, 512>, prec, c, s, 512)
+                        : launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
         case P_F64: return launch_conv<FmtF64>(k_rader<FmtF64>, prec, c, s);
         default: return launch_conv<FmtDF64>(k_rader<FmtDF64>, prec, c, s);
     }
