@@ -13,8 +13,8 @@ __global__ void __launch_bounds__(NT) k_rader(ConvArgs c) {
     rader_tile<Fmt, MED>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
 }
 
-template <class Fmt>
-__global__ void __launch_bounds__(256) k_bluestein(ConvArgs c) {
+template <class Fmt, int NT = 256>
+__global__ void __launch_bounds__(NT) k_bluestein(ConvArgs c) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     bluestein_tile<Fmt>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
 }
@@ -44,16 +44,27 @@ hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s) {
                             : launch_conv<FmtF32>(k_rader<FmtF32, true>, prec, c, s);
             return wide ? launch_conv<FmtF32>(k_rader<FmtF32, false, 512>, prec, c, s, 512)
                         : launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
-        case P_F64: return launch_conv<FmtF64>(k_rader<FmtF64>, prec, c, s);
-        default: return launch_conv<FmtDF64>(k_rader<FmtDF64>, prec, c, s);
+        case P_F64:
+            return wide ? launch_conv<FmtF64>(k_rader<FmtF64, false, 512>, prec, c, s, 512)
+                        : launch_conv<FmtF64>(k_rader<FmtF64>, prec, c, s);
+        default:
+            return wide ? launch_conv<FmtDF64>(k_rader<FmtDF64, false, 512>, prec, c, s, 512)
+                        : launch_conv<FmtDF64>(k_rader<FmtDF64>, prec, c, s);
     }
 }
 
 hipError_t launch_bluestein(int prec, const ConvArgs& c, hipStream_t s) {
+    const bool wide = conv_kernel_bytes(prec, c.m, c.sc.lpw) > kLdsBudget / 2;
     switch (prec) {
-        case P_F32: return launch_conv<FmtF32>(k_bluestein<FmtF32>, prec, c, s);
-        case P_F64: return launch_conv<FmtF64>(k_bluestein<FmtF64>, prec, c, s);
-        default: return launch_conv<FmtDF64>(k_bluestein<FmtDF64>, prec, c, s);
+        case P_F32:
+            return wide ? launch_conv<FmtF32>(k_bluestein<FmtF32, 512>, prec, c, s, 512)
+                        : launch_conv<FmtF32>(k_bluestein<FmtF32>, prec, c, s);
+        case P_F64:
+            return wide ? launch_conv<FmtF64>(k_bluestein<FmtF64, 512>, prec, c, s, 512)
+                        : launch_conv<FmtF64>(k_bluestein<FmtF64>, prec, c, s);
+        default:
+            return wide ? launch_conv<FmtDF64>(k_bluestein<FmtDF64, 512>, prec, c, s, 512)
+                        : launch_conv<FmtDF64>(k_bluestein<FmtDF64>, prec, c, s);
     }
 }
 
