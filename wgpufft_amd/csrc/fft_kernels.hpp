@@ -992,8 +992,10 @@ __device__ void lds_fft_tile(const LineArgs& a, const Sched& sc, unsigned char* 
     }
 }
 
-template <class Fmt, int LDM, int STM, bool POST>
-__global__ void __launch_bounds__(256) lds_fft_kernel(LineArgs a, Sched sc) {
+// NT threads per workgroup: 256, or 512 where a tile's LDS leaves room for
+// one workgroup per CU only (the tile's loops stride by blockDim.x).
+template <class Fmt, int LDM, int STM, bool POST, int NT = 256>
+__global__ void __launch_bounds__(NT) lds_fft_kernel(LineArgs a, Sched sc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
     lds_fft_tile<Fmt, LDM, STM, POST>(a, sc, smem_raw, bid * sc.lpw);

@@ -401,25 +401,31 @@ size_t lds_kernel_bytes(int prec, uint32_t n, uint32_t lpw) {
 template <class Fmt>
 static hipError_t launch_lds_fmt(int ldm, int stm, int post, const LineArgs& a, const Sched& sc,
                                  hipStream_t s) {
+    const size_t bytes = lds_kernel_bytes(Fmt::kId == 0 ? P_F32 : P_F64, sc.n, sc.lpw);
+    // one workgroup per CU (a tile over half the LDS): twice the threads
+    const bool wide = bytes > kLdsBudget / 2;
     const void* fn = nullptr;
     if (ldm == LD_COMPLEX && stm == ST_COMPLEX && !post)
-        fn = (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, false>;
+        fn = wide ? (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, false, 512>
+                  : (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, false>;
     else if (ldm == LD_COMPLEX && stm == ST_COMPLEX && post)
-        fn = (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, true>;
+        fn = wide ? (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, true, 512>
+                  : (const void*)&lds_fft_kernel<Fmt, LD_COMPLEX, ST_COMPLEX, true>;
     else if (ldm == LD_REAL && stm == ST_PACKED && !post)
-        fn = (const void*)&lds_fft_kernel<Fmt, LD_REAL, ST_PACKED, false>;
+        fn = wide ? (const void*)&lds_fft_kernel<Fmt, LD_REAL, ST_PACKED, false, 512>
+                  : (const void*)&lds_fft_kernel<Fmt, LD_REAL, ST_PACKED, false>;
     else if (ldm == LD_HERMITIAN && stm == ST_REAL && !post)
-        fn = (const void*)&lds_fft_kernel<Fmt, LD_HERMITIAN, ST_REAL, false>;
+        fn = wide ? (const void*)&lds_fft_kernel<Fmt, LD_HERMITIAN, ST_REAL, false, 512>
+                  : (const void*)&lds_fft_kernel<Fmt, LD_HERMITIAN, ST_REAL, false>;
     else
         return hipErrorInvalidValue;
-    const size_t bytes = lds_kernel_bytes(Fmt::kId == 0 ? P_F32 : P_F64, sc.n, sc.lpw);
     hipError_t e = ensure_lds_attr(fn, bytes);
     if (e != hipSuccess) return e;
     const uint32_t blocks = (a.lines + sc.lpw - 1) / sc.lpw;
     LineArgs ca = a;
     Sched cs = sc;
     void* args[] = {&ca, &cs};
-    return hipLaunchKernel(fn, dim3(blocks), dim3(256), args, bytes, s);
+    return hipLaunchKernel(fn, dim3(blocks), dim3(wide ? 512 : 256), args, bytes, s);
 }
 
 hipError_t launch_lds(int prec, int ldm, int stm, int post, const LineArgs& a, const Sched& sc,
