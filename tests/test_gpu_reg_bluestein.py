@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from tests.helpers import c2c_expected, empty_dev, from_dev, seeded_complex, to_dev
-from wgpufft_amd import FftConfig, FftDirection, FftPlan, FftTuning, Normalization
+from wgpufft_amd import FftBatch, FftConfig, FftDirection, FftPlan, FftTuning, Normalization
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +110,28 @@ def test_register_bluestein_strided_axes(torch_cuda, shape, axes, batch, inverse
     e = oracle.rel_l2(y, ref)
     print(f"{shape} axes={axes} b={batch} inverse={inverse} rel_l2={e:.3e} [{plan.describe()}]", flush=True)
     assert e <= 1e-6
+
+
+def test_register_bluestein_batched_bit_exact(torch_cuda):
+    # the persistent batched launch runs the 256-thread kernels; standalone
+    # plans of M = 8192 run the 512-thread form of the same schedule:
+    # results bit for bit the same
+    import torch
+    items = []
+    for t, (n, b) in enumerate([(3256, 2), (4093, 1), (1229, 3), (34, 9)]):
+        x = seeded_complex(950 + t, n * b)
+        items.append(dict(n=n, batch=b, x=x, input=to_dev(x), output=empty_dev(x.nbytes),
+                          direction=FftDirection.Forward, normalization=Normalization.Inverse,
+                          algorithm=2 if n == 4093 else 0, in_place=False))
+    batch = FftBatch(items)
+    batch.execute()
+    torch.cuda.synchronize()
+    for it in items:
+        got = from_dev(it["output"], np.complex64)
+        cfg = FftConfig.new(it["n"]).with_batch(it["batch"])
+        if it["algorithm"] == 2:
+            cfg = cfg.with_tuning(FftTuning().with_force_bluestein_axes([0]))
+        plan, want = run(cfg, it["x"])
+        assert "bluestein-reg" in plan.describe(), plan.describe()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), it["n"]
+        assert oracle.rel_l2(got, c2c_expected(it["x"], [it["n"]], batch=it["batch"])) <= 1e-6

@@ -9,6 +9,8 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes);
 #define REGB(K, RADSTR) {K::kM, K::kTPL, K::kLines, K::NT, K::lds_bytes(), (const void*)&reg_bluestein_kernel<K>, RADSTR}
 
 static const RegConvVariant kRegBluestein[] = {
+    {RegB8192W::kM, RegB8192W::kTPL, RegB8192W::kLines, RegB8192W::NT, RegB8192W::lds_bytes(),
+     (const void*)&reg_bluestein_kernel_mb<RegB8192W, 4>, "8x16x8x8"},
     REGB(RegB64, "8x8"),
     REGB(RegB128, "8x2x8"),
     REGB(RegB256, "16x16"),

@@ -155,9 +155,24 @@ using RegB1024 = RegBluestein<1024, 64, 4, RadixList<16, 4, 16>>;
 using RegB2048 = RegBluestein<2048, 128, 2, RadixList<16, 8, 16>>;
 using RegB4096 = RegBluestein<4096, 256, 1, RadixList<16, 16, 16>>;
 using RegB8192 = RegBluestein<8192, 256, 1, RadixList<8, 16, 8, 8>>;
+// 512-thread form of the longest (16 points per thread, 128 registers: two
+// workgroups per CU where the 256-thread form's 256 registers allow one) for
+// the standalone kernel; every butterfly's arithmetic is that of the
+// 256-thread form, which the batched kernel runs (N = 3256 over 1 GiB 7.00
+// -> 3.71 ms per pair).  A 512-thread 4096-point form (8x8x8x8) measured
+// 5-12 % slower than RegB4096.
+using RegB8192W = RegBluestein<8192, 512, 1, RadixList<8, 16, 8, 8>>;
 
 template <class K>
 __global__ void __launch_bounds__(K::NT) reg_bluestein_kernel(ConvArgs c) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    K::run(c, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
+}
+
+// The same with at least MINB waves per SIMD (the register budget that lets
+// two 512-thread workgroups share a CU).
+template <class K, int MINB>
+__global__ void __launch_bounds__(K::NT, MINB) reg_bluestein_kernel_mb(ConvArgs c) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     K::run(c, reinterpret_cast<cpx<typename K::T>*>(smem_raw));
 }
