@@ -12,6 +12,16 @@ __global__ void __launch_bounds__(NT) k_rader(ConvArgs c) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     rader_tile<Fmt, MED>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
 }
+// With at least MINB waves per SIMD: the medium-prime tiles that share a CU
+// with a second one run 512 threads at 128 registers (four waves per SIMD;
+// 256 threads at 175 registers left two; the stages of the largest primes
+// spill to scratch at this budget and still gain): N = 4093 over 1 GiB 5.27
+// -> 4.38 ms per pair, N = 4241 (radix 53) 0.80 -> 0.67 ms.
+template <class Fmt, bool MED, int NT, int MINB>
+__global__ void __launch_bounds__(NT, MINB) k_rader_mb(ConvArgs c) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    rader_tile<Fmt, MED>(c, smem_raw, xcd_remap(blockIdx.x, gridDim.x) * c.sc.lpw);
+}
 
 template <class Fmt, int NT = 256>
 __global__ void __launch_bounds__(NT) k_bluestein(ConvArgs c) {
@@ -41,7 +51,7 @@ hipError_t launch_rader(int prec, const ConvArgs& c, hipStream_t s) {
         case P_F32:
             if (sched_has_medium_prime(c.sc))
                 return wide ? launch_conv<FmtF32>(k_rader<FmtF32, true, 512>, prec, c, s, 512)
-                            : launch_conv<FmtF32>(k_rader<FmtF32, true>, prec, c, s);
+                            : launch_conv<FmtF32>(k_rader_mb<FmtF32, true, 512, 4>, prec, c, s, 512);
             return wide ? launch_conv<FmtF32>(k_rader<FmtF32, false, 512>, prec, c, s, 512)
                         : launch_conv<FmtF32>(k_rader<FmtF32>, prec, c, s);
         case P_F64:
