@@ -82,13 +82,14 @@ hipError_t ensure_lds_attr(const void* fn, size_t bytes) {
      (const void*)&reg_fft_kernel<MIFFT_KF(FMT, N, TPL, F, LDM, STM, false, __VA_ARGS__)>, RADSTR, 0, F}
 #define REGF(FMT, PREC, N, TPL, F, RADSTR, ...) \
     REGFM(FMT, PREC, N, TPL, F, LD_COMPLEX, ST_COMPLEX, RADSTR, __VA_ARGS__)
-#define REGFP(FMT, PREC, N, TPL, F, MINB, RADSTR, ...)                                                \
-    {PREC, N, TPL, F, 0, 0, LD_COMPLEX, ST_COMPLEX, 0, F,                                          \
-     MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__)::NT,                      \
-     MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__)::lds_bytes(),             \
-     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, F, LD_COMPLEX, ST_COMPLEX, true, __VA_ARGS__), \
-                                       MINB>,                                                      \
+#define REGFMP(FMT, PREC, N, TPL, F, LDM, STM, MINB, RADSTR, ...)                                  \
+    {PREC, N, TPL, F, 0, 0, LDM, STM, 0, F,                                                        \
+     MIFFT_KF(FMT, N, TPL, F, LDM, STM, true, __VA_ARGS__)::NT,                                    \
+     MIFFT_KF(FMT, N, TPL, F, LDM, STM, true, __VA_ARGS__)::lds_bytes(),                           \
+     (const void*)&reg_fft_pipe_kernel<MIFFT_KF(FMT, N, TPL, F, LDM, STM, true, __VA_ARGS__), MINB>, \
      RADSTR, MINB, F}
+#define REGFP(FMT, PREC, N, TPL, F, MINB, RADSTR, ...) \
+    REGFMP(FMT, PREC, N, TPL, F, LD_COMPLEX, ST_COMPLEX, MINB, RADSTR, __VA_ARGS__)
 
 // Adjacent-butterfly forms (16-byte global accesses) of contiguous complex
 // lines: REGVA one line per thread group, REGFA the fold.
