@@ -56,11 +56,11 @@ __device__ __forceinline__ void rader_tile(const ConvArgs& c, unsigned char* sme
     cpx<T>* bufB = bufA + lpw * lstr;
     cpx<T>* x0 = bufB + lpw * lstr;  // lpw entries
     cpx<T>* s0 = x0 + lpw;           // lpw entries
-    // Long lines: the line in natural order into bufB (coalesced loads,
-    // issued together with the permutation's), then the gather a[q] =
-    // x[g^q] (zero-padded to M) and x0 = x[0] from LDS.  Short lines (a few
-    // cache lines each) gather straight from memory.  Lines past the end
-    // read the last line and are never stored.
+    // Long lines: the line in natural order into bufB (coalesced loads),
+    // then the gather a[q] = x[g^q] (zero-padded to M) and x0 = x[0] from
+    // LDS, so that no global load waits on the permutation table's.  Short
+    // lines (a few cache lines each) gather straight from memory.  Lines
+    // past the end read the last line and are never stored.
     const uint32_t total = lpw * M;
     const uint32_t tpl = slot_tpl(lpw), slots = tpl * lpw;
     const bool staged = p > kRaderStagedMin;
