@@ -47,6 +47,8 @@ CASES = [
     ("613 inverse batch 3", [613], None, 3, True, Normalization.Inverse, "17x"),
     ("64x613 axis 1 batch 16", [64, 613], [1], 16, False, Normalization.NONE, "17x"),
     ("1381 inverse batch 2", [1381], None, 2, True, Normalization.Inverse, "23x"),
+    ("1381 batch 1100", [1381], None, 1100, False, Normalization.Inverse, "23x"),  # two lines per tile
+    ("1999 inverse batch 1030", [1999], None, 1030, True, Normalization.Inverse, "37x"),
     ("4093", [4093], None, 1, False, Normalization.Inverse, "31x"),
     ("4241 batch 2", [4241], None, 2, False, Normalization.Inverse, "53x16x5"),
     ("5003 inverse", [5003], None, 1, True, Normalization.Inverse, "61x41x2"),
@@ -65,6 +67,8 @@ def test_medium_prime_rader_matches_oracle(torch_cuda, label, shape, axes, batch
     x = seeded_complex(4100 + p + batch, int(np.prod(shape)) * batch)
     plan, y = run(cfg, x)
     assert f"rader p={p} M={p - 1} {sched}" in plan.describe(), plan.describe()
+    if batch >= 1024 and axes is None and 1024 <= p - 1 <= 2400:
+        assert "lpw=2" in plan.describe(), plan.describe()
     ref = c2c_expected(x, shape, axes=axes, batch=batch, inverse=inverse, norm=int(norm))
     e = check(y, ref, label)
     print(f"{label}: rel_l2={e:.3e} [{plan.describe()}]", flush=True)

@@ -1109,9 +1109,19 @@ struct Builder {
     bool medium_rader(uint64_t p, const LineSet& ls, const Layout& lay, std::vector<int>* radices,
                       uint32_t* lpw) {
         const uint64_t linear = 2 * (p - 1) - 1;
-        return P.prec == P_F32 && medium_rader_schedule(p, radices) &&
-               std::max<uint64_t>(linear, p - 1) >= P.cfg.tuning.fused_min_convolution_length &&
-               conv_fits(p - 1, lpw, adjacent_lines(ls, lay), ls.lines);
+        if (!(P.prec == P_F32 && medium_rader_schedule(p, radices) &&
+              std::max<uint64_t>(linear, p - 1) >= P.cfg.tuning.fused_min_convolution_length &&
+              conv_fits(p - 1, lpw, adjacent_lines(ls, lay), ls.lines)))
+            return false;
+        // Contiguous lines of 1024 to about 2400 points: two per tile where
+        // two tiles still share a CU (the 512-thread medium-prime kernel
+        // leaves most threads idle in the largest prime's stage of one
+        // line: N = 1381 over 128 MiB 1.21 -> 0.72 ms per pair; N = 613,
+        // three lines per tile, measured 12 % slower at six).
+        if (*lpw == 1 && p - 1 >= 1024 && !adjacent_lines(ls, lay) && ls.lines >= 4ull * device_cu_count() &&
+            conv_kernel_bytes(P.arith, (uint32_t)(p - 1), 2) <= kLdsBudget / 2)
+            *lpw = 2;
+        return true;
     }
     bool direct_prime_op(uint64_t p, const LineSet& ls, const Layout& lay, int src, uint64_t src_off, int dst,
                          uint64_t dst_off, bool inverse, double scale) {
