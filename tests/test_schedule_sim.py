@@ -12,7 +12,9 @@ exponent in that map fails here without a GPU:
   decomposition i = g % S1, j = (g / S1) % S2, o = g / (S1 S2) and the planner's
   strides (plan.cpp four_step / three_pass);
 * the twiddle recurrence of a butterfly's outputs (PostSeq);
-* the direct prime DFT's output pairs (k_direct_prime);
+* the direct prime DFT's output pairs (k_direct_prime), and the medium-prime
+  stages of Rader convolutions (lds_stage_prime: symmetric pairs, radices
+  17..61, inside the Stockham schedules of 613 ... 6841);
 * the packed R2C store and the Hermitian-extended C2R load;
 * the XCD tile remap (a bijection for every grid size);
 * the small-volume gather order (k_small_volume).
@@ -242,6 +244,60 @@ def test_post_twiddle_recurrence():
         for m in range(R):
             assert abs(w - W(P, j * (p0 + m * L))) < 4 * R * 2.2e-16
             w = w * step
+
+
+def prime_butterfly(x):
+    """lds_stage_prime's medium-prime DFT: inputs m and R - m as their sum and
+    difference, X_0 = x_0 + sum ap, X_q = A - iB and X_{R-q} = A + iB with
+    A = x_0 + sum_m cos(2 pi mq/R) ap_m, B = sum_m sin(2 pi mq/R) am_m."""
+    R = len(x)
+    h = (R - 1) // 2
+    ap = [x[m] + x[R - m] for m in range(1, h + 1)]
+    am = [x[m] - x[R - m] for m in range(1, h + 1)]
+    y = np.empty(R, complex)
+    y[0] = x[0] + sum(ap)
+    for q in range(1, h + 1):
+        A, B = x[0], 0
+        for m in range(1, h + 1):
+            e = (m * q) % R
+            A = A + np.cos(2 * np.pi * e / R) * ap[m - 1]
+            B = B + np.sin(2 * np.pi * e / R) * am[m - 1]
+        y[q] = A - 1j * B
+        y[R - q] = A + 1j * B
+    return y
+
+
+# Schedules of the medium-prime Rader convolutions (p - 1 points, the medium
+# primes first): 613, 1381, 1999, 4093, 4241, 5003, 6841
+@pytest.mark.parametrize("rad", [[17, 4, 9], [23, 4, 15], [37, 6, 9], [31, 4, 3, 11], [53, 16, 5], [61, 41, 2],
+                                 [19, 8, 9, 5]])
+def test_medium_prime_stages(rad):
+    N = int(np.prod(rad))
+    x0 = crand(N)
+    x = x0.copy()
+    tab = stage_table(rad)
+    L = 1
+    for s, R in enumerate(rad):
+        nr = N // R
+        y = np.empty_like(x)
+        for j in range(nr):
+            k = j % L
+            w = np.array([x[j + m * nr] for m in range(R)])
+            if L > 1:
+                for m in range(1, R):
+                    w[m] *= tab[twoff(rad, s) + (m - 1) * L + k]
+            w = prime_butterfly(w) if R > 16 else np.fft.fft(w)
+            for m in range(R):
+                y[(j - k) * R + k + m * L] = w[m]
+        x, L = y, L * R
+    ref = np.fft.fft(x0)
+    assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("R", [17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61])
+def test_prime_butterfly(R):
+    x = crand(R)
+    assert np.abs(prime_butterfly(x) - np.fft.fft(x)).max() <= 1e-9 * np.abs(np.fft.fft(x)).max()
 
 
 @pytest.mark.parametrize("p", [47, 59, 83, 103, 107])
