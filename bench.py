@@ -230,7 +230,7 @@ def main():
 
     # Correctness of what is timed (outside the timed region): the forward
     # transform of buffer 0 against torch.fft in complex128 on the device (an
-    # FP64 checker independent of this library), and for pairs the round trip
+    # FP64 reference independent of this library), and for pairs the round trip
     # against the input.
     fwd_err, rt_err = forward_check(spec, plans, bufs, aux, stream)
     for i in range(args.warmup):

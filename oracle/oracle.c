@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product path (the HIP library,
  * the C ABI, the Python mirror of FftConfig/FftPlan) links, loads or calls
  * this file.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
- * bench.py use it, and only as the checker.
+ * bench.py use it, and only as the reference results are compared against.
  *
  * What it restates (file:line in the wgpuFFT sources):
  *   oracle_fft1d            src/math/mod.rs:274-364  fft_f64 / fft_f64_recursive

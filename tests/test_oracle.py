@@ -1,4 +1,4 @@
-"""Pins the FP64 oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the FP64 oracle (oracle/oracle.c) before the tests trust it.
 
 Each oracle function is checked against something it was not derived from:
 the wgpuFFT known-answer tests (tests/golden/kat.json), mpmath DFTs at 40
