@@ -1,7 +1,8 @@
 """FP64 oracle for the FFT hot path -- TEST INFRASTRUCTURE ONLY.
 
-This package is the reference the tests compare against.  Only ``tests/``, ``__graft_entry__.smoke()`` and
-the ``cpu_baseline`` leg of ``bench.py`` import it.  The product
+This package is the reference the tests compare against.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+import it.  The product
 (``wgpufft_amd`` and ``libmifft.so``) never imports, links or calls it.
 
 It wraps ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by
