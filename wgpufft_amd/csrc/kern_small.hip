@@ -62,40 +62,26 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
         // from memory in every iteration
         const uint32_t n = sc.n, S = v.S[ax], lstr = lds_line_stride(n), lines = v.elems / n;
         const DivMagic dn = sc.dn, dL = v.dL[ax], dS = v.dS[ax];
-        // Point p of line l = (i, o), i < S, sits at o S n + i + p S.  Adjacent
-        // threads take adjacent points of axis 0 but adjacent lines (adjacent
-        // i) of the other axes, so that they read adjacent volume words: the
-        // points of one line are S words apart, the same LDS bank for S a
-        // multiple of 32.
-        for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
-            uint32_t l, p;
-            if (S == 1) {
-                l = fdiv(t, dn);
-                p = t - l * n;
-            } else {
-                p = fdiv(t, dL);
-                l = t - p * lines;
-            }
-            const uint32_t o = fdiv(l, dS), i = l - o * S;
-            bufA[l * lstr + lpad(p)] = vol[o * S * n + i + p * S];
-        }
-        __syncthreads();
+        const bool direct = sc.ns == 0 && n > 1;
         const cpx<float>* res;
-        if (sc.ns == 0 && n > 1) {
+        if (direct) {
             // a prime axis up to direct_max_prime: the direct DFT of each line,
-            // X[k] = sum_p x[p] W_n^{pk}, one output per thread, W_n^t (t < n)
-            // from LDS (small_volume.rs:220-245)
+            // X[k] = sum_p x[p] W_n^{pk}, one output per thread, read straight
+            // from the volume (point p of line l = (i, o), i < S, at
+            // o S n + i + p S; the threads of one line read the same words)
+            // with W_n^t (t < n) from LDS (small_volume.rs:220-245)
             const cpx<float>* wt = twl + two;
             for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
                 const uint32_t l = fdiv(t, dn), k = t - l * n;
-                const cpx<float>* ln = bufA + l * lstr;
+                const uint32_t o = fdiv(l, dS), i = l - o * S;
+                const cpx<float>* ln = vol + o * S * n + i;
                 cpx<float> acc = mk<float>(0.f, 0.f);
                 uint32_t e = 0;
                 // unrolled so that the LDS reads of several terms are in
                 // flight together (the sum's order is unchanged)
 #pragma unroll 4
                 for (uint32_t q = 0; q < n; q++) {
-                    acc = acc + cmul(ln[lpad(q)], wt[e]);
+                    acc = acc + cmul(ln[q * S], wt[e]);
                     e += k;
                     e = e >= n ? e - n : e;
                 }
@@ -104,6 +90,24 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
             __syncthreads();
             res = bufB;
         } else {
+            // Point p of line l = (i, o), i < S, sits at o S n + i + p S.
+            // Adjacent threads take adjacent points of axis 0 but adjacent
+            // lines (adjacent i) of the other axes, so that they read
+            // adjacent volume words: the points of one line are S words
+            // apart, the same LDS bank for S a multiple of 32.
+            for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
+                uint32_t l, p;
+                if (S == 1) {
+                    l = fdiv(t, dn);
+                    p = t - l * n;
+                } else {
+                    p = fdiv(t, dL);
+                    l = t - p * lines;
+                }
+                const uint32_t o = fdiv(l, dS), i = l - o * S;
+                bufA[l * lstr + lpad(p)] = vol[o * S * n + i + p * S];
+            }
+            __syncthreads();
             res = lds_fft_lines<float>(bufA, bufB, lines, sc, twl + two);
         }
         two += v.tw_len[ax];
