@@ -111,6 +111,10 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
             res = lds_fft_lines<float>(bufA, bufB, lines, sc, twl + two);
         }
         two += v.tw_len[ax];
+        // back into the volume, or after the last axis straight out to
+        // memory (adjacent threads write adjacent volume words either way)
+        const bool last = ax + 1 == v.rank;
+        const float sre = v.scale, sim = v.conj_out ? -v.scale : v.scale;
         for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x) {
             uint32_t l, p;
             if (S == 1) {
@@ -121,13 +125,14 @@ __global__ void __launch_bounds__(1024) k_small_volume(SmallVolArgs v) {
                 l = t - p * lines;
             }
             const uint32_t o = fdiv(l, dS), i = l - o * S;
-            vol[o * S * n + i + p * S] = res[l * lstr + lpad(p)];
+            const cpx<float> r = res[l * lstr + lpad(p)];
+            if (last)
+                FmtF32::st(v.out, base + o * S * n + i + p * S, mk<float>(r.x * sre, r.y * sim));
+            else
+                vol[o * S * n + i + p * S] = r;
         }
-        __syncthreads();
+        if (!last) __syncthreads();
     }
-    const float sre = v.scale, sim = v.conj_out ? -v.scale : v.scale;
-    for (uint32_t t = threadIdx.x; t < v.elems; t += blockDim.x)
-        FmtF32::st(v.out, base + t, mk<float>(vol[t].x * sre, vol[t].y * sim));
 }
 
 size_t small_volume_lds_bytes(const SmallVolArgs& v) {
