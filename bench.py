@@ -11,10 +11,12 @@ passes) = 536,870,912 bytes; GFLOP/s counts 5*N*log2(N) per FFT.
 The inputs rotate over 8 independent 64 MiB buffers (512 MiB > the 256 MB
 MALL), so every pass streams from HBM; `value` is that HBM-resident rate.  The
 single-buffer (cache-resident, as the reference's README figure) rate is
-reported beside it.  The roofline entry times each kernel of the dominant
-pass live with HIP events on the stream it is launched on.  The CPU baseline
-times the FP64 oracle (oracle/oracle.c, a restatement of the reference's
-CPU/host transforms) on the host cores.
+reported beside it.  The roofline entry times every launch of one step live,
+inside the step sequence, with HIP events on the stream it is launched on
+(events that skip the system-scope cache fence; default events' figures
+beside them).  The CPU baseline times the host plans (mifft_cpu_*, the
+reference's CpuFftPlan restated in C++) on the same workload on the host
+cores.
 
 Prints one JSON line.
 """
@@ -256,7 +258,8 @@ def main():
                 "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom["desc"], "kernel_us": round(dom["us"], 2),
                 "alg_bytes_per_launch": dom["alg_bytes"],
-                "timing": "HIP events around each launch inside the timed step sequence",
+                "kernel_us_default_events": dom["us_default_events"],
+                "timing": f"HIP events ({dom['timing']}) around each launch inside the timed step sequence",
                 "step_counter_bytes": pair_bytes, "step_counter_source": pair_src}
 
     cpu = None
@@ -644,26 +647,118 @@ def forward_check(spec, plans, bufs, aux, stream):
     return fwd, rt
 
 
+HIP_EVENT_DISABLE_SYSTEM_FENCE = 0x20000000  # hip_runtime_api.h
+
+
+class _FenceFreeEvents:
+    """Timing events created with hipEventDisableSystemFence through the HIP
+    runtime torch loaded (the same runtime libmifft.so runs on).  A default
+    event performs a system-scope release when it is recorded: a writeback and
+    invalidation of the caches that adds a few microseconds to every launch it
+    brackets and makes the next launch start on cold caches.  These events take
+    only the timestamp.  Any refusal by the runtime raises RuntimeError."""
+
+    def __init__(self, count):
+        import ctypes
+
+        import torch
+        self.ct = ctypes
+        path = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+        hip = ctypes.CDLL(path if os.path.exists(path) else "libamdhip64.so.7")
+        vp = ctypes.c_void_p
+        for name, args in (("hipEventCreateWithFlags", [ctypes.POINTER(vp), ctypes.c_uint]),
+                           ("hipEventRecord", [vp, vp]),
+                           ("hipEventElapsedTime", [ctypes.POINTER(ctypes.c_float), vp, vp]),
+                           ("hipEventDestroy", [vp])):
+            fn = getattr(hip, name)
+            fn.argtypes, fn.restype = args, ctypes.c_int
+        self.hip, self.evs = hip, []
+        for _ in range(count):
+            ev = vp()
+            rc = hip.hipEventCreateWithFlags(ctypes.byref(ev), HIP_EVENT_DISABLE_SYSTEM_FENCE)
+            if rc != 0:
+                self.close()
+                raise RuntimeError(f"hipEventCreateWithFlags(hipEventDisableSystemFence) returned {rc}")
+            self.evs.append(ev)
+
+    def record(self, i, stream):
+        rc = self.hip.hipEventRecord(self.evs[i], self.ct.c_void_p(int(stream.cuda_stream)))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord returned {rc}")
+
+    def elapsed_ms(self, i, j):
+        ms = self.ct.c_float()
+        rc = self.hip.hipEventElapsedTime(self.ct.byref(ms), self.evs[i], self.evs[j])
+        if rc != 0:
+            raise RuntimeError(f"hipEventElapsedTime returned {rc}")
+        return ms.value
+
+    def close(self):
+        for ev in self.evs:
+            self.hip.hipEventDestroy(ev)
+        self.evs = []
+
+
+class _TorchEvents:
+    """Default HIP events (torch.cuda.Event) with the same interface."""
+
+    def __init__(self, count):
+        import torch
+        self.evs = [torch.cuda.Event(enable_timing=True) for _ in range(count)]
+
+    def record(self, i, stream):
+        self.evs[i].record(stream)
+
+    def elapsed_ms(self, i, j):
+        return self.evs[i].elapsed_time(self.evs[j])
+
+    def close(self):
+        self.evs = []
+
+
+def _sequence_durations(spec, plans, launches, bufs, aux, stream, nbuf, reps, events):
+    """Per launch of one step, its average duration in µs over `reps` steps of
+    the timed sequence (all plans in order, rotating buffers), each launch
+    bracketed by events of the given kind on the launch stream."""
+    import torch
+    per = len(launches) + 1
+    ev = events(reps * per)
+    try:
+        torch.cuda.synchronize()
+        for r in range(reps):
+            k = r % nbuf
+            ev.record(r * per, stream)
+            for j, (pi, li) in enumerate(launches):
+                if spec["kind"] == "r2c_c2r":
+                    src, dst = (bufs[k], aux[k]) if pi == 0 else (aux[k], bufs[k])
+                else:
+                    src = dst = bufs[k]
+                plans[pi].execute_launch(li, src, dst, None, stream)
+                ev.record(r * per + j + 1, stream)
+        torch.cuda.synchronize()
+        return [sum(ev.elapsed_ms(r * per + j, r * per + j + 1) for r in range(reps)) / reps * 1e3
+                for j in range(len(launches))]
+    finally:
+        torch.cuda.synchronize()
+        ev.close()
+
+
 def in_sequence_kernel_times(spec, plans, bufs, aux, stream, nbuf, reps):
     """Average duration of every launch of one step as the timed region runs
     it (all plans in order, rotating buffers), each launch bracketed by HIP
     events on the launch stream; with its algorithmic bytes (one read and one
-    write of what the pass moves)."""
-    import torch
+    write of what the pass moves).  `us` comes from events that skip the
+    system-scope fence when the runtime accepts them (`timing` says which),
+    `us_default_events` from default events."""
     launches = [(pi, li) for pi, p in enumerate(plans) for li in range(p.launch_count())]
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(launches) + 1)] for _ in range(reps)]
-    torch.cuda.synchronize()
-    for r in range(reps):
-        k = r % nbuf
-        evs[r][0].record(stream)
-        for j, (pi, li) in enumerate(launches):
-            if spec["kind"] == "r2c_c2r":
-                src, dst = (bufs[k], aux[k]) if pi == 0 else (aux[k], bufs[k])
-            else:
-                src = dst = bufs[k]
-            plans[pi].execute_launch(li, src, dst, None, stream)
-            evs[r][j + 1].record(stream)
-    torch.cuda.synchronize()
+    fenced = _sequence_durations(spec, plans, launches, bufs, aux, stream, nbuf, reps, _TorchEvents)
+    try:
+        times = _sequence_durations(spec, plans, launches, bufs, aux, stream, nbuf, reps, _FenceFreeEvents)
+        timing = "hipEventDisableSystemFence"
+    except (OSError, AttributeError, RuntimeError) as e:
+        print(f"bench: fence-free events unavailable ({e}); timing launches with default events",
+              file=sys.stderr, flush=True)
+        times, timing = fenced, "default"
     es = spec.get("elem_bytes", 8)
     if spec["kind"] == "r2c_c2r":
         n = spec["shape"][0]
@@ -675,8 +770,9 @@ def in_sequence_kernel_times(spec, plans, bufs, aux, stream, nbuf, reps):
         per = {lp: 2 * es * spec["elems"] for lp in launches}
     out = []
     for j, (pi, li) in enumerate(launches):
-        us = sum(evs[r][j].elapsed_time(evs[r][j + 1]) for r in range(reps)) / reps * 1e3
-        k = dict(plan=pi, launch=li, desc=plans[pi].launch_describe(li), us=us, alg_bytes=per.get((pi, li)))
+        us = times[j]
+        k = dict(plan=pi, launch=li, desc=plans[pi].launch_describe(li), us=us,
+                 us_default_events=round(fenced[j], 2), timing=timing, alg_bytes=per.get((pi, li)))
         if k["alg_bytes"]:
             k["gbs"] = k["alg_bytes"] / (us * 1e-6) / 1e9
         out.append(k)
