@@ -101,7 +101,7 @@ def workload_spec(name):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build has no "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the library has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -794,8 +794,9 @@ def profiled_step_traffic(workload):
 def profiled_traffic(desc, prec="f32"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summaries (FETCH_SIZE + WRITE_SIZE, copy-calibrated), matched on precision,
-    length, threads per line, radices and lines per thread; (None, None) when
-    no summary covers it."""
+    length, radices, threads per line, vector width, line-first load / store
+    and the fold, post-twiddle and pipelined forms; (None, None) when no
+    summary covers it."""
     import glob
     import re
     m = re.search(r"N=(\d+) (\S+) .*tpl=(\d+) vec=(\d)", desc)
